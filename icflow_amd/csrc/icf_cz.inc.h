@@ -1,0 +1,424 @@
+// icf_cz.inc.h -- 6-node zero-thickness cohesive zone with the Park-Paulino-Roesler (PPR)
+// potential (included by icf_lib.hip).
+//
+// Restated from cz_kernels9/cz_kernels.h:37-434 (cohesive law, kczCZForce), PPR_math.h:2-57
+// (Tn_, Tt_, Dnn_, Dtt_, Dnt_) and matrix_math.h:15-68 (CZRotationMatrix).
+// Differences of formulation (same mathematics):
+//  - the five PPR functions share their powers: at an opening (Dn, Dt) the law needs
+//    A^{a, a-1, a-2} and Bq^{p, p-1, p-2} for A = 1 - D/delta, Bq = p/a + D/delta, in the
+//    normal and tangential directions; these come from one log and one exp per base instead of
+//    one pow per factor;
+//  - B = [s (x) I3, -s (x) I3] (ModelPrms.cs:324-349), so the 18x18 tangent is
+//    K_ab = sgn_a sgn_b sum_gp s_a,gp s_b,gp Q_gp with Q_gp = (w a_J) R^T T_d R; phase 1
+//    stages the three symmetric Q_gp (18 values), phase 2 forms each block from them.
+#pragma once
+
+namespace icf {
+
+// x^e, x^(e-1), x^(e-2)
+__device__ inline void pow3(double x, double e, double &p0, double &p1, double &p2)
+{
+    if (x > 0.0) {
+        double lx = log(x);
+        p0 = exp(e * lx);
+        double r = 1.0 / x;
+        p1 = p0 * r;
+        p2 = p1 * r;
+    } else {
+        p0 = pow(x, e);
+        p1 = pow(x, e - 1.0);
+        p2 = pow(x, e - 2.0);
+    }
+}
+
+// Powers of one direction of the PPR potential at one opening value.
+struct PprPow {
+    double a0, a1, a2;  // (1 - D/delta)^{a, a-1, a-2}
+    double b0, b1, b2;  // (p/a + D/delta)^{p, p-1, p-2}
+};
+
+__device__ inline PprPow ppr_pow_n(double Dn)
+{
+    PprPow r;
+    if (Dn == 0.0) {
+        r.a0 = r.a1 = r.a2 = 1.0;
+        r.b0 = g_c.bn0[0]; r.b1 = g_c.bn0[1]; r.b2 = g_c.bn0[2];
+        return r;
+    }
+    double d = Dn / g_c.deln;
+    pow3(1.0 - d, g_c.alpha, r.a0, r.a1, r.a2);
+    pow3(g_c.p_m / g_c.alpha + d, g_c.p_m, r.b0, r.b1, r.b2);
+    return r;
+}
+
+__device__ inline PprPow ppr_pow_t(double Dt)
+{
+    PprPow r;
+    if (Dt == 0.0) {
+        r.a0 = r.a1 = r.a2 = 1.0;
+        r.b0 = g_c.bt0[0]; r.b1 = g_c.bt0[1]; r.b2 = g_c.bt0[2];
+        return r;
+    }
+    double d = Dt / g_c.delt;
+    pow3(1.0 - d, g_c.beta, r.a0, r.a1, r.a2);
+    pow3(g_c.p_n / g_c.beta + d, g_c.p_n, r.b0, r.b1, r.b2);
+    return r;
+}
+
+// Factors of the PPR expressions (PPR_math.h):
+//  X = p A^a Bq^(p-1) - a A^(a-1) Bq^p           (the bracket of Tn_ / Tt_)
+//  Y = gam A^a Bq^p + Macaulay                    (the cross-direction factor)
+//  Z = (p^2-p) A^a Bq^(p-2) + (a^2-a) A^(a-2) Bq^p - 2 a p A^(a-1) Bq^(p-1)   (Dnn_/Dtt_)
+__device__ inline double pprX(const PprPow &w, double p, double a) { return p * w.a0 * w.b1 - a * w.a1 * w.b0; }
+__device__ inline double pprZ(const PprPow &w, double p, double a)
+{
+    return (p * p - p) * w.a0 * w.b2 + (a * a - a) * w.a2 * w.b0 - 2.0 * a * p * w.a1 * w.b1;
+}
+__device__ inline double Tn_at(const PprPow &n, const PprPow &t)
+{
+    return (g_c.gam_n / g_c.deln) * pprX(n, g_c.p_m, g_c.alpha) * (g_c.gam_t * t.a0 * t.b0 + g_c.pMtn);
+}
+__device__ inline double Tt_at(const PprPow &n, const PprPow &t)
+{
+    return (g_c.gam_t / g_c.delt) * pprX(t, g_c.p_n, g_c.beta) * (g_c.gam_n * n.a0 * n.b0 + g_c.pMnt);
+}
+__device__ inline double Dnn_at(const PprPow &n, const PprPow &t)
+{
+    return (g_c.gam_n / (g_c.deln * g_c.deln)) * pprZ(n, g_c.p_m, g_c.alpha) * (g_c.gam_t * t.a0 * t.b0 + g_c.pMtn);
+}
+__device__ inline double Dtt_at(const PprPow &n, const PprPow &t)
+{
+    return (g_c.gam_t / (g_c.delt * g_c.delt)) * pprZ(t, g_c.p_n, g_c.beta) * (g_c.gam_n * n.a0 * n.b0 + g_c.pMnt);
+}
+__device__ inline double Dnt_at(const PprPow &n, const PprPow &t)
+{
+    return (g_c.gam_n * g_c.gam_t / (g_c.deln * g_c.delt)) * pprX(n, g_c.p_m, g_c.alpha) * pprX(t, g_c.p_n, g_c.beta);
+}
+
+// PPR law with history, one Gauss point.  Same branches and failure tests as
+// cohesive_law_thermodynamically_consistent (cz_kernels.h:37-174).
+// L = {Tn, Tt, Dnn, Dtt, Dnt}; Dtn == Dnt.
+__device__ inline void ppr_law(bool &contact, bool &failed, double &pmax, double &tmax, double opn, double opt,
+                               double L[5])
+{
+    double Tn = 0, Tt = 0, Dnn = 0, Dtt = 0, Dnt = 0;
+    if (opn > g_c.deln || opt > g_c.delt) {
+        failed = true;
+        L[0] = L[1] = L[2] = L[3] = L[4] = 0;
+        return;
+    }
+    contact = (opn < 0);
+    const double eps = -1e-9;
+    const double thr_t = g_c.f_tt * 0.05, thr_n = g_c.f_tn * 0.05;
+    const PprPow n0 = ppr_pow_n(0.0);
+    if (contact) {
+        const PprPow t_opt = ppr_pow_t(opt);
+        const PprPow t_tmax = ppr_pow_t(tmax);
+        if (pmax != 0) {
+            const PprPow n_pm = ppr_pow_n(pmax);
+            double peak = Tn_at(n_pm, t_tmax);
+            Tn = peak * opn / pmax;
+            Dnn = peak / pmax;
+        } else {
+            Dnn = Dnn_at(n0, t_tmax);
+            Tn = Dnn * opn;
+        }
+        Tt = Tt_at(n0, t_opt);
+        if (Tt >= eps && !(opt > g_c.delt * g_c.rt / 5 && Tt < thr_t)) {
+            if (opt >= tmax) {
+                tmax = opt;
+                Dtt = Dtt_at(n0, t_opt);
+            } else {
+                double peak = Tt_at(n0, t_tmax);
+                Tt = peak * opt / tmax;
+                Dtt = peak / tmax;
+            }
+        } else {
+            Tt = Dtt = Dnt = 0;
+            Tn = Dnn = 0;
+            failed = true;
+        }
+    } else {
+        const PprPow n_opn = ppr_pow_n(opn);
+        const PprPow t_opt = ppr_pow_t(opt);
+        Tt = Tt_at(n_opn, t_opt);
+        Tn = Tn_at(n_opn, t_opt);
+        if (Tt >= eps && Tn >= eps && !(opt > g_c.delt * g_c.rt / 5 && Tt < thr_t) &&
+            !(opn > g_c.deln * g_c.rn / 5 && Tn < thr_n)) {
+            const bool tsoft = (opt >= tmax), nsoft = (opn >= pmax);
+            if (tsoft && nsoft) {
+                tmax = opt;
+                pmax = opn;
+                Dnn = Dnn_at(n_opn, t_opt);
+                Dnt = Dnt_at(n_opn, t_opt);
+                Dtt = Dtt_at(n_opn, t_opt);
+            } else if (tsoft && !nsoft) {
+                const PprPow n_pm = ppr_pow_n(pmax);
+                Dnt = 0;
+                if (pmax != 0) {
+                    double peak = Tn_at(n_pm, ppr_pow_t(tmax));
+                    Tn = peak * opn / pmax;
+                    Dnn = peak / pmax;
+                } else {
+                    Tn = 0;
+                    Dnn = Dnn_at(n0, ppr_pow_t(tmax));
+                }
+                tmax = opt;
+                Tt = Tt_at(n_pm, t_opt);
+                Dtt = Dtt_at(n_pm, t_opt);
+            } else if (!tsoft && nsoft) {
+                const PprPow n_pm = ppr_pow_n(pmax);
+                Dnt = 0;
+                if (tmax != 0) {
+                    double peak = Tt_at(n_pm, ppr_pow_t(tmax));
+                    Tt = peak * opt / tmax;
+                    Dtt = peak / tmax;
+                } else {
+                    Tt = 0;
+                    Dtt = Dtt_at(n_pm, ppr_pow_t(0.0));
+                }
+                pmax = opn;
+                const PprPow t_tm = ppr_pow_t(tmax);
+                Tn = Tn_at(n_opn, t_tm);
+                Dnn = Dnn_at(n_opn, t_tm);
+            } else {
+                const PprPow n_pm = ppr_pow_n(pmax);
+                const PprPow t_tm = ppr_pow_t(tmax);
+                Dnt = 0;
+                double peakN = Tn_at(n_pm, t_tm);
+                if (pmax != 0) {
+                    Tn = peakN * opn / pmax;
+                    Dnn = peakN / pmax;
+                } else {
+                    Tn = 0;
+                    Dnn = Dnn_at(n0, t_tm);
+                }
+                if (tmax != 0) {
+                    double peakT = Tt_at(n_pm, t_tm);
+                    Tt = peakT * opt / tmax;
+                    Dtt = peakT / tmax;
+                } else {
+                    Tt = 0;
+                    Dtt = Dtt_at(n_pm, ppr_pow_t(0.0));
+                }
+            }
+        } else {
+            failed = true;
+            Tn = Tt = Dnn = Dtt = Dnt = 0;
+        }
+    }
+    L[0] = Tn; L[1] = Tt; L[2] = Dnn; L[3] = Dtt; L[4] = Dnt;
+}
+
+// CZ midplane frame (rows e1, e2 = n x e1, e3 = n) and area, rounded like
+// CZRotationMatrix (matrix_math.h:94-147).
+__device__ inline void cz_frame(const double p[9], double R[9], double &area)
+{
+#pragma clang fp contract(off)
+    double a0 = p[3] - p[0], a1 = p[4] - p[1], a2 = p[5] - p[2];
+    double c0 = p[0] - p[6], c1 = p[1] - p[7], c2 = p[2] - p[8];
+    double la = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    R[0] = a0 / la; R[1] = a1 / la; R[2] = a2 / la;
+    double n0 = -a2 * c1 + a1 * c2, n1 = a2 * c0 - a0 * c2, n2 = -a1 * c0 + a0 * c1;
+    double ln = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    area = ln / 2;
+    n0 /= ln; n1 /= ln; n2 /= ln;
+    R[6] = n0; R[7] = n1; R[8] = n2;
+    a0 /= la; a1 /= la; a2 /= la;
+    double r0 = -n2 * a1 + n1 * a2, r1 = n2 * a0 - n0 * a2, r2 = -n1 * a0 + n0 * a1;
+    double lr = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    R[3] = r0 / lr; R[4] = r1 / lr; R[5] = r2 / lr;
+}
+
+// Phase 1 for CZs: law, history, flags and averages written straight to dcz/icz (as the
+// reference does); rhs (18, FP64) and Q (18, FP64 or FP32) staged.  A CZ that is currently
+// failed or fails now stages zeros, so its slots receive +0.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_cz_phase1(double *__restrict__ dcz, int *__restrict__ icz,
+                                                   const double *__restrict__ dn, int n, int cs, int ns,
+                                                   double *__restrict__ st_rhs, void *__restrict__ st_tan)
+{
+    const int u = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    double *orhs = st_rhs + (size_t)u * CZ_RHS_REC;
+    auto write_zero = [&]() {
+#pragma unroll
+        for (int i = 0; i < CZ_RHS_REC; i++) orhs[i] = 0.;
+        if (MIXED) {
+            float *t = (float *)st_tan + (size_t)u * CZ_TAN_REC;
+#pragma unroll
+            for (int i = 0; i < CZ_TAN_REC; i++) t[i] = 0.f;
+        } else {
+            double *t = (double *)st_tan + (size_t)u * CZ_TAN_REC;
+#pragma unroll
+            for (int i = 0; i < CZ_TAN_REC; i++) t[i] = 0.;
+        }
+    };
+    if (icz[u + cs * CZI_FAILED] != 0) {  // cz_kernels.h:219
+        write_zero();
+        return;
+    }
+    double xc[18];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int v = icz[u + cs * (CZI_VRTS + i)];
+#pragma unroll
+        for (int k = 0; k < 3; k++) xc[3 * i + k] = dn[v + ns * (ND_X0 + k)] + dn[v + ns * (ND_UN + k)];
+    }
+    double pmax[3], tmax[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        pmax[i] = dcz[u + cs * (CZD_PMAX + i)];
+        tmax[i] = dcz[u + cs * (CZD_TMAX + i)];
+    }
+    double R[9], area;
+    {
+#pragma clang fp contract(off)
+        double mid[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) mid[i] = (xc[i] + xc[i + 9]) * 0.5;
+        cz_frame(mid, R, area);
+    }
+    // local coordinates and their top-bottom differences (cz_kernels.h:259-290)
+    double dx[9];
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                double top = xc[3 * i] * R[r * 3] + xc[3 * i + 1] * R[r * 3 + 1] + xc[3 * i + 2] * R[r * 3 + 2];
+                double bot = xc[3 * i + 9] * R[r * 3] + xc[3 * i + 10] * R[r * 3 + 1] + xc[3 * i + 11] * R[r * 3 + 2];
+                dx[3 * i + r] = top - bot;
+            }
+    }
+
+    double rhs[18];
+#pragma unroll
+    for (int i = 0; i < 18; i++) rhs[i] = 0;
+    double Q[18];
+    bool anyContact = false, anyFailed = false;
+    double avgDn = 0, avgDt = 0, avgTn = 0, avgTt = 0;
+    const double wa = (1.0 / 3.0) * area;
+
+    for (int gp = 0; gp < 3; gp++) {
+        double dt1, dt2, dnn;
+        {
+#pragma clang fp contract(off)
+            dt1 = 0; dt2 = 0; dnn = 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double s = g_c.sf[i * 3 + gp];
+                dt1 += dx[3 * i + 0] * s;
+                dt2 += dx[3 * i + 1] * s;
+                dnn += dx[3 * i + 2] * s;
+            }
+        }
+        const double opn = dnn, opt = sqrt(dt1 * dt1 + dt2 * dt2);
+        bool gc = false, gf = false;
+        double L[5];
+        ppr_law(gc, gf, pmax[gp], tmax[gp], opn, opt, L);
+        anyContact |= gc;
+        anyFailed |= gf;
+        const double Tn = L[0], Tt = L[1], Dnn = L[2], Dtt = L[3], Dnt = L[4];
+        avgDn += opn / 3; avgDt += opt / 3; avgTn += Tn / 3; avgTt += Tt / 3;
+
+        double T0, T1, T2 = Tn, Td[9];
+        if (opt < 1e-20) {
+            T0 = T1 = 0;
+            Td[0] = Dtt; Td[1] = 0; Td[2] = Dnt;
+            Td[3] = 0; Td[4] = Dtt; Td[5] = Dnt;
+            Td[6] = Dnt; Td[7] = Dnt; Td[8] = Dnn;
+        } else {
+            T0 = Tt * dt1 / opt;
+            T1 = Tt * dt2 / opt;
+            const double o2 = opt * opt, o3 = o2 * opt;
+            const double d00 = dt1 * dt1, d10 = dt2 * dt1, d11 = dt2 * dt2;
+            Td[0] = Dtt * d00 / o2 + Tt * d11 / o3;
+            Td[4] = Dtt * d11 / o2 + Tt * d00 / o3;
+            Td[8] = Dnn;
+            Td[1] = Td[3] = Dtt * d10 / o2 - Tt * d10 / o3;
+            Td[2] = Td[6] = Dnt * dt1 / opt;
+            Td[5] = Td[7] = Dnt * dt2 / opt;
+        }
+        // global traction R^T T, scattered with -sgn_a s_a w a_J (cz_kernels.h:343-364)
+        double Tg[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) Tg[r] = R[r] * T0 + R[3 + r] * T1 + R[6 + r] * T2;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double c = g_c.sf[a * 3 + gp] * wa;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                rhs[3 * a + r] -= c * Tg[r];
+                rhs[9 + 3 * a + r] += c * Tg[r];
+            }
+        }
+        // Q_gp = w a_J R^T Td R (symmetric), cz_kernels.h:366-401
+        double TR[9];
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) TR[m * 3 + l] = Td[m * 3] * R[l] + Td[m * 3 + 1] * R[3 + l] + Td[m * 3 + 2] * R[6 + l];
+        double q[9];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) q[k * 3 + l] = wa * (R[k] * TR[l] + R[3 + k] * TR[3 + l] + R[6 + k] * TR[6 + l]);
+        Q[6 * gp + 0] = q[0]; Q[6 * gp + 1] = q[1]; Q[6 * gp + 2] = q[2];
+        Q[6 * gp + 3] = q[4]; Q[6 * gp + 4] = q[5]; Q[6 * gp + 5] = q[8];
+    }
+
+    // tentative history = max over Gauss points in all three slots; flags (cz_kernels.h:404-422)
+    int damaged = 0;
+    const double pm = fmax(fmax(pmax[0], pmax[1]), pmax[2]);
+    const double tm = fmax(fmax(tmax[0], tmax[1]), tmax[2]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        dcz[u + cs * (CZD_TPMAX + i)] = pm;
+        dcz[u + cs * (CZD_TTMAX + i)] = tm;
+        if (pmax[i] >= g_c.deln * g_c.rn || tmax[i] >= g_c.delt * g_c.rt) damaged = 1;
+    }
+    icz[u + cs * CZI_TCONTACT] = anyContact ? 1 : 0;
+    icz[u + cs * CZI_TFAILED] = anyFailed ? 1 : 0;
+    icz[u + cs * CZI_TDAMAGED] = anyFailed ? 0 : damaged;
+    if (anyFailed) {
+        write_zero();
+        return;
+    }
+    dcz[u + cs * CZD_DN] = avgDn;
+    dcz[u + cs * CZD_DT] = avgDt;
+    dcz[u + cs * CZD_TN] = avgTn;
+    dcz[u + cs * CZD_TT] = avgTt;
+#pragma unroll
+    for (int i = 0; i < 18; i++) orhs[i] = rhs[i];
+    if (MIXED) {
+        float *t = (float *)st_tan + (size_t)u * CZ_TAN_REC;
+#pragma unroll
+        for (int i = 0; i < 18; i++) t[i] = (float)Q[i];
+    } else {
+        double *t = (double *)st_tan + (size_t)u * CZ_TAN_REC;
+#pragma unroll
+        for (int i = 0; i < 18; i++) t[i] = Q[i];
+    }
+}
+
+// Phase 2: add tangent block (a,b), a,b in 0..5, of CZ record `rec` to acc.
+template <typename T>
+__device__ inline void cz_block(double acc[9], const T *__restrict__ rec, int a, int b)
+{
+    const int ia = a % 3, ib = b % 3;
+    const T sg = ((a < 3) == (b < 3)) ? (T)1 : (T)-1;
+    T blk[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int gp = 0; gp < 3; gp++) {
+        const T w = sg * (T)g_c.sf[ia * 3 + gp] * (T)g_c.sf[ib * 3 + gp];
+#pragma unroll
+        for (int q = 0; q < 6; q++) blk[q] += w * rec[6 * gp + q];
+    }
+    acc[0] += (double)blk[0]; acc[1] += (double)blk[1]; acc[2] += (double)blk[2];
+    acc[3] += (double)blk[1]; acc[4] += (double)blk[3]; acc[5] += (double)blk[4];
+    acc[6] += (double)blk[2]; acc[7] += (double)blk[4]; acc[8] += (double)blk[5];
+}
+
+}  // namespace icf

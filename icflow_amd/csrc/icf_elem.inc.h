@@ -1,0 +1,324 @@
+// icf_elem.inc.h -- corotational linear tetrahedron (included by icf_lib.hip).
+//
+// Phase 1 (k_tet_phase1, one thread per tet) computes the tet's residual contribution in FP64
+// and a compact tangent record; phase 2 (the slot reduction in icf_reduce.inc.h) rebuilds
+// each 3x3 tangent block from that record inside the BCSR slot that owns it.
+//
+// Restated from elem_kernels9/element_kernels2.h:19-263 (kelElementElasticityForce and
+// F_and_Df_Corotational) and matrix_math1.h:32-71 (fastRotationMatrix).  Differences of
+// formulation (same mathematics):
+//  - K = B^T E B V is never formed: K y is applied as V B^T (E (B y)) (B is sparse);
+//  - the tangent block (i,j) is R K_ij R^T; for isotropic E it equals
+//    V (lam g~_i g~_j^T + mu g~_j g~_i^T + mu (g~_i . g~_j) I) with corotated gradients
+//    g~ = R g, so only g~ and V are staged (16 values per tet).
+#pragma once
+
+namespace icf {
+
+// XCD-aware block order: the hardware deals workgroups round-robin over the 8 XCDs, each of
+// which has its own L2.  This maps the physical block id so that the blocks running on one
+// XCD cover one contiguous range of units (neighbouring tets share nodes).  Bijective on
+// [0, nb).
+__device__ inline int xcd_remap(int b, int nb)
+{
+    const int X = 8;
+    int q = nb / X, r = nb % X;
+    int x = b % X, k = b / X;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// Orthonormal frame of three points; columns = (p1-p0), (p1-p0)x(p2-p0), col1 x col0, each
+// normalised.  R[row*3+col].  Evaluated without FMA contraction so that it rounds exactly
+// like the reference's expression order (matrix_math1.h:57-96).
+__device__ inline void frame3(const double *p0, const double *p1, const double *p2, double R[9])
+{
+#pragma clang fp contract(off)
+    double d0 = p1[0] - p0[0], d1 = p1[1] - p0[1], d2 = p1[2] - p0[2];
+    double len = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    R[0] = d0 / len; R[3] = d1 / len; R[6] = d2 / len;
+    double w0 = p2[0] - p0[0], w1 = p2[1] - p0[1], w2 = p2[2] - p0[2];
+    double c0 = -d2 * w1 + d1 * w2;
+    double c1 = d2 * w0 - d0 * w2;
+    double c2 = -d1 * w0 + d0 * w1;
+    len = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    R[1] = c0 / len; R[4] = c1 / len; R[7] = c2 / len;
+    double e0 = R[4] * R[6] - R[3] * R[7];
+    double e1 = -R[1] * R[6] + R[0] * R[7];
+    double e2 = R[1] * R[3] - R[0] * R[4];
+    len = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    R[2] = e0 / len; R[5] = e1 / len; R[8] = e2 / len;
+}
+
+// Rest gradients of the four linear shape functions, g[n*3+k], and the rest volume.
+// Same cofactor formulas as element_kernels2.h:41-57.
+__device__ inline void rest_gradients(const double x0[12], double g[12], double &V)
+{
+#pragma clang fp contract(off)
+    const double X1 = x0[0], Y1 = x0[1], Z1 = x0[2], X2 = x0[3], Y2 = x0[4], Z2 = x0[5];
+    const double X3 = x0[6], Y3 = x0[7], Z3 = x0[8], X4 = x0[9], Y4 = x0[10], Z4 = x0[11];
+    double x12 = X1 - X2, x13 = X1 - X3, x14 = X1 - X4, x23 = X2 - X3, x24 = X2 - X4, x34 = X3 - X4;
+    double y12 = Y1 - Y2, y13 = Y1 - Y3, y14 = Y1 - Y4, y23 = Y2 - Y3, y24 = Y2 - Y4, y34 = Y3 - Y4;
+    double z12 = Z1 - Z2, z13 = Z1 - Z3, z14 = Z1 - Z4, z23 = Z2 - Z3, z24 = Z2 - Z4, z34 = Z3 - Z4;
+    double x21 = -x12, x31 = -x13, x32 = -x23, x42 = -x24, x43 = -x34;
+    double y31 = -y13, y32 = -y23, y42 = -y24, y43 = -y34, y21 = -y12;
+    double z21 = -z12, z31 = -z13, z32 = -z23, z42 = -z24, z43 = -z34;
+    double J = x21 * (y23 * z34 - y34 * z23) + x32 * (y34 * z12 - y12 * z34) + x43 * (y12 * z23 - y23 * z12);
+    V = J / 6.;
+    g[0] = y42 * z32 - y32 * z42;  g[1] = x32 * z42 - x42 * z32;  g[2] = x42 * y32 - x32 * y42;
+    g[3] = y31 * z43 - y34 * z13;  g[4] = x43 * z31 - x13 * z34;  g[5] = x31 * y43 - x34 * y13;
+    g[6] = y24 * z14 - y14 * z24;  g[7] = x14 * z24 - x24 * z14;  g[8] = x24 * y14 - x14 * y24;
+    g[9] = y13 * z21 - y12 * z31;  g[10] = x21 * z13 - x31 * z12; g[11] = x13 * y21 - x12 * y31;
+#pragma unroll
+    for (int i = 0; i < 12; i++) g[i] /= J;
+}
+
+// out = V B^T E B y for the 12-vector y (the rest stiffness applied to y).
+__device__ inline void apply_K0(const double g[12], double V, const double y[12], double out[12])
+{
+    double e[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const double a = g[3 * n], b = g[3 * n + 1], c = g[3 * n + 2];
+        const double yx = y[3 * n], yy = y[3 * n + 1], yz = y[3 * n + 2];
+        e[0] += a * yx;
+        e[1] += b * yy;
+        e[2] += c * yz;
+        e[3] += b * yx + a * yy;
+        e[4] += c * yy + b * yz;
+        e[5] += c * yx + a * yz;
+    }
+    double s[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        double v = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) v += g_c.E[r * 6 + k] * e[k];
+        s[r] = v * V;
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const double a = g[3 * n], b = g[3 * n + 1], c = g[3 * n + 2];
+        out[3 * n + 0] = a * s[0] + b * s[3] + c * s[5];
+        out[3 * n + 1] = b * s[1] + a * s[3] + c * s[4];
+        out[3 * n + 2] = c * s[2] + b * s[4] + a * s[5];
+    }
+}
+
+// Phase 1 for tets.  rhs (12 per tet) = gravity - f - dampK * Df vn - V M (dampM vn + an)
+// (element_kernels2.h:228-251), written FP64 to st_rhs; tangent record to st_tan
+// (double, or float when MIXED).
+template <bool MIXED, bool ISO>
+__global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, const double *__restrict__ dn,
+                                                    int n, int es, int ns, double *__restrict__ st_rhs,
+                                                    void *__restrict__ st_tan)
+{
+    const int nb = gridDim.x;
+    const int u = xcd_remap(blockIdx.x, nb) * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+
+    int nid[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) nid[i] = ie[u + es * (EL_NODES + i)];
+    double x0[12], un[12];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            x0[3 * i + k] = dn[nid[i] + ns * (ND_X0 + k)];
+            un[3 * i + k] = dn[nid[i] + ns * (ND_UN + k)];
+        }
+
+    double g[12], V;
+    rest_gradients(x0, g, V);
+
+    double R[9];
+    {
+        double R0[9], R1[9], xc[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) xc[i] = x0[i] + un[i];
+        frame3(&x0[0], &x0[3], &x0[6], R0);
+        frame3(&xc[0], &xc[3], &xc[6], R1);
+        {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    R[i * 3 + j] = R1[i * 3 + 0] * R0[j * 3 + 0] + R1[i * 3 + 1] * R0[j * 3 + 1] + R1[i * 3 + 2] * R0[j * 3 + 2];
+        }
+    }
+
+    // y = R^T x_c - x0, rounded like element_kernels2.h:127-150
+    double y[12];
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double p0 = x0[3 * i] + un[3 * i], p1 = x0[3 * i + 1] + un[3 * i + 1], p2 = x0[3 * i + 2] + un[3 * i + 2];
+            y[3 * i + 0] = (p0 * R[0] + p1 * R[3] + p2 * R[6]) - x0[3 * i + 0];
+            y[3 * i + 1] = (p0 * R[1] + p1 * R[4] + p2 * R[7]) - x0[3 * i + 1];
+            y[3 * i + 2] = (p0 * R[2] + p1 * R[5] + p2 * R[8]) - x0[3 * i + 2];
+        }
+    }
+
+    double rhs[12];
+    {
+        double Ky[12];
+        apply_K0(g, V, y, Ky);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                rhs[3 * i + k] = -(R[k * 3 + 0] * Ky[3 * i] + R[k * 3 + 1] * Ky[3 * i + 1] + R[k * 3 + 2] * Ky[3 * i + 2]);
+    }
+    {
+        // Rayleigh stiffness damping: Df vn = R K0 R^T vn
+        double w[12], Kw[12];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double v0 = dn[nid[i] + ns * (ND_VN + 0)], v1 = dn[nid[i] + ns * (ND_VN + 1)], v2 = dn[nid[i] + ns * (ND_VN + 2)];
+            w[3 * i + 0] = R[0] * v0 + R[3] * v1 + R[6] * v2;
+            w[3 * i + 1] = R[1] * v0 + R[4] * v1 + R[7] * v2;
+            w[3 * i + 2] = R[2] * v0 + R[5] * v1 + R[8] * v2;
+        }
+        apply_K0(g, V, w, Kw);
+        const double dK = g_c.dampK;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                rhs[3 * i + k] -= dK * (R[k * 3 + 0] * Kw[3 * i] + R[k * 3 + 1] * Kw[3 * i + 1] + R[k * 3 + 2] * Kw[3 * i + 2]);
+    }
+    {
+        // inertia and mass damping: V M (dampM vn + an); gravity rho g V / 4 on z
+        double q[12];
+        const double dM = g_c.dampM;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                q[3 * i + k] = dM * dn[nid[i] + ns * (ND_VN + k)] + dn[nid[i] + ns * (ND_AN + k)];
+        const double gz = g_c.grav * g_c.rho * V / 4;
+#pragma unroll
+        for (int r = 0; r < 12; r++) {
+            double m = 0;
+#pragma unroll
+            for (int c = 0; c < 12; c++) m += g_c.M[r * 12 + c] * q[c];
+            rhs[r] -= V * m;
+            if (r % 3 == 2) rhs[r] += gz;
+        }
+    }
+    double *o = st_rhs + (size_t)u * TET_RHS_REC;
+#pragma unroll
+    for (int i = 0; i < 12; i++) o[i] = rhs[i];
+
+    if (ISO) {
+        double gt[12];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                gt[3 * i + k] = R[k * 3 + 0] * g[3 * i] + R[k * 3 + 1] * g[3 * i + 1] + R[k * 3 + 2] * g[3 * i + 2];
+        if (MIXED) {
+            float *t = (float *)st_tan + (size_t)u * TET_ISO_REC;
+#pragma unroll
+            for (int i = 0; i < 12; i++) t[i] = (float)gt[i];
+            t[12] = (float)V;
+            t[13] = t[14] = t[15] = 0.f;
+        } else {
+            double *t = (double *)st_tan + (size_t)u * TET_ISO_REC;
+#pragma unroll
+            for (int i = 0; i < 12; i++) t[i] = gt[i];
+            t[12] = V;
+            t[13] = t[14] = t[15] = 0.;
+        }
+    } else {
+        if (MIXED) {
+            float *t = (float *)st_tan + (size_t)u * TET_GEN_REC;
+#pragma unroll
+            for (int i = 0; i < 9; i++) t[i] = (float)R[i];
+#pragma unroll
+            for (int i = 0; i < 12; i++) t[9 + i] = (float)g[i];
+            t[21] = (float)V;
+            t[22] = t[23] = 0.f;
+        } else {
+            double *t = (double *)st_tan + (size_t)u * TET_GEN_REC;
+#pragma unroll
+            for (int i = 0; i < 9; i++) t[i] = R[i];
+#pragma unroll
+            for (int i = 0; i < 12; i++) t[9 + i] = g[i];
+            t[21] = V;
+            t[22] = t[23] = 0.;
+        }
+    }
+}
+
+// Phase 2: add LHS block (i,j) of tet record `rec` to acc.  LHS = stiff * R K R^T + mcoef V M
+// with stiff = 1 + dampK gamma / (h beta), mcoef = (1/h^2 + dampM gamma / h) / beta
+// (element_kernels2.h:239-249).
+template <typename T>
+__device__ inline void tet_block_iso(double acc[9], const T *__restrict__ rec, int i, int j, T stiff, T mcoef)
+{
+    const T lam = sizeof(T) == 4 ? (T)g_c.lamf : (T)g_c.lam;
+    const T mu = sizeof(T) == 4 ? (T)g_c.muf : (T)g_c.mu;
+    T gi[3] = {rec[3 * i], rec[3 * i + 1], rec[3 * i + 2]};
+    T gj[3] = {rec[3 * j], rec[3 * j + 1], rec[3 * j + 2]};
+    const T V = rec[12];
+    const T c1 = stiff * V, cm = mcoef * V;
+    const T dg = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+            const int mi = (3 * i + k) * 12 + 3 * j + l;
+            const T m = sizeof(T) == 4 ? (T)g_c.Mf[mi] : (T)g_c.M[mi];
+            T v = c1 * (lam * gi[k] * gj[l] + mu * gj[k] * gi[l]) + cm * m;
+            if (k == l) v += c1 * mu * dg;
+            acc[3 * k + l] += (double)v;
+        }
+}
+
+template <typename T>
+__device__ inline void tet_block_gen(double acc[9], const T *__restrict__ rec, int i, int j, T stiff, T mcoef)
+{
+    T R[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) R[q] = rec[q];
+    const T *g = rec + 9;
+    const T V = rec[21];
+    const T ai = g[3 * i], bi = g[3 * i + 1], ci = g[3 * i + 2];
+    const T aj = g[3 * j], bj = g[3 * j + 1], cj = g[3 * j + 2];
+    auto E = [](int r, int c) -> T { return sizeof(T) == 4 ? (T)g_c.Ef[r * 6 + c] : (T)g_c.E[r * 6 + c]; };
+    // columns of B_j (Voigt xx,yy,zz,xy,yz,zx): x: (a,0,0,b,0,c)  y: (0,b,0,a,c,0)  z: (0,0,c,0,b,a)
+    T EB[6][3];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        EB[r][0] = E(r, 0) * aj + E(r, 3) * bj + E(r, 5) * cj;
+        EB[r][1] = E(r, 1) * bj + E(r, 3) * aj + E(r, 4) * cj;
+        EB[r][2] = E(r, 2) * cj + E(r, 4) * bj + E(r, 5) * aj;
+    }
+    T K[3][3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        K[0][l] = (ai * EB[0][l] + bi * EB[3][l] + ci * EB[5][l]) * V;
+        K[1][l] = (bi * EB[1][l] + ai * EB[3][l] + ci * EB[4][l]) * V;
+        K[2][l] = (ci * EB[2][l] + bi * EB[4][l] + ai * EB[5][l]) * V;
+    }
+    T RK[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) RK[k][l] = R[k * 3] * K[0][l] + R[k * 3 + 1] * K[1][l] + R[k * 3 + 2] * K[2][l];
+    const T cm = mcoef * V;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+            const int mi = (3 * i + k) * 12 + 3 * j + l;
+            const T m = sizeof(T) == 4 ? (T)g_c.Mf[mi] : (T)g_c.M[mi];
+            T v = stiff * (RK[k][0] * R[l * 3] + RK[k][1] * R[l * 3 + 1] + RK[k][2] * R[l * 3 + 2]) + cm * m;
+            acc[3 * k + l] += (double)v;
+        }
+}
+
+}  // namespace icf

@@ -1,0 +1,682 @@
+// icf_lib.hip -- libicflow_hip.so: the C ABI declared in include/icflow_hip.h.
+//
+// Single translation unit: the kernels of icf_*.inc.h share one __constant__ DevConsts.
+// Device memory for staging records and slot maps is owned by the context and grows on
+// demand; the caller's buffers (nodes, pcsr arrays, BCSR, rhs) are never retained.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+#include <string>
+#include <vector>
+#include <list>
+#include <type_traits>
+#include <atomic>
+
+#include "../../include/icflow_hip.h"
+
+#include "icf_device.h"
+namespace icf {
+__constant__ DevConsts g_c;
+}
+#include "icf_elem.inc.h"
+#include "icf_cz.inc.h"
+#include "icf_contact.inc.h"
+#include "icf_reduce.inc.h"
+
+using namespace icf;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Target { TGT_VALS = 0, TGT_RHS = 1, TGT_FORCE = 2 };
+
+struct SrcKey {
+    uint32_t kind;
+    const int *base;
+    int n, stride, field0, nq;
+    bool operator==(const SrcKey &o) const
+    {
+        return kind == o.kind && base == o.base && n == o.n && stride == o.stride && field0 == o.field0 && nq == o.nq;
+    }
+};
+
+struct SlotMap {
+    int target = 0;
+    int fixed_nkeys = -1;          // -1: number of keys = largest key + 1
+    std::vector<SrcKey> srcs;
+    int nkeys = 0;
+    int ncodes = 0;
+    int *off = nullptr;            // nkeys + 1
+    uint32_t *codes = nullptr;     // ncodes
+};
+
+std::atomic<int> g_next_ctx_id{1};
+int g_const_owner[64];             // per device: id of the context whose constants are loaded
+
+}  // namespace
+
+struct icf_ctx {
+    int id = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool has_consts = false;
+    int precision = ICF_FP64;
+    DevConsts hc;
+    DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
+    int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
+    std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
+    std::string err;
+    double last_ms = -1;
+    bool timing = false;
+};
+
+namespace {
+
+int fail(icf_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), (int)e_, std::string(#x) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+int grow(icf_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes) return 0;
+    if (b.p) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4 + 256;
+    HIPCHK(ctx, hipMalloc(&b.p, want));
+    b.bytes = want;
+    return 0;
+}
+
+void free_map(SlotMap &m)
+{
+    if (m.off) (void)hipFree(m.off);
+    if (m.codes) (void)hipFree(m.codes);
+    m.off = nullptr;
+    m.codes = nullptr;
+}
+
+// Drop cached maps whose key fields overlap [lo, hi).
+void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi)
+{
+    for (auto it = ctx->maps.begin(); it != ctx->maps.end();) {
+        bool hit = false;
+        for (const SrcKey &s : it->srcs) {
+            uintptr_t a = (uintptr_t)(s.base + (size_t)s.stride * s.field0);
+            uintptr_t b = (uintptr_t)(s.base + (size_t)s.stride * (s.field0 + s.nq));
+            if (a < hi && lo < b) hit = true;
+        }
+        if (hit) {
+            (void)hipStreamSynchronize(ctx->stream);
+            free_map(*it);
+            it = ctx->maps.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+int upload_consts(icf_ctx *ctx)
+{
+    if (!ctx->has_consts) return fail(ctx, ICF_E_NOCONST, "icf_set_constants has not been called");
+    if (g_const_owner[ctx->device] != ctx->id) {
+        HIPCHK(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(g_c), &ctx->hc, sizeof(DevConsts), 0, hipMemcpyHostToDevice,
+                                           ctx->stream));
+        g_const_owner[ctx->device] = ctx->id;
+    }
+    return 0;
+}
+
+unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Build (or fetch) the slot map for `srcs` with keys of `target`.
+int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey> &srcs, SlotMap **out)
+{
+    for (SlotMap &m : ctx->maps)
+        if (m.target == target && m.fixed_nkeys == fixed_nkeys && m.srcs == srcs) {
+            *out = &m;
+            return 0;
+        }
+    for (const SrcKey &s : srcs)
+        if ((uint32_t)s.n > CODE_UNIT_MAX + 1u)
+            return fail(ctx, ICF_E_CAPACITY, "more than 2^24 units in one pcsr array");
+
+    SlotMap m;
+    m.target = target;
+    m.fixed_nkeys = fixed_nkeys;
+    m.srcs = srcs;
+    int *d_max = ctx->small, *d_err = ctx->small + 1;
+    int nkeys = fixed_nkeys;
+    if (nkeys < 0) {
+        int init = -1;
+        HIPCHK(ctx, hipMemcpyAsync(d_max, &init, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        for (const SrcKey &s : srcs) {
+            if (s.n == 0) continue;
+            MapSrc ms{s.base, s.n, s.stride, s.field0, s.nq, s.kind};
+            long long tot = (long long)s.n * s.nq;
+            hipLaunchKernelGGL(k_map_maxkey, dim3(blocks_for(tot, 256)), dim3(256), 0, ctx->stream, ms, d_max);
+        }
+        int mx = -1;
+        HIPCHK(ctx, hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        nkeys = mx + 1;
+    }
+    m.nkeys = nkeys;
+    HIPCHK(ctx, hipMalloc(&m.off, sizeof(int) * ((size_t)nkeys + 1)));
+    int *d_cnt = nullptr;
+    HIPCHK(ctx, hipMalloc(&d_cnt, sizeof(int) * ((size_t)nkeys + 1)));
+    HIPCHK(ctx, hipMemsetAsync(d_cnt, 0, sizeof(int) * ((size_t)nkeys + 1), ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(d_err, 0, sizeof(int), ctx->stream));
+    for (const SrcKey &s : srcs) {
+        if (s.n == 0) continue;
+        MapSrc ms{s.base, s.n, s.stride, s.field0, s.nq, s.kind};
+        long long tot = (long long)s.n * s.nq;
+        hipLaunchKernelGGL(k_map_count, dim3(blocks_for(tot, 256)), dim3(256), 0, ctx->stream, ms, nkeys, d_cnt, d_err);
+    }
+    size_t tmp_bytes = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_cnt, m.off, nkeys + 1, ctx->stream));
+    if (grow(ctx, ctx->scan_tmp, tmp_bytes)) {
+        (void)hipFree(d_cnt);
+        return (int)hipErrorOutOfMemory;
+    }
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, d_cnt, m.off, nkeys + 1, ctx->stream));
+    int total = 0, errflag = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&total, m.off + nkeys, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&errflag, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (errflag) {
+        (void)hipFree(d_cnt);
+        free_map(m);
+        return fail(ctx, ICF_E_INDEX, "a slot/row/node index exceeds the declared size");
+    }
+    m.ncodes = total;
+    HIPCHK(ctx, hipMalloc(&m.codes, sizeof(uint32_t) * ((size_t)total + 1)));
+    HIPCHK(ctx, hipMemsetAsync(d_cnt, 0, sizeof(int) * ((size_t)nkeys + 1), ctx->stream));
+    for (const SrcKey &s : srcs) {
+        if (s.n == 0) continue;
+        MapSrc ms{s.base, s.n, s.stride, s.field0, s.nq, s.kind};
+        long long tot = (long long)s.n * s.nq;
+        hipLaunchKernelGGL(k_map_fill, dim3(blocks_for(tot, 256)), dim3(256), 0, ctx->stream, ms, nkeys, m.off, d_cnt,
+                           m.codes);
+    }
+    if (nkeys > 0)
+        hipLaunchKernelGGL(k_map_sort, dim3(blocks_for(nkeys, 256)), dim3(256), 0, ctx->stream, m.off, m.codes, nkeys);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(d_cnt));
+    ctx->maps.push_back(m);
+    *out = &ctx->maps.back();
+    return 0;
+}
+
+int check_ctx(icf_ctx *ctx)
+{
+    if (!ctx) return ICF_E_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return 0;
+}
+
+struct Timer {
+    icf_ctx *ctx;
+    explicit Timer(icf_ctx *c) : ctx(c)
+    {
+        ctx->last_ms = -1;
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+    }
+    int finish()
+    {
+        HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_ms = ms;
+        HIPCHK(ctx, hipGetLastError());
+        return 0;
+    }
+};
+
+// ---- launch helpers ---------------------------------------------------------------------------
+
+int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int ns)
+{
+    const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
+    const size_t rec = iso ? TET_ISO_REC : TET_GEN_REC;
+    if (int r = grow(ctx, ctx->tet_rhs, sizeof(double) * TET_RHS_REC * (size_t)(n > 0 ? n : 1))) return r;
+    if (int r = grow(ctx, ctx->tet_tan, (mixed ? 4 : 8) * rec * (size_t)(n > 0 ? n : 1))) return r;
+    if (n == 0) return 0;
+    dim3 g(blocks_for(n, 256)), b(256);
+    double *sr = (double *)ctx->tet_rhs.p;
+    void *st = ctx->tet_tan.p;
+    if (mixed && iso) hipLaunchKernelGGL((k_tet_phase1<true, true>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
+    else if (mixed) hipLaunchKernelGGL((k_tet_phase1<true, false>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
+    else if (iso) hipLaunchKernelGGL((k_tet_phase1<false, true>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
+    else hipLaunchKernelGGL((k_tet_phase1<false, false>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
+    return 0;
+}
+
+int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int cs, int ns)
+{
+    const bool mixed = ctx->precision == ICF_MIXED;
+    if (int r = grow(ctx, ctx->cz_rhs, sizeof(double) * CZ_RHS_REC * (size_t)(n > 0 ? n : 1))) return r;
+    if (int r = grow(ctx, ctx->cz_tan, (mixed ? 4 : 8) * CZ_TAN_REC * (size_t)(n > 0 ? n : 1))) return r;
+    if (n == 0) return 0;
+    dim3 g(blocks_for(n, 256)), b(256);
+    if (mixed)
+        hipLaunchKernelGGL((k_cz_phase1<true>), g, b, 0, ctx->stream, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
+                           ctx->cz_tan.p);
+    else
+        hipLaunchKernelGGL((k_cz_phase1<false>), g, b, 0, ctx->stream, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
+                           ctx->cz_tan.p);
+    return 0;
+}
+
+Stage make_stage(icf_ctx *ctx, double h)
+{
+    Stage sv;
+    sv.tet_rhs = (const double *)ctx->tet_rhs.p;
+    sv.tet_tan = ctx->tet_tan.p;
+    sv.cz_rhs = (const double *)ctx->cz_rhs.p;
+    sv.cz_tan = ctx->cz_tan.p;
+    sv.ct = (const double *)ctx->ct.p;
+    const DevConsts &c = ctx->hc;
+    sv.stiff = 1.0 + c.dampK * c.ngamma / (h * c.nbeta);
+    sv.mcoef = (1.0 / (h * h) + c.dampM * c.ngamma / h) / c.nbeta;
+    return sv;
+}
+
+void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, const Stage &sv)
+{
+    if (m->nkeys == 0) return;
+    const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
+    dim3 g(blocks_for(m->nkeys, 256)), b(256);
+    if (mixed && iso) hipLaunchKernelGGL((k_reduce_vals<true, true>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    else if (mixed) hipLaunchKernelGGL((k_reduce_vals<true, false>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    else if (iso) hipLaunchKernelGGL((k_reduce_vals<false, true>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    else hipLaunchKernelGGL((k_reduce_vals<false, false>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
+}
+
+void launch_vec3(icf_ctx *ctx, const SlotMap *m, double *out, int comp_stride, int elem_stride, int accumulate,
+                 const Stage &sv)
+{
+    if (m->nkeys == 0) return;
+    hipLaunchKernelGGL(k_reduce_vec3, dim3(blocks_for(m->nkeys, 256)), dim3(256), 0, ctx->stream, m->off, m->codes,
+                       m->nkeys, out, comp_stride, elem_stride, accumulate, sv);
+}
+
+bool bad_units(int n, int stride) { return n < 0 || stride < 0 || (n > 0 && stride < n); }
+
+}  // namespace
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+extern "C" {
+
+int icf_create(int device, icf_ctx **out)
+{
+    if (!out || device < 0 || device >= 64) return ICF_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) return (int)e;
+    if (device >= ndev) return ICF_E_INVALID;
+    icf_ctx *ctx = new icf_ctx();
+    ctx->id = g_next_ctx_id++;
+    ctx->device = device;
+    memset(&ctx->hc, 0, sizeof(ctx->hc));
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipMalloc(&ctx->small, 8 * sizeof(int));
+    if (e != hipSuccess) {
+        delete ctx;
+        return (int)e;
+    }
+    *out = ctx;
+    return 0;
+}
+
+int icf_destroy(icf_ctx *ctx)
+{
+    if (!ctx) return ICF_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (SlotMap &m : ctx->maps) free_map(m);
+    DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct, &ctx->scan_tmp};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (ctx->small) (void)hipFree(ctx->small);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
+    delete ctx;
+    return 0;
+}
+
+const char *icf_last_error_string(const icf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int icf_set_constants(icf_ctx *ctx, const icf_consts *c)
+{
+    if (!ctx || !c) return ICF_E_INVALID;
+    // E symmetric
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++)
+            if (c->E[i * 6 + j] != c->E[j * 6 + i]) return fail(ctx, ICF_E_CONSTANTS, "E is not symmetric");
+    // B must be [s (x) I3, -s (x) I3] built from sf (ModelPrms.k_Bmatrix)
+    for (int gp = 0; gp < 3; gp++)
+        for (int p = 0; p < 3; p++)
+            for (int k = 0; k < 18; k++) {
+                int node = k / 3;
+                double want = 0;
+                if (k % 3 == p) want = node < 3 ? c->sf[node * 3 + gp] : -c->sf[(node - 3) * 3 + gp];
+                if (c->B[gp * 54 + p * 18 + k] != want)
+                    return fail(ctx, ICF_E_CONSTANTS, "CZ B table is not the shape-function Kronecker form");
+            }
+    DevConsts &d = ctx->hc;
+    memset(&d, 0, sizeof(d));
+    memcpy(d.E, c->E, sizeof(d.E));
+    memcpy(d.M, c->M, sizeof(d.M));
+    d.nbeta = c->NewmarkBeta; d.ngamma = c->NewmarkGamma;
+    d.dampM = c->dampingMass; d.dampK = c->dampingStiffness;
+    d.rho = c->rho; d.grav = c->gravity;
+    d.G_fn = c->G_fn; d.G_ft = c->G_ft; d.f_tn = c->f_tn; d.f_tt = c->f_tt;
+    d.alpha = c->alpha; d.beta = c->beta; d.rn = c->rn; d.rt = c->rt;
+    d.p_m = c->p_m; d.p_n = c->p_n; d.deln = c->deln; d.delt = c->delt;
+    d.pMtn = c->pMtn; d.pMnt = c->pMnt; d.gam_n = c->gam_n; d.gam_t = c->gam_t;
+    memcpy(d.sf, c->sf, sizeof(d.sf));
+    // isotropic split: E = lam 1 1^T (normal block) + diag(2mu,2mu,2mu,mu,mu,mu)
+    {
+        double lam = c->E[1], mu = c->E[3 * 6 + 3], dg = c->E[0];
+        bool iso = mu > 0;
+        for (int i = 0; i < 6 && iso; i++)
+            for (int j = 0; j < 6 && iso; j++) {
+                double v = c->E[i * 6 + j];
+                if (i < 3 && j < 3) iso = (i == j) ? (v == dg) : (v == lam);
+                else if (i == j) iso = (v == mu);
+                else iso = (v == 0);
+            }
+        if (iso) iso = fabs(dg - (lam + 2 * mu)) <= 1e-12 * fabs(dg);
+        d.iso = iso ? 1 : 0;
+        d.lam = lam;
+        d.mu = mu;
+        d.lamf = (float)lam;
+        d.muf = (float)mu;
+    }
+    for (int e = 0; e < 3; e++) {
+        d.bn0[e] = pow(c->p_m / c->alpha, c->p_m - e);
+        d.bt0[e] = pow(c->p_n / c->beta, c->p_n - e);
+    }
+    for (int i = 0; i < 36; i++) d.Ef[i] = (float)d.E[i];
+    for (int i = 0; i < 144; i++) d.Mf[i] = (float)d.M[i];
+    ctx->has_consts = true;
+    g_const_owner[ctx->device] = 0;  // force re-upload
+    if (int r = check_ctx(ctx)) return r;
+    return upload_consts(ctx);
+}
+
+int icf_set_precision(icf_ctx *ctx, int mode)
+{
+    if (!ctx || (mode != ICF_FP64 && mode != ICF_MIXED)) return ICF_E_INVALID;
+    ctx->precision = mode;
+    return 0;
+}
+
+int icf_get_precision(const icf_ctx *ctx) { return ctx ? ctx->precision : ICF_E_INVALID; }
+
+int icf_alloc(icf_ctx *ctx, size_t bytes, void **dptr)
+{
+    if (!ctx || !dptr) return ICF_E_INVALID;
+    if (int r = check_ctx(ctx)) return r;
+    *dptr = nullptr;
+    HIPCHK(ctx, hipMalloc(dptr, bytes ? bytes : 1));
+    return 0;
+}
+
+int icf_free(icf_ctx *ctx, void *dptr)
+{
+    if (!ctx) return ICF_E_INVALID;
+    if (!dptr) return 0;
+    if (int r = check_ctx(ctx)) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    size_t sz = 0;
+    void *basep = nullptr;
+    if (hipMemGetAddressRange(&basep, &sz, dptr) == hipSuccess && sz)
+        invalidate_range(ctx, (uintptr_t)basep, (uintptr_t)basep + sz);
+    else
+        invalidate_range(ctx, (uintptr_t)dptr, (uintptr_t)dptr + 1);
+    HIPCHK(ctx, hipFree(dptr));
+    return 0;
+}
+
+int icf_h2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return ICF_E_INVALID;
+    if (int r = check_ctx(ctx)) return r;
+    if (!bytes) return 0;
+    char *d = (char *)dst + dst_off;
+    invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
+    HIPCHK(ctx, hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_d2h(icf_ctx *ctx, void *dst, const void *src, size_t src_off, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return ICF_E_INVALID;
+    if (int r = check_ctx(ctx)) return r;
+    if (!bytes) return 0;
+    HIPCHK(ctx, hipMemcpyAsync(dst, (const char *)src + src_off, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_memset(icf_ctx *ctx, void *dst, int value, size_t bytes)
+{
+    if (!ctx || (!dst && bytes)) return ICF_E_INVALID;
+    if (int r = check_ctx(ctx)) return r;
+    if (!bytes) return 0;
+    invalidate_range(ctx, (uintptr_t)dst, (uintptr_t)dst + bytes);
+    HIPCHK(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_synchronize(icf_ctx *ctx)
+{
+    if (int r = check_ctx(ctx)) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_pattern_changed(icf_ctx *ctx)
+{
+    if (int r = check_ctx(ctx)) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (SlotMap &m : ctx->maps) free_map(m);
+    ctx->maps.clear();
+    return 0;
+}
+
+double icf_last_device_ms(const icf_ctx *ctx) { return ctx ? ctx->last_ms : -1; }
+
+int icf_elem_force(icf_ctx *ctx, const int *ie, double *dn, double h, double *A, double *b, int nElems, int el_stride,
+                   int nd_stride)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nElems, el_stride) || nd_stride < 0 || (nElems && (!ie || !dn || !A || !b)) || !(h > 0))
+        return fail(ctx, ICF_E_INVALID, "icf_elem_force: invalid argument");
+    if (int r = upload_consts(ctx)) return r;
+    if (nElems == 0) return 0;
+    SlotMap *mv, *mr, *mf;
+    std::vector<SrcKey> sv{{KIND_TET, ie, nElems, el_stride, EL_SLOTS, 16}};
+    std::vector<SrcKey> sr{{KIND_TET, ie, nElems, el_stride, EL_ROWS, 4}};
+    std::vector<SrcKey> sfv{{KIND_TET, ie, nElems, el_stride, EL_NODES, 4}};
+    if (int r = get_map(ctx, TGT_VALS, -1, sv, &mv)) return r;
+    if (int r = get_map(ctx, TGT_RHS, -1, sr, &mr)) return r;
+    if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
+    Timer tm(ctx);
+    if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
+    Stage st = make_stage(ctx, h);
+    launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
+    launch_vals(ctx, mv, A, 1, st);
+    launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    return tm.finish();
+}
+
+int icf_cz_force(icf_ctx *ctx, double *dcz, int *icz, double *dn, double *A, double *b, double h, int nCZs,
+                 int cz_stride, int nd_stride)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nCZs, cz_stride) || nd_stride < 0 || (nCZs && (!dcz || !icz || !dn || !A || !b)))
+        return fail(ctx, ICF_E_INVALID, "icf_cz_force: invalid argument");
+    if (int r = upload_consts(ctx)) return r;
+    if (nCZs == 0) return 0;
+    SlotMap *mv, *mr;
+    std::vector<SrcKey> sv{{KIND_CZ, icz, nCZs, cz_stride, CZI_SLOTS, 36}};
+    std::vector<SrcKey> sr{{KIND_CZ, icz, nCZs, cz_stride, CZI_ROWS, 6}};
+    if (int r = get_map(ctx, TGT_VALS, -1, sv, &mv)) return r;
+    if (int r = get_map(ctx, TGT_RHS, -1, sr, &mr)) return r;
+    Timer tm(ctx);
+    if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
+    Stage st = make_stage(ctx, h);
+    launch_vals(ctx, mv, A, 1, st);
+    launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    return tm.finish();
+}
+
+int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, int *narrowList, int el_all_stride,
+                     int nd_stride)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (nPairs < 0 || el_all_stride < 0 || nd_stride < 0 || (nPairs && (!dn || !ie || !narrowList)))
+        return fail(ctx, ICF_E_INVALID, "icf_narrow_phase: invalid argument");
+    if (nPairs == 0) return 0;
+    Timer tm(ctx);
+    hipLaunchKernelGGL(k_narrow_phase, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, nPairs, dn, ie,
+                       narrowList, el_all_stride, nd_stride);
+    return tm.finish();
+}
+
+int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, const double *dn, const int *ie,
+                          const int *faces, int nd_stride, int el_all_stride, int fc_stride)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nPairs, tet_stride) || nd_stride < 0 || el_all_stride < 0 || fc_stride < 0 ||
+        (nPairs && (!nl2 || !dn || !ie || !faces)))
+        return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: invalid argument");
+    if (nPairs == 0) return 0;
+    Timer tm(ctx);
+    hipLaunchKernelGGL(k_closest_face, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, nPairs, tet_stride,
+                       nl2, dn, ie, faces, nd_stride, el_all_stride, fc_stride);
+    return tm.finish();
+}
+
+int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, double *A, double *b, int nImpacts,
+                           int cr_stride, int nd_stride, double k, double distanceEpsilon)
+{
+    (void)h;
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nImpacts, cr_stride) || nd_stride < 0 || (nImpacts && (!icr || !dn || !A || !b)))
+        return fail(ctx, ICF_E_INVALID, "icf_collision_response: invalid argument");
+    if (nImpacts == 0) return 0;
+    SlotMap *mv, *mr, *mf;
+    std::vector<SrcKey> sv{{KIND_CONTACT, icr, nImpacts, cr_stride, CR_SLOTS, 16}};
+    std::vector<SrcKey> sr{{KIND_CONTACT, icr, nImpacts, cr_stride, CR_ROWS, 4}};
+    std::vector<SrcKey> sfv{{KIND_CONTACT, icr, nImpacts, cr_stride, CR_NODES, 4}};
+    if (int r = get_map(ctx, TGT_VALS, -1, sv, &mv)) return r;
+    if (int r = get_map(ctx, TGT_RHS, -1, sr, &mr)) return r;
+    if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
+    if (int r = grow(ctx, ctx->ct, sizeof(double) * CT_REC * (size_t)nImpacts)) return r;
+    Timer tm(ctx);
+    hipLaunchKernelGGL(k_contact_phase1, dim3(blocks_for(nImpacts, 128)), dim3(128), 0, ctx->stream, icr, dn,
+                       nImpacts, cr_stride, nd_stride, k, distanceEpsilon, (double *)ctx->ct.p);
+    Stage st = make_stage(ctx, 1.0);
+    launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
+    launch_vals(ctx, mv, A, 1, st);
+    launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    return tm.finish();
+}
+
+int icf_sum_int(icf_ctx *ctx, const int *d, int n, int *out)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (n < 0 || !out || (n && !d)) return fail(ctx, ICF_E_INVALID, "icf_sum_int: invalid argument");
+    *out = 0;
+    if (n == 0) return 0;
+    HIPCHK(ctx, hipMemsetAsync(ctx->small + 2, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_sum_int, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, d, n, ctx->small + 2);
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->small + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_stride, double *dcz, int *icz, int nCZs,
+                               int cz_stride, double *dn, int nd_stride, double h, double *A, int nnzb, double *b,
+                               int N, int *nCZDamaged, int *nCZFailed)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) || nd_stride < 0 || nnzb < 0 || N < 0 ||
+        (nElems && !ie) || (nCZs && (!dcz || !icz)) || !dn || (nnzb && !A) || (N && !b) || !(h > 0))
+        return fail(ctx, ICF_E_INVALID, "icf_assemble_elems_and_czs: invalid argument");
+    if (int r = upload_consts(ctx)) return r;
+    std::vector<SrcKey> sv, sr, sfv;
+    if (nElems) {
+        sv.push_back({KIND_TET, ie, nElems, el_stride, EL_SLOTS, 16});
+        sr.push_back({KIND_TET, ie, nElems, el_stride, EL_ROWS, 4});
+        sfv.push_back({KIND_TET, ie, nElems, el_stride, EL_NODES, 4});
+    }
+    if (nCZs) {
+        sv.push_back({KIND_CZ, icz, nCZs, cz_stride, CZI_SLOTS, 36});
+        sr.push_back({KIND_CZ, icz, nCZs, cz_stride, CZI_ROWS, 6});
+    }
+    SlotMap *mv, *mr, *mf = nullptr;
+    if (int r = get_map(ctx, TGT_VALS, nnzb, sv, &mv)) return r;
+    if (int r = get_map(ctx, TGT_RHS, N, sr, &mr)) return r;
+    if (nElems)
+        if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
+    Timer tm(ctx);
+    if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
+    if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
+    Stage st = make_stage(ctx, h);
+    if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
+    launch_vals(ctx, mv, A, 0, st);
+    launch_vec3(ctx, mr, b, 1, 3, 0, st);
+    if (nCZs) {
+        HIPCHK(ctx, hipMemsetAsync(ctx->small + 2, 0, 2 * sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256), 0, ctx->stream,
+                           icz + (size_t)cz_stride * CZI_TDAMAGED, nCZs, ctx->small + 2);
+        hipLaunchKernelGGL(k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256), 0, ctx->stream,
+                           icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 3);
+    }
+    if (int r = tm.finish()) return r;
+    int sums[2] = {0, 0};
+    if (nCZs) {
+        HIPCHK(ctx, hipMemcpy(sums, ctx->small + 2, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    if (nCZDamaged) *nCZDamaged = sums[0];
+    if (nCZFailed) *nCZFailed = sums[1];
+    return 0;
+}
+
+}  // extern "C"

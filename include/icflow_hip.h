@@ -1,0 +1,137 @@
+/*
+ * icflow_hip.h -- C ABI of libicflow_hip.so, the MI355X (gfx950) drop-in for the device side
+ * of icFlow's per-Newton-iteration assembly path.
+ *
+ * It replaces the ManagedCUDA calls made by v3Library/Simulation/GPU/GPU_Functionality.cs:
+ * one context object instead of CudaContext + three PTX modules, one call per kernel launch
+ * with the reference kernel's own argument list, and the same SoA buffers and strides
+ * (value(item, field) = buf[item + stride * field]).  Plain pointers and sizes only; the
+ * caller may be C#, via [DllImport("icflow_hip")].
+ *
+ * Semantics kept from the reference:
+ *  - vals (A), rhs (b) and the node force column F of dn are ACCUMULATED into, in the fixed
+ *    order unit 0..n-1 (the reference used CAS-loop atomics, whose order is arbitrary; here
+ *    the order is the sequential one, so results are bitwise reproducible run to run);
+ *  - a slot or row index of -1 means "skip"; only the upper triangle (altId_j >= altId_i) is
+ *    written; blocks are row-major 3x3, A[slot*9 + 3k + l];
+ *  - every call is blocking (synchronous on the context's stream), like the reference's Run().
+ *
+ * All functions return 0 on success, otherwise a positive hipError_t value or a negative
+ * ICF_E_* code; icf_last_error_string() describes the last failure on that context.
+ * A context is not re-entrant: one host thread at a time.
+ */
+#ifndef ICFLOW_HIP_H
+#define ICFLOW_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICF_OK 0
+#define ICF_E_INVALID (-1)      /* bad argument (null pointer, negative count, bad stride)  */
+#define ICF_E_NOCONST (-2)      /* icf_set_constants has not been called                    */
+#define ICF_E_CAPACITY (-3)     /* more units than the 24-bit unit field of the slot maps   */
+#define ICF_E_CONSTANTS (-4)    /* constants outside the supported form (see icf_consts)    */
+#define ICF_E_INDEX (-5)        /* a slot / row / node index beyond the declared size       */
+
+#define ICF_FP64 0              /* plain FP64 throughout                                    */
+#define ICF_MIXED 1             /* FP32 tangent blocks, FP64 residual, law and accumulation */
+
+typedef struct icf_ctx icf_ctx;
+
+/* Model constants, the __constant__ symbols that GPU_Functionality.SetConstants uploads by
+ * name (GPU_Functionality.cs:151-194; elem_kernels9/kernel.cu:6-9; cz_kernels9/kernel.cu:5-20),
+ * in the layout of the host arrays ModelPrms.SetComputedVariables builds
+ * (v3Library/Simulation/DataSheets/ModelPrms.cs:237-322).
+ * Supported form: E symmetric; B[gp][p][k] = +sf[k/3][gp] (k<9) or -sf[k/3-3][gp] (k>=9) when
+ * p == k%3 and 0 otherwise, which is what ModelPrms.k_Bmatrix (ModelPrms.cs:324-349) produces. */
+typedef struct icf_consts {
+    double E[36];          /* 6x6 elasticity matrix, E[i*6+j]                    */
+    double M[144];         /* 12x12 consistent mass per unit volume, M[i*12+j]   */
+    double NewmarkBeta, NewmarkGamma, dampingMass, dampingStiffness, rho, gravity;
+    double YoungsModulus;
+    double G_fn, G_ft, f_tn, f_tt, alpha, beta, rn, rt;
+    double p_m, p_n, deln, delt, pMtn, pMnt, gam_n, gam_t;
+    double B[162];         /* CZ B[gp][3][18]                                     */
+    double sf[9];          /* CZ shape functions at the Gauss points, sf[node*3+gp] */
+} icf_consts;
+
+/* --- context (replaces new CudaContext + LoadModulePTX, GPU_Functionality.cs:112-137) --- */
+int icf_create(int device, icf_ctx **out);
+int icf_destroy(icf_ctx *ctx);
+const char *icf_last_error_string(const icf_ctx *ctx);
+int icf_set_constants(icf_ctx *ctx, const icf_consts *c);     /* SetConstants, :151-194    */
+int icf_set_precision(icf_ctx *ctx, int mode);                 /* ICF_FP64 | ICF_MIXED      */
+int icf_get_precision(const icf_ctx *ctx);
+
+/* --- device memory (replaces CudaDeviceVariable<T>, :211-248; CopyToDevice/CopyToHost;
+ *     MemsetAsync(0, NullStream), :433-434) --- */
+int icf_alloc(icf_ctx *ctx, size_t bytes, void **dptr);
+int icf_free(icf_ctx *ctx, void *dptr);
+int icf_h2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t bytes);
+int icf_d2h(icf_ctx *ctx, void *dst, const void *src, size_t src_off, size_t bytes);
+int icf_memset(icf_ctx *ctx, void *dst, int value, size_t bytes);
+int icf_synchronize(icf_ctx *ctx);
+
+/* --- the five reference kernels, same argument lists (plus the context) --- */
+
+/* kelElementElasticityForce (elem_kernels9/element_kernels2.h:195-263; launch
+ * GPU_Functionality.cs:448-454): corotational tet, Newmark effective tangent + residual.
+ * ie: int SoA 28 x el_stride (0-3 node ids, 4-19 slots, 20-23 rhs rows). */
+int icf_elem_force(icf_ctx *ctx, const int *ie, double *dn, double h, double *A, double *b,
+                   int nElems, int el_stride, int nd_stride);
+
+/* kczCZForce (cz_kernels9/cz_kernels.h:210-434; launch GPU_Functionality.cs:463-468):
+ * PPR cohesive zone, 3 Gauss points; writes tentative pmax/tmax/averages and flags.
+ * icz: int SoA 58 x cz_stride; dcz: double SoA 16 x cz_stride. */
+int icf_cz_force(icf_ctx *ctx, double *dcz, int *icz, double *dn, double *A, double *b, double h,
+                 int nCZs, int cz_stride, int nd_stride);
+
+/* kNarrowPhase_new (collision_kernels9/narrowphase.h:46-155; launch GPU_Functionality.cs:557-559):
+ * narrowList[2i], [2i+1] = surface-tet ids in, narrowList[2i] = 8-bit containment mask out. */
+int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, int *narrowList,
+                     int el_all_stride, int nd_stride);
+
+/* kFindClosestFace (narrowphase.h:159-201; launch GPU_Functionality.cs:602-607):
+ * nl2 SoA 4 x tet_stride, rows (node, tet) in -> rows 1..3 = closest face's node ids out. */
+int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, const double *dn,
+                          const int *ie, const int *faces, int nd_stride, int el_all_stride,
+                          int fc_stride);
+
+/* kCollisionResponseForce (collision_kernels9/collision_response.h:38-126; launch
+ * GPU_Functionality.cs:668-672): point-triangle penalty force and tangent.
+ * icr: int SoA 28 x cr_stride (0-3 node ids p,v0,v1,v2; 4-19 slots; 20-23 rows). */
+int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, double *A,
+                           double *b, int nImpacts, int cr_stride, int nd_stride, double k,
+                           double distanceEpsilon);
+
+/* --- composite calls of the reference's host methods --- */
+
+/* AssembleElemsAndCZs (GPU_Functionality.cs:439-479 with AllocateMemoryForLinearSystem
+ * :419-437): zero A[0..nnzb*9) and b[0..N*3), element pass, CZ pass, then the two flag sums
+ * (tentative damaged / failed counts, :470-475).  Each BCSR block and rhs entry is written
+ * once.  nCZs may be 0 (dcz/icz may then be NULL).  Count outputs may be NULL. */
+int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_stride,
+                               double *dcz, int *icz, int nCZs, int cz_stride,
+                               double *dn, int nd_stride, double h,
+                               double *A, int nnzb, double *b, int N,
+                               int *nCZDamaged, int *nCZFailed);
+
+/* Sum of n ints in device memory (GPU_Functionality.Sum, :93-107). */
+int icf_sum_int(icf_ctx *ctx, const int *d, int n, int *out);
+
+/* The library caches, per pcsr buffer, the slot -> contribution lists of its deterministic
+ * reduction.  Writes through icf_h2d / icf_memset / icf_free invalidate the lists of the
+ * buffers they touch; call this after writing pcsr arrays by any other means. */
+int icf_pattern_changed(icf_ctx *ctx);
+
+/* Time, in milliseconds, of the last call's device work (HIP events on the context's
+ * stream); -1 if the last call launched nothing. */
+double icf_last_device_ms(const icf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,136 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle on seeded specimens.
+
+Tolerances: FP64 mode, 1e-9 normwise relative per assembled array (the assembled BCSR
+values, rhs and node forces); integer outputs (flags, masks, closest faces) bit-exact.
+Mixed mode: tangent blocks within 1e-5 normwise relative (FP32 blocks, FP64 accumulation),
+residual and node forces within 1e-9 (computed in FP64).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from icflow_amd import _lib
+from workloads import H, Pipeline, make_state, oracle_assemble, small_specimen
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-9
+
+
+@pytest.fixture(scope="module")
+def col():
+    return small_specimen()
+
+
+def _compare(got, ref, tol_vals=TOL64, tol_rhs=TOL64):
+    errs = dict(vals=rel_err(got["vals"], ref["vals"]), rhs=rel_err(got["rhs"], ref["rhs"]),
+                F=rel_err(got["F"], ref["F"]))
+    print("normwise rel errors:", errs, flush=True)
+    assert errs["vals"] <= tol_vals
+    assert errs["rhs"] <= tol_rhs
+    assert errs["F"] <= tol_rhs
+    return errs
+
+
+def _cz_fields(gf, got, ref, C):
+    cs = gf.cz_stride
+    gi, ri = got["icz"], ref["icz"]
+    for f in (1, 2, 3):   # tentative contact, damaged, failed: bit-exact
+        assert np.array_equal(gi[f * cs:f * cs + C], ri[f * cs:f * cs + C]), f"icz field {f}"
+    gd, rd = got["dcz"], ref["dcz"]
+    for f in range(6, 12):   # tentative pmax / tmax
+        assert rel_err(gd[f * cs:f * cs + C], rd[f * cs:f * cs + C]) <= 1e-12, f"dcz field {f}"
+    for f in range(12, 16):  # averages (written for non-failed CZs)
+        assert rel_err(gd[f * cs:f * cs + C], rd[f * cs:f * cs + C]) <= TOL64, f"dcz field {f}"
+
+
+def test_fused_assembly_fp64_matches_oracle(col, orc, specimen_consts):
+    st = make_state(col, specimen_consts, seed=1)
+    p = Pipeline(col, specimen_consts, st)
+    got = p.assemble()
+    ref = oracle_assemble(orc, p)
+    _compare(got, ref)
+    _cz_fields(p.gf, got, ref, len(col.czs))
+    C = len(col.czs)
+    cs = p.gf.cz_stride
+    assert got["counts"] == (int(ref["icz"][2 * cs:2 * cs + C].sum()), int(ref["icz"][3 * cs:3 * cs + C].sum()))
+    # the state exercises the failure branch and keeps most CZs alive
+    assert 0 < got["counts"][1] < C
+    p.gf.close()
+
+
+def test_separate_kernels_accumulate_like_reference(col, orc, specimen_consts):
+    """icf_elem_force then icf_cz_force on zeroed buffers = the reference's two launches."""
+    st = make_state(col, specimen_consts, seed=2)
+    p = Pipeline(col, specimen_consts, st)
+    gf, lib = p.gf, p.gf.lib
+    nnz, N = gf.csr.nnz, gf.csr.N
+    vals = gf.buf("vals", 2 * nnz * 9, np.float64)
+    rhs = gf.buf("rhs", N * 3, np.float64)
+    vals.zero()
+    rhs.zero()
+    _lib.check(lib.icf_elem_force(gf.ctx, gf.g_ie_pcsr.p, gf.g_dn.p, H, vals.p, rhs.p, len(col.elastic),
+                                  gf.el_elastic_stride, gf.nd_stride), gf.ctx, "icf_elem_force")
+    _lib.check(lib.icf_cz_force(gf.ctx, gf.g_dcz.p, gf.g_icz.p, gf.g_dn.p, vals.p, rhs.p, H, len(col.czs),
+                                gf.cz_stride, gf.nd_stride), gf.ctx, "icf_cz_force")
+    got = dict(vals=vals.download(nnz * 9), rhs=rhs.download(N * 3),
+               F=gf.g_dn.download(3 * gf.nd_stride, offset_elems=12 * gf.nd_stride))
+    ref = oracle_assemble(orc, p)
+    _compare(got, ref)
+    gf.close()
+
+
+def test_assembly_is_bitwise_deterministic(col, specimen_consts):
+    st = make_state(col, specimen_consts, seed=3)
+    p = Pipeline(col, specimen_consts, st)
+    a = p.assemble()
+    p.gf.TransferNodesToGPU(st)          # F back to zero, same inputs
+    p.gf.TransferPCSR(st)                # re-upload: rebuilds the slot maps
+    b = p.assemble()
+    for k in ("vals", "rhs", "F"):
+        assert np.array_equal(a[k], b[k]), k
+    p.gf.close()
+
+
+def test_contact_pipeline_matches_oracle(orc, specimen_consts):
+    col = small_specimen(cells=(5, 4, 4), size=(0.05, 0.04, 0.04), grains=5, seed=4)
+    st = make_state(col, specimen_consts, seed=5, platen_dz_frac=0.3)
+    p = Pipeline(col, specimen_consts, st, contact=True)
+    gf = p.gf
+    # narrow phase: masks bit-exact against the oracle on the same candidate pairs
+    Q = len(p.pairs)
+    nl = p.pairs.astype(np.int32).ravel().copy()
+    orc.narrow_phase(Q, gf.c_dn, gf.c_ie, nl, gf.el_all_stride, gf.nd_stride)
+    assert np.array_equal(nl[0::2], gf.masks)
+    assert np.count_nonzero(gf.masks) > 0
+    # closest face: bit-exact
+    ts = gf.tet_stride
+    nT = gf.nImpacts
+    # rebuild the (node, tet) input rows from NL2set
+    masks2 = np.zeros(2 * Q, np.int32)
+    masks2[0::2] = gf.masks
+    out = np.zeros(4 * ts, np.int32)
+    n = ctypes.c_int()
+    _lib.check(gf.lib.icfh_nl2set(_lib.ptr(masks2), _lib.ptr(p.pairs.astype(np.int32).ravel().copy()), Q,
+                                  _lib.ptr(gf.c_ie), gf.el_all_stride, _lib.ptr(out), ts, ts, ctypes.byref(n)))
+    assert n.value == nT
+    orc.find_closest_face(nT, ts, out, gf.c_dn, gf.c_ie, gf.c_ifc, gf.nd_stride, gf.el_all_stride, gf.fc_stride)
+    for k in range(4):
+        assert np.array_equal(out[k * ts:k * ts + nT], gf.c_itet[k * ts:k * ts + nT]), k
+    # full iteration with contact response
+    got = p.assemble()
+    ref = oracle_assemble(orc, p)
+    _compare(got, ref)
+    gf.close()
+
+
+def test_mixed_precision_tolerances(col, orc, specimen_consts):
+    st = make_state(col, specimen_consts, seed=6)
+    p = Pipeline(col, specimen_consts, st, precision=_lib.ICF_MIXED)
+    got = p.assemble()
+    ref = oracle_assemble(orc, p)
+    errs = _compare(got, ref, tol_vals=1e-5, tol_rhs=TOL64)
+    assert errs["vals"] > 0          # really ran in FP32
+    _cz_fields(p.gf, got, ref, len(col.czs))
+    p.gf.close()
