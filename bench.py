@@ -1,0 +1,299 @@
+#!/usr/bin/env python3
+"""Newton-iteration assembly throughput of libicflow_hip.so on one MI355X.
+
+Workload (default): uniaxial compression with contact -- a 0.1 m cube as a 40x40x40 hex grid
+split into 6 tets per hex (384,000 tets), 500 Voronoi grains (seed 1) with PPR cohesive zones
+on every inter-grain face, and two rigid one-cell-thick tet platens (32x32 cells) pressed
+0.3 cell into the cube.  Material E=9 GPa, nu=0.3, rho=916; PPR sigma_max=1 MPa,
+tau_max=2 MPa, phi=10 J/m^2, alpha=beta=4, lambda=0.02.  Synthetic seeded state.
+
+One step = one Newton iteration's device work with every input already in HBM:
+narrow phase over the broad-phase candidate pairs, closest face for the (node, tet) tuples,
+the fused element + CZ assembly (AssembleElemsAndCZs) and the contact response.  The in-place
+inputs of the two narrow-phase kernels are restored by device-to-device copies before each
+step.  Throughput = (tets + active CZs + impacts) / step time.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(cells, grains, seed, platen_frac):
+    from icflow_amd import _lib, specimens
+    consts = _lib.consts_from_prms(**_lib.SPECIMEN_PRMS)
+    size = 0.1
+    t = time.time()
+    col = specimens.make_specimen((cells, cells, cells), (size, size, size), grains, seed,
+                                  platens=(32, 32)).finalize()
+    hz = size / cells
+    st = specimens.synthetic_state(col, consts, seed, platen_dz=platen_frac * hz, equal_gp=True)
+    log(f"specimen: {len(col.x0)} nodes, {len(col.elastic)} tets, {len(col.czs)} CZs, "
+        f"{len(col.surf)} surface tets ({time.time() - t:.1f} s)")
+    return consts, col, st
+
+
+class Runner:
+    """Device state for the timed step, built once through the GPU_Functionality mirror."""
+
+    def __init__(self, consts, col, st, mode):
+        from icflow_amd import _lib
+        from icflow_amd.gpu_functionality import GPUFunctionality, broad_phase, stride_of
+        self.lib = _lib.load()
+        self.check = _lib.check
+        gf = self.gf = GPUFunctionality(0)
+        gf.SetConstants(consts)
+        gf.set_precision(mode)
+        gf.TransferStaticDataToDevice(col)
+        gf.TransferNodesToGPU(st)
+        t = time.time()
+        self.pairs = broad_phase(gf, gf.c_dn, gf.nd_stride, gf.c_ie, gf.el_all_stride, len(col.surf))
+        log(f"broad phase: {len(self.pairs)} candidate pairs ({time.time() - t:.1f} s)")
+        impacts = gf.NarrowPhaseCollisionDetection(self.pairs)
+        self.nT = gf.nImpacts
+        log(f"narrow phase: {int(np.count_nonzero(gf.masks))} pairs with containment, {self.nT} impacts")
+        t = time.time()
+        gf.CreateStructure(impacts)
+        gf.TransferPCSR(st)
+        log(f"pattern: N={gf.csr.N} nnzb={gf.csr.nnz} ({time.time() - t:.1f} s)")
+        # pristine copies of the in-place inputs of the narrow-phase kernels
+        Q = len(self.pairs)
+        self.Q = Q
+        self.itet = gf.buf("itet", 2 * Q, np.int32)
+        self.itet_src = gf.buf("itet_src", 2 * Q, np.int32)
+        self.itet_src.upload(self.pairs.ravel())
+        ts = gf.tet_stride
+        self.nl2 = gf.buf("itet2", 4 * ts, np.int32)
+        self.nl2_src = gf.buf("itet2_src", 2 * ts, np.int32)
+        # rows 0-1 of the closest-face buffer are the (node, tet) tuples of NL2set
+        tup = np.zeros(2 * ts, np.int32)
+        masks = np.zeros(2 * Q, np.int32)
+        masks[0::2] = gf.masks
+        n = ctypes.c_int()
+        self.check(self.lib.icfh_nl2set(_ptr(masks), _ptr(self.pairs.astype(np.int32).ravel().copy()), Q,
+                                        _ptr(gf.c_ie), gf.el_all_stride, _ptr(tup), ts, ts, ctypes.byref(n)))
+        self.nl2_src.upload(tup)
+        # impacts: icr packed once (same impacts every step)
+        gf.c_icr, gf.cr_stride = gf.pack_icr(impacts)
+        self.icr = gf.buf("icr", len(gf.c_icr), np.int32)
+        self.icr.upload(gf.c_icr)
+        nnz, N = gf.csr.nnz, gf.csr.N
+        self.vals = gf.buf("vals", 2 * nnz * 9, np.float64)
+        self.rhs = gf.buf("rhs", N * 3, np.float64)
+        self.col = col
+        self.h = 5e-5
+        self.k = 1e11 * (0.1 / 32) ** 2
+        self.eps = 1e-15
+
+    def step(self):
+        gf, lib, ctx, chk = self.gf, self.lib, self.gf.ctx, self.check
+        chk(lib.icf_d2d(ctx, self.itet.p, 0, self.itet_src.p, 0, 8 * self.Q), ctx, "icf_d2d")
+        chk(lib.icf_d2d(ctx, self.nl2.p, 0, self.nl2_src.p, 0, 8 * gf.tet_stride), ctx, "icf_d2d")
+        chk(lib.icf_narrow_phase(ctx, self.Q, gf.g_dn.p, gf.g_ie.p, self.itet.p, gf.el_all_stride, gf.nd_stride),
+            ctx, "icf_narrow_phase")
+        chk(lib.icf_find_closest_face(ctx, self.nT, gf.tet_stride, self.nl2.p, gf.g_dn.p, gf.g_ie.p, gf.g_ifc.p,
+                                      gf.nd_stride, gf.el_all_stride, gf.fc_stride), ctx, "icf_find_closest_face")
+        dmg, fld = ctypes.c_int(), ctypes.c_int()
+        chk(lib.icf_assemble_elems_and_czs(
+            ctx, gf.g_ie_pcsr.p, len(self.col.elastic), gf.el_elastic_stride, gf.g_dcz.p, gf.g_icz.p,
+            len(self.col.czs), gf.cz_stride, gf.g_dn.p, gf.nd_stride, self.h, self.vals.p, gf.csr.nnz,
+            self.rhs.p, gf.csr.N, ctypes.byref(dmg), ctypes.byref(fld)), ctx, "icf_assemble_elems_and_czs")
+        chk(lib.icf_collision_response(ctx, self.icr.p, gf.g_dn.p, self.h, self.vals.p, self.rhs.p, self.nT,
+                                       gf.cr_stride, gf.nd_stride, self.k, self.eps), ctx, "icf_collision_response")
+        self.failed = fld.value
+
+    def set_mode(self, mode):
+        self.gf.set_precision(mode)
+
+    def profile(self):
+        lib, ctx = self.lib, self.gf.ctx
+        out = {}
+        buf = ctypes.create_string_buffer(128)
+        for i in range(lib.icf_profile_count(ctx)):
+            ms, n = ctypes.c_double(), ctypes.c_longlong()
+            lib.icf_profile_entry(ctx, i, buf, 128, ctypes.byref(ms), ctypes.byref(n))
+            out[buf.value.decode()] = (ms.value, n.value)
+        return out
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def timed(runner, steps, warmup, profile):
+    lib, ctx = runner.lib, runner.gf.ctx
+    for _ in range(warmup):
+        runner.step()
+    lib.icf_synchronize(ctx)
+    lib.icf_profile_reset(ctx)
+    lib.icf_set_profiling(ctx, 1 if profile else 0)
+    lib.icf_synchronize(ctx)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        runner.step()
+        if (i + 1) % max(1, steps // 4) == 0:
+            log(f"  step {i + 1}/{steps}")
+    lib.icf_synchronize(ctx)
+    dt = time.perf_counter() - t0
+    lib.icf_set_profiling(ctx, 0)
+    return dt / steps, runner.profile()
+
+
+def algorithmic_bytes(r):
+    """Compulsory HBM bytes per launch of each kernel (every byte counted once; staging records
+    and slot maps are implementation traffic and are not counted)."""
+    col, gf = r.col, r.gf
+    T, C, P, Q = len(col.elastic), len(col.czs), r.nT, r.Q
+    nnzb, N = gf.csr.nnz, gf.csr.N
+    n_def = int(np.unique(col.elastic).size)
+    n_cz = int(np.unique(col.czs).size) if C else 0
+    n_surf = int(np.unique(col.surf).size)
+    adj = gf.maxAdjacentFaces
+    return {
+        # ids 16 B/tet + x0,un,vn,an 96 B per node
+        "k_tet_phase1": 16 * T + 96 * n_def,
+        # failed flag + 6 ids + pmax/tmax in (76 B), tentative pmax/tmax/averages + 3 flags out (92 B);
+        # x0, un per CZ node (48 B)
+        "k_cz_phase1": 168 * C + 48 * n_cz,
+        # BCSR values written once (72 B/block) + the slot fields the reference scatter reads
+        "k_reduce_vals": 72 * nnzb + 64 * T + 144 * C,
+        "k_reduce_vec3<rhs>": 24 * N + 16 * T + 24 * C,
+        "k_reduce_vec3<force>": 48 * n_def + 16 * T,
+        # 2 tet ids in, mask out (12 B/pair) + surface-tet node ids 16 B + x_current 24 B/node
+        "k_narrow_phase": 12 * Q + 16 * len(col.surf) + 24 * n_surf,
+        # tuple in/out 24 B + adjacency count and ids + face ids
+        "k_closest_face": 24 * P + 4 * P * (1 + adj),
+        "k_contact_phase1": 96 * P + 96 * P,
+        "k_sum_int": 4 * C,
+    }
+
+
+def cpu_baseline(runner, seconds):
+    """The CPU oracle (plain C, one core) over the same iteration's element, CZ and contact
+    passes, repeated for about `seconds`."""
+    from oracle import oracle as orc
+    gf, col = runner.gf, runner.col
+    orc.set_constants(gf.consts)
+    T, C, P = len(col.elastic), len(col.czs), runner.nT
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        dn = gf.c_dn.copy()
+        A = np.zeros(gf.csr.nnz * 9)
+        b = np.zeros(gf.csr.N * 3)
+        ie, icz, dcz = gf.c_ie_pcsr.copy(), gf.c_icz.copy(), gf.c_dcz.copy()
+        orc.elem_force(ie, dn, runner.h, A, b, T, gf.el_elastic_stride, gf.nd_stride)
+        orc.cz_force(dcz, icz, dn, A, b, runner.h, C, gf.cz_stride, gf.nd_stride)
+        orc.collision_response(gf.c_icr, dn, runner.h, A, b, P, gf.cr_stride, gf.nd_stride, runner.k, runner.eps)
+        passes += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or passes >= 50:
+            break
+    units = T + C + P
+    return dict(value=round(units * passes / el / 1e6, 4), unit="Munits/s", cores=1, kind="port",
+                sample=f"{passes} full passes of the same iteration (elements, CZs, contacts) through the "
+                       f"C oracle, single thread, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=40)
+    ap.add_argument("--grains", type=int, default=500)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--platen-frac", type=float, default=0.3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from icflow_amd import _lib
+    consts, col, st = build_workload(args.cells, args.grains, args.seed, args.platen_frac)
+    r = Runner(consts, col, st, _lib.ICF_MIXED)
+
+    # correctness guard of the two modes on this workload (assembled arrays, normwise)
+    res = {}
+    for mode in (_lib.ICF_FP64, _lib.ICF_MIXED):
+        r.set_mode(mode)
+        r.gf.TransferNodesToGPU(st)        # F back to zero
+        r.step()
+        res[mode] = (r.vals.download(r.gf.csr.nnz * 9), r.rhs.download(r.gf.csr.N * 3))
+    errs = {}
+    for i, k in enumerate(("vals", "rhs")):
+        ref = res[_lib.ICF_FP64][i]
+        errs[k] = float(np.max(np.abs(res[_lib.ICF_MIXED][i] - ref)) / max(np.max(np.abs(ref)), 1e-300))
+    log(f"mixed vs fp64 normwise rel err: {errs}")
+
+    out = {}
+    for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
+        r.set_mode(mode)
+        log(f"timing {name}: {args.warmup} warmup + {args.steps} steps")
+        sec, prof = timed(r, args.steps, args.warmup, profile=True)
+        out[name] = (sec, prof)
+        log(f"  {name}: {sec * 1e3:.3f} ms/step")
+        for kname, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+            log(f"    {kname:28s} {1e3 * ms / max(n, 1):9.1f} us/launch  x{n}")
+
+    T, C, P = len(col.elastic), len(col.czs), r.nT
+    active_C = C - r.failed
+    units = T + active_C + P
+    sec, prof = out["mixed"]
+    alg = algorithmic_bytes(r)
+    # dominant kernel of the mixed step and its roofline
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dn) = dom
+    base = dname.split("<")[0] if not dname.startswith("k_reduce_vec3") else dname
+    dbytes = alg.get(base, alg.get(dname))
+    avg_s = dms / dn / 1e3
+    achieved = dbytes / avg_s / 1e9
+    roof = dict(kernel=dname, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes=int(dbytes),
+                avg_us=round(avg_s * 1e6, 2))
+
+    cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "Newton-iter assembly Melem+CZ/s, mixed vs FP64 at <=1e-5 rel err; HBM %peak",
+        "value": round(units / sec / 1e6, 3),
+        "unit": "Melem+CZ/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"compression with contact: cube {args.cells}^3 hex x 6 tets, {args.grains} "
+                               f"grains (seed {args.seed}), PPR CZs on inter-grain faces, 2 rigid 32x32 platens",
+                   "tets": T, "czs": C, "active_czs": active_C, "impacts": P, "candidate_pairs": r.Q,
+                   "nodes": len(col.x0), "rows": r.gf.csr.N, "nnzb": r.gf.csr.nnz, "mode": "mixed"},
+        "fp64": {"value": round(units / out["fp64"][0] / 1e6, 3), "ms_per_step": round(out["fp64"][0] * 1e3, 4)},
+        "mixed_over_fp64": round(out["fp64"][0] / sec, 3),
+        "mixed_rel_err_vs_fp64": errs,
+        "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    r.gf.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,12 @@ _SIGS = {
     "icf_sum_int": (c_int, [c_void_p, c_void_p, c_int, P_int]),
     "icf_pattern_changed": (c_int, [c_void_p]),
     "icf_last_device_ms": (c_double, [c_void_p]),
+    "icf_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t]),
+    "icf_set_profiling": (c_int, [c_void_p, c_int]),
+    "icf_profile_reset": (c_int, [c_void_p]),
+    "icf_profile_count": (c_int, [c_void_p]),
+    "icf_profile_entry": (c_int, [c_void_p, c_int, ctypes.c_char_p, c_int, ctypes.POINTER(c_double),
+                                  ctypes.POINTER(ctypes.c_longlong)]),
     # host helpers
     "icfh_compute_constants": (c_int, [ctypes.POINTER(IcfModelPrms), ctypes.POINTER(IcfConsts)]),
     "icfh_csr_create": (c_int, [ctypes.POINTER(c_void_p)]),

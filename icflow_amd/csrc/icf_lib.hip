@@ -74,7 +74,17 @@ struct icf_ctx {
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     std::string err;
     double last_ms = -1;
-    bool timing = false;
+    // per-kernel profiling (icf_set_profiling)
+    bool profiling = false;
+    struct Pending {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<long long> prof_n;
 };
 
 namespace {
@@ -147,6 +157,61 @@ int upload_consts(icf_ctx *ctx)
 }
 
 unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipEvent_t prof_begin(icf_ctx *ctx)
+{
+    if (!ctx->profiling) return nullptr;
+    hipEvent_t e = nullptr;
+    if (!ctx->ev_pool.empty()) {
+        e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    (void)hipEventRecord(e, ctx->stream);
+    return e;
+}
+
+void prof_end(icf_ctx *ctx, const char *name, hipEvent_t a)
+{
+    if (!a) return;
+    hipEvent_t b = prof_begin(ctx);
+    if (!b) {
+        ctx->ev_pool.push_back(a);
+        return;
+    }
+    ctx->pending.push_back({name, a, b});
+}
+
+// After the stream has drained: fold pending event pairs into the per-kernel totals.
+void prof_flush(icf_ctx *ctx)
+{
+    for (auto &p : ctx->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            size_t i = 0;
+            while (i < ctx->prof_names.size() && ctx->prof_names[i] != p.name) i++;
+            if (i == ctx->prof_names.size()) {
+                ctx->prof_names.push_back(p.name);
+                ctx->prof_ms.push_back(0);
+                ctx->prof_n.push_back(0);
+            }
+            ctx->prof_ms[i] += ms;
+            ctx->prof_n[i] += 1;
+        }
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+}
+
+// Kernel launch on the context's stream, bracketed by events when profiling is on.
+#define LAUNCH(ctx, name, kern, g, b, ...)                                   \
+    do {                                                                     \
+        hipEvent_t ea_ = prof_begin(ctx);                                    \
+        hipLaunchKernelGGL(kern, g, b, 0, (ctx)->stream, __VA_ARGS__);       \
+        prof_end(ctx, name, ea_);                                            \
+    } while (0)
 
 // Build (or fetch) the slot map for `srcs` with keys of `target`.
 int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey> &srcs, SlotMap **out)
@@ -249,6 +314,7 @@ struct Timer {
         float ms = 0;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->last_ms = ms;
+        prof_flush(ctx);
         HIPCHK(ctx, hipGetLastError());
         return 0;
     }
@@ -266,10 +332,10 @@ int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int
     dim3 g(blocks_for(n, 256)), b(256);
     double *sr = (double *)ctx->tet_rhs.p;
     void *st = ctx->tet_tan.p;
-    if (mixed && iso) hipLaunchKernelGGL((k_tet_phase1<true, true>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
-    else if (mixed) hipLaunchKernelGGL((k_tet_phase1<true, false>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
-    else if (iso) hipLaunchKernelGGL((k_tet_phase1<false, true>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
-    else hipLaunchKernelGGL((k_tet_phase1<false, false>), g, b, 0, ctx->stream, ie, dn, n, es, ns, sr, st);
+    if (mixed && iso) LAUNCH(ctx, "k_tet_phase1<mixed,iso>", (k_tet_phase1<true, true>), g, b, ie, dn, n, es, ns, sr, st);
+    else if (mixed) LAUNCH(ctx, "k_tet_phase1<mixed,gen>", (k_tet_phase1<true, false>), g, b, ie, dn, n, es, ns, sr, st);
+    else if (iso) LAUNCH(ctx, "k_tet_phase1<fp64,iso>", (k_tet_phase1<false, true>), g, b, ie, dn, n, es, ns, sr, st);
+    else LAUNCH(ctx, "k_tet_phase1<fp64,gen>", (k_tet_phase1<false, false>), g, b, ie, dn, n, es, ns, sr, st);
     return 0;
 }
 
@@ -281,11 +347,11 @@ int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int 
     if (n == 0) return 0;
     dim3 g(blocks_for(n, 256)), b(256);
     if (mixed)
-        hipLaunchKernelGGL((k_cz_phase1<true>), g, b, 0, ctx->stream, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
-                           ctx->cz_tan.p);
+        LAUNCH(ctx, "k_cz_phase1<mixed>", (k_cz_phase1<true>), g, b, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
+               ctx->cz_tan.p);
     else
-        hipLaunchKernelGGL((k_cz_phase1<false>), g, b, 0, ctx->stream, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
-                           ctx->cz_tan.p);
+        LAUNCH(ctx, "k_cz_phase1<fp64>", (k_cz_phase1<false>), g, b, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
+               ctx->cz_tan.p);
     return 0;
 }
 
@@ -308,18 +374,19 @@ void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, cons
     if (m->nkeys == 0) return;
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     dim3 g(blocks_for(m->nkeys, 256)), b(256);
-    if (mixed && iso) hipLaunchKernelGGL((k_reduce_vals<true, true>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
-    else if (mixed) hipLaunchKernelGGL((k_reduce_vals<true, false>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
-    else if (iso) hipLaunchKernelGGL((k_reduce_vals<false, true>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
-    else hipLaunchKernelGGL((k_reduce_vals<false, false>), g, b, 0, ctx->stream, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    if (mixed && iso) LAUNCH(ctx, "k_reduce_vals<mixed,iso>", (k_reduce_vals<true, true>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    else if (mixed) LAUNCH(ctx, "k_reduce_vals<mixed,gen>", (k_reduce_vals<true, false>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    else if (iso) LAUNCH(ctx, "k_reduce_vals<fp64,iso>", (k_reduce_vals<false, true>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    else LAUNCH(ctx, "k_reduce_vals<fp64,gen>", (k_reduce_vals<false, false>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
 }
 
 void launch_vec3(icf_ctx *ctx, const SlotMap *m, double *out, int comp_stride, int elem_stride, int accumulate,
                  const Stage &sv)
 {
     if (m->nkeys == 0) return;
-    hipLaunchKernelGGL(k_reduce_vec3, dim3(blocks_for(m->nkeys, 256)), dim3(256), 0, ctx->stream, m->off, m->codes,
-                       m->nkeys, out, comp_stride, elem_stride, accumulate, sv);
+    LAUNCH(ctx, elem_stride == 1 ? "k_reduce_vec3<force>" : "k_reduce_vec3<rhs>", k_reduce_vec3,
+           dim3(blocks_for(m->nkeys, 256)), dim3(256), m->off, m->codes, m->nkeys, out, comp_stride, elem_stride,
+           accumulate, sv);
 }
 
 bool bad_units(int n, int stride) { return n < 0 || stride < 0 || (n > 0 && stride < n); }
@@ -368,6 +435,11 @@ int icf_destroy(icf_ctx *ctx)
     if (ctx->small) (void)hipFree(ctx->small);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    for (auto &p : ctx->pending) {
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
     delete ctx;
@@ -520,6 +592,48 @@ int icf_pattern_changed(icf_ctx *ctx)
 
 double icf_last_device_ms(const icf_ctx *ctx) { return ctx ? ctx->last_ms : -1; }
 
+int icf_d2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t src_off, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return ICF_E_INVALID;
+    if (int r = check_ctx(ctx)) return r;
+    if (!bytes) return 0;
+    char *d = (char *)dst + dst_off;
+    invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
+    HIPCHK(ctx, hipMemcpyAsync(d, (const char *)src + src_off, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_set_profiling(icf_ctx *ctx, int on)
+{
+    if (!ctx) return ICF_E_INVALID;
+    ctx->profiling = on != 0;
+    return 0;
+}
+
+int icf_profile_reset(icf_ctx *ctx)
+{
+    if (!ctx) return ICF_E_INVALID;
+    ctx->prof_names.clear();
+    ctx->prof_ms.clear();
+    ctx->prof_n.clear();
+    return 0;
+}
+
+int icf_profile_count(const icf_ctx *ctx) { return ctx ? (int)ctx->prof_names.size() : ICF_E_INVALID; }
+
+int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches)
+{
+    if (!ctx || i < 0 || i >= (int)ctx->prof_names.size()) return ICF_E_INVALID;
+    if (name && cap > 0) {
+        strncpy(name, ctx->prof_names[i].c_str(), (size_t)cap - 1);
+        name[cap - 1] = 0;
+    }
+    if (total_ms) *total_ms = ctx->prof_ms[i];
+    if (launches) *launches = ctx->prof_n[i];
+    return 0;
+}
+
 int icf_elem_force(icf_ctx *ctx, const int *ie, double *dn, double h, double *A, double *b, int nElems, int el_stride,
                    int nd_stride)
 {
@@ -573,8 +687,8 @@ int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, 
         return fail(ctx, ICF_E_INVALID, "icf_narrow_phase: invalid argument");
     if (nPairs == 0) return 0;
     Timer tm(ctx);
-    hipLaunchKernelGGL(k_narrow_phase, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, nPairs, dn, ie,
-                       narrowList, el_all_stride, nd_stride);
+    LAUNCH(ctx, "k_narrow_phase", k_narrow_phase, dim3(blocks_for(nPairs, 256)), dim3(256), nPairs, dn, ie,
+           narrowList, el_all_stride, nd_stride);
     return tm.finish();
 }
 
@@ -587,8 +701,8 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
         return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: invalid argument");
     if (nPairs == 0) return 0;
     Timer tm(ctx);
-    hipLaunchKernelGGL(k_closest_face, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, nPairs, tet_stride,
-                       nl2, dn, ie, faces, nd_stride, el_all_stride, fc_stride);
+    LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for(nPairs, 256)), dim3(256), nPairs, tet_stride,
+           nl2, dn, ie, faces, nd_stride, el_all_stride, fc_stride);
     return tm.finish();
 }
 
@@ -609,8 +723,8 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
     if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
     if (int r = grow(ctx, ctx->ct, sizeof(double) * CT_REC * (size_t)nImpacts)) return r;
     Timer tm(ctx);
-    hipLaunchKernelGGL(k_contact_phase1, dim3(blocks_for(nImpacts, 128)), dim3(128), 0, ctx->stream, icr, dn,
-                       nImpacts, cr_stride, nd_stride, k, distanceEpsilon, (double *)ctx->ct.p);
+    LAUNCH(ctx, "k_contact_phase1", k_contact_phase1, dim3(blocks_for(nImpacts, 128)), dim3(128), icr, dn,
+           nImpacts, cr_stride, nd_stride, k, distanceEpsilon, (double *)ctx->ct.p);
     Stage st = make_stage(ctx, 1.0);
     launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
     launch_vals(ctx, mv, A, 1, st);
@@ -664,10 +778,10 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
     launch_vec3(ctx, mr, b, 1, 3, 0, st);
     if (nCZs) {
         HIPCHK(ctx, hipMemsetAsync(ctx->small + 2, 0, 2 * sizeof(int), ctx->stream));
-        hipLaunchKernelGGL(k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256), 0, ctx->stream,
-                           icz + (size_t)cz_stride * CZI_TDAMAGED, nCZs, ctx->small + 2);
-        hipLaunchKernelGGL(k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256), 0, ctx->stream,
-                           icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 3);
+        LAUNCH(ctx, "k_sum_int", k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256),
+               icz + (size_t)cz_stride * CZI_TDAMAGED, nCZs, ctx->small + 2);
+        LAUNCH(ctx, "k_sum_int", k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256),
+               icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 3);
     }
     if (int r = tm.finish()) return r;
     int sums[2] = {0, 0};

@@ -73,6 +73,8 @@ int icf_free(icf_ctx *ctx, void *dptr);
 int icf_h2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t bytes);
 int icf_d2h(icf_ctx *ctx, void *dst, const void *src, size_t src_off, size_t bytes);
 int icf_memset(icf_ctx *ctx, void *dst, int value, size_t bytes);
+/* device-to-device copy (CudaDeviceVariable.CopyToDevice(CudaDeviceVariable)) */
+int icf_d2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t src_off, size_t bytes);
 int icf_synchronize(icf_ctx *ctx);
 
 /* --- the five reference kernels, same argument lists (plus the context) --- */
@@ -130,6 +132,16 @@ int icf_pattern_changed(icf_ctx *ctx);
 /* Time, in milliseconds, of the last call's device work (HIP events on the context's
  * stream); -1 if the last call launched nothing. */
 double icf_last_device_ms(const icf_ctx *ctx);
+
+/* Per-kernel timing (the role of FrameInfo's KerElemForce / KerCZForce / CollForce
+ * stopwatches, FrameInfo.cs:75-105): when on, every kernel launch is bracketed by HIP events
+ * on the context's stream and its duration is accumulated under the kernel's name.
+ * icf_profile_entry fills name (NUL-terminated, truncated to cap), total milliseconds and
+ * launch count of entry i < icf_profile_count(). */
+int icf_set_profiling(icf_ctx *ctx, int on);
+int icf_profile_reset(icf_ctx *ctx);
+int icf_profile_count(const icf_ctx *ctx);
+int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
 }
