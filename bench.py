@@ -101,6 +101,8 @@ class Runner:
         self.eps = 1e-15
 
     def step(self):
+        """One iteration's device calls.  Blocking by default (the drop-in sequence); with
+        icf_set_blocking(0) they only enqueue, and between icf_graph_begin/end they are captured."""
         gf, lib, ctx, chk = self.gf, self.lib, self.gf.ctx, self.check
         chk(lib.icf_d2d(ctx, self.itet.p, 0, self.itet_src.p, 0, 8 * self.Q), ctx, "icf_d2d")
         chk(lib.icf_d2d(ctx, self.nl2.p, 0, self.nl2_src.p, 0, 8 * gf.tet_stride), ctx, "icf_d2d")
@@ -108,14 +110,26 @@ class Runner:
             ctx, "icf_narrow_phase")
         chk(lib.icf_find_closest_face(ctx, self.nT, gf.tet_stride, self.nl2.p, gf.g_dn.p, gf.g_ie.p, gf.g_ifc.p,
                                       gf.nd_stride, gf.el_all_stride, gf.fc_stride), ctx, "icf_find_closest_face")
-        dmg, fld = ctypes.c_int(), ctypes.c_int()
         chk(lib.icf_assemble_elems_and_czs(
             ctx, gf.g_ie_pcsr.p, len(self.col.elastic), gf.el_elastic_stride, gf.g_dcz.p, gf.g_icz.p,
             len(self.col.czs), gf.cz_stride, gf.g_dn.p, gf.nd_stride, self.h, self.vals.p, gf.csr.nnz,
-            self.rhs.p, gf.csr.N, ctypes.byref(dmg), ctypes.byref(fld)), ctx, "icf_assemble_elems_and_czs")
+            self.rhs.p, gf.csr.N, None, None), ctx, "icf_assemble_elems_and_czs")
         chk(lib.icf_collision_response(ctx, self.icr.p, gf.g_dn.p, self.h, self.vals.p, self.rhs.p, self.nT,
                                        gf.cr_stride, gf.nd_stride, self.k, self.eps), ctx, "icf_collision_response")
-        self.failed = fld.value
+
+    def failed_czs(self):
+        d, f = ctypes.c_int(), ctypes.c_int()
+        self.check(self.lib.icf_cz_counts(self.gf.ctx, ctypes.byref(d), ctypes.byref(f)), self.gf.ctx, "icf_cz_counts")
+        return f.value
+
+    def capture(self):
+        """The whole iteration as one HIP graph (icf_graph_begin / icf_graph_end)."""
+        lib, ctx = self.lib, self.gf.ctx
+        self.check(lib.icf_graph_begin(ctx), ctx, "icf_graph_begin")
+        self.step()
+        g = ctypes.c_void_p()
+        self.check(lib.icf_graph_end(ctx, ctypes.byref(g)), ctx, "icf_graph_end")
+        return g
 
     def set_mode(self, mode):
         self.gf.set_precision(mode)
@@ -135,22 +149,37 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def timed(runner, steps, warmup, profile):
+def timed(runner, steps, warmup, how):
+    """Seconds per step.  how = "dropin": blocking calls (each returns after its work, as the
+    reference's Run()); "graph": replays of one captured iteration graph; "profile": the calls
+    enqueued without waiting, each kernel bracketed by HIP events on the library's stream."""
     lib, ctx = runner.lib, runner.gf.ctx
+    g = None
+    if how == "graph":
+        lib.icf_set_blocking(ctx, 0)
+        g = runner.capture()
+        fn = lambda: lib.icf_graph_launch(ctx, g)      # noqa: E731
+    elif how == "profile":
+        lib.icf_set_blocking(ctx, 0)
+        fn = runner.step
+    else:
+        lib.icf_set_blocking(ctx, 1)
+        fn = runner.step
     for _ in range(warmup):
-        runner.step()
+        fn()
     lib.icf_synchronize(ctx)
     lib.icf_profile_reset(ctx)
-    lib.icf_set_profiling(ctx, 1 if profile else 0)
-    lib.icf_synchronize(ctx)
+    lib.icf_set_profiling(ctx, 1 if how == "profile" else 0)
     t0 = time.perf_counter()
-    for i in range(steps):
-        runner.step()
-        if (i + 1) % max(1, steps // 4) == 0:
-            log(f"  step {i + 1}/{steps}")
+    for _ in range(steps):
+        fn()
     lib.icf_synchronize(ctx)
     dt = time.perf_counter() - t0
     lib.icf_set_profiling(ctx, 0)
+    lib.icf_set_blocking(ctx, 1)
+    if g is not None:
+        lib.icf_graph_destroy(g)
+    log(f"  {how:8s}: {1e3 * dt / steps:.3f} ms/step over {steps} steps")
     return dt / steps, runner.profile()
 
 
@@ -246,17 +275,18 @@ def main():
     out = {}
     for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
         r.set_mode(mode)
-        log(f"timing {name}: {args.warmup} warmup + {args.steps} steps")
-        sec, prof = timed(r, args.steps, args.warmup, profile=True)
-        out[name] = (sec, prof)
-        log(f"  {name}: {sec * 1e3:.3f} ms/step")
+        log(f"timing {name}: {args.warmup} warmup + {args.steps} steps each way")
+        res = {how: timed(r, args.steps, args.warmup, how) for how in ("dropin", "graph", "profile")}
+        out[name] = res
+        prof = res["profile"][1]
         for kname, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
             log(f"    {kname:28s} {1e3 * ms / max(n, 1):9.1f} us/launch  x{n}")
 
     T, C, P = len(col.elastic), len(col.czs), r.nT
-    active_C = C - r.failed
+    active_C = C - r.failed_czs()
     units = T + active_C + P
-    sec, prof = out["mixed"]
+    sec = out["mixed"]["graph"][0]
+    prof = out["mixed"]["profile"][1]
     alg = algorithmic_bytes(r)
     # dominant kernel of the mixed step and its roofline
     dom = max(prof.items(), key=lambda kv: kv[1][0])
@@ -284,8 +314,12 @@ def main():
                                f"grains (seed {args.seed}), PPR CZs on inter-grain faces, 2 rigid 32x32 platens",
                    "tets": T, "czs": C, "active_czs": active_C, "impacts": P, "candidate_pairs": r.Q,
                    "nodes": len(col.x0), "rows": r.gf.csr.N, "nnzb": r.gf.csr.nnz, "mode": "mixed"},
-        "fp64": {"value": round(units / out["fp64"][0] / 1e6, 3), "ms_per_step": round(out["fp64"][0] * 1e3, 4)},
-        "mixed_over_fp64": round(out["fp64"][0] / sec, 3),
+        "timing": "one captured HIP graph per Newton iteration, replayed; per-kernel times from a separate "
+                  "pass with HIP events around each kernel on the library's stream",
+        "dropin_ms_per_step": {k: round(out[k]["dropin"][0] * 1e3, 4) for k in out},
+        "fp64": {"value": round(units / out["fp64"]["graph"][0] / 1e6, 3),
+                 "ms_per_step": round(out["fp64"]["graph"][0] * 1e3, 4)},
+        "mixed_over_fp64": round(out["fp64"]["graph"][0] / sec, 3),
         "mixed_rel_err_vs_fp64": errs,
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
         "roofline": roof,

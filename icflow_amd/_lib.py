@@ -16,7 +16,7 @@ P_int = ctypes.POINTER(ctypes.c_int)
 
 ICF_FP64, ICF_MIXED = 0, 1
 ERRORS = {-1: "ICF_E_INVALID", -2: "ICF_E_NOCONST", -3: "ICF_E_CAPACITY", -4: "ICF_E_CONSTANTS",
-          -5: "ICF_E_INDEX"}
+          -5: "ICF_E_INDEX", -6: "ICF_E_CAPTURE"}
 
 
 class IcfConsts(ctypes.Structure):
@@ -67,6 +67,12 @@ _SIGS = {
     "icf_pattern_changed": (c_int, [c_void_p]),
     "icf_last_device_ms": (c_double, [c_void_p]),
     "icf_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t]),
+    "icf_set_blocking": (c_int, [c_void_p, c_int]),
+    "icf_cz_counts": (c_int, [c_void_p, P_int, P_int]),
+    "icf_graph_begin": (c_int, [c_void_p]),
+    "icf_graph_end": (c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "icf_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "icf_graph_destroy": (c_int, [c_void_p]),
     "icf_set_profiling": (c_int, [c_void_p, c_int]),
     "icf_profile_reset": (c_int, [c_void_p]),
     "icf_profile_count": (c_int, [c_void_p]),

@@ -1,0 +1,381 @@
+// icf_contact_resp.inc.h -- point-triangle penalty response, phase 1 (included by icf_lib.hip).
+//
+// Restated from collision_kernels9/collision_response.h:38-126 (kCollisionResponseForce) and
+// point_triange_v2.h:13-842 (pt and its helpers).  The reference evaluates one impact per
+// thread with 12x12 temporaries; here one impact is shared by a group of 8 lanes: lane l
+// (l < 6) produces Hessian rows l and 11-l (13 upper-triangle entries each) and the forces of
+// those rows.  Every first-derivative term an entry needs is recomputed from the impact's few
+// vectors by index selection, so no lane holds per-index arrays, and each entry is the same
+// formula as the reference's.
+#pragma once
+
+namespace icf {
+
+constexpr int CT_GROUP = 8;
+
+__device__ inline double sel3(const double v[3], int c) { return c == 0 ? v[0] : (c == 1 ? v[1] : v[2]); }
+
+// Quantities of one impact shared by its lanes.
+struct Impact {
+    double p[3], v0[3], v1[3], v2[3];
+    int branch;          // 0 interior, 1..3 vertex, 4..7 edge (point_triange_v2.h:589-842)
+    double dsq, s, t;
+    // vertex branch: vertices (12-index node numbers 0..3) i0, i1
+    int i0, i1;
+    // edge branch: nodes of the line, nearer endpoint first
+    int ea, eb;
+    double y0[3], y1[3], y2[3];
+    double q, g, f2, f2sq, f2cu, if2, if2sq, if2cu;   // if* = reciprocals
+    // interior branch
+    double E1[3], E2[3], W[3], a, b, c, d, e, det, det2, det3, Ns, Nt, ps, pt, pu, r[3];
+    double idet, idet2, idet3;
+};
+
+// coordinate k of impact node nd (0 = point, 1..3 = triangle vertices), by selection so that
+// the Impact stays in registers
+__device__ inline double xnode(const Impact &I, int nd, int k)
+{
+    return nd == 0 ? sel3(I.p, k) : (nd == 1 ? sel3(I.v0, k) : (nd == 2 ? sel3(I.v1, k) : sel3(I.v2, k)));
+}
+
+__device__ inline void setup_vv(Impact &I, int i0, int i1)
+{
+    I.i0 = i0;
+    I.i1 = i1;
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double dd = xnode(I, i0, k) - xnode(I, i1, k);
+        s += dd * dd;
+    }
+    I.dsq = s;
+}
+
+// vertex_edge_distance_and_derivs_12 (point_triange_v2.h:237-294): nearer endpoint first.
+__device__ inline void setup_ve(Impact &I, int na, int nb)
+{
+    double pa = 0, pb = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double xa = xnode(I, na, k), xb = xnode(I, nb, k);
+        pa += (I.p[k] - xa) * (I.p[k] - xa);
+        pb += (I.p[k] - xb) * (I.p[k] - xb);
+    }
+    if (pa > pb) { int tmp = na; na = nb; nb = tmp; }
+    I.ea = na;
+    I.eb = nb;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { I.y0[k] = I.p[k]; I.y1[k] = xnode(I, na, k); I.y2[k] = xnode(I, nb, k); }
+    const double L2 = (I.y2[0] - I.y1[0]) * (I.y2[0] - I.y1[0]) + (I.y2[1] - I.y1[1]) * (I.y2[1] - I.y1[1]) +
+                      (I.y2[2] - I.y1[2]) * (I.y2[2] - I.y1[2]);
+    const double q = (I.y1[0] - I.y0[0]) * (I.y2[0] - I.y1[0]) + (I.y1[1] - I.y0[1]) * (I.y2[1] - I.y1[1]) +
+                     (I.y1[2] - I.y0[2]) * (I.y2[2] - I.y1[2]);
+    const double tt = -q / L2, uu = 1 - tt;
+    double v = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double rr = I.y0[k] - uu * I.y1[k] - tt * I.y2[k];
+        v += rr * rr;
+    }
+    I.dsq = v;
+    I.q = q;
+    I.g = q * q;
+    I.f2 = L2;
+    I.f2sq = L2 * L2;
+    I.f2cu = I.f2sq * L2;
+    I.if2 = 1.0 / I.f2;
+    I.if2sq = 1.0 / I.f2sq;
+    I.if2cu = 1.0 / I.f2cu;
+}
+
+// point_plane_distance (point_triange_v2.h:449-582), the values shared by all entries.
+__device__ inline void setup_pp(Impact &I)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        I.E1[k] = I.v1[k] - I.v0[k];
+        I.E2[k] = I.v2[k] - I.v0[k];
+        I.W[k] = I.v0[k] - I.p[k];
+    }
+    I.a = dot3(I.E1, I.E1);
+    I.b = dot3(I.E2, I.E1);
+    I.c = dot3(I.E2, I.E2);
+    I.d = dot3(I.W, I.E1);
+    I.e = dot3(I.W, I.E2);
+    I.det = I.a * I.c - I.b * I.b;
+    I.det2 = I.det * I.det;
+    I.det3 = I.det2 * I.det;
+    I.idet = 1.0 / I.det;
+    I.idet2 = 1.0 / I.det2;
+    I.idet3 = 1.0 / I.det3;
+    I.Ns = I.b * I.e - I.c * I.d;
+    I.Nt = I.b * I.d - I.a * I.e;
+    I.ps = I.Ns * (1. / I.det);
+    I.pt = I.Nt * (1. / I.det);
+    I.pu = 1 - (I.ps + I.pt);
+    double v = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        I.r[k] = I.v1[k] * I.ps + I.v2[k] * I.pt + I.v0[k] * I.pu - I.p[k];
+        v += I.r[k] * I.r[k];
+    }
+    I.dsq = v;
+}
+
+// Region classification of pt (point_triange_v2.h:589-842); fills branch data, s, t, dsq.
+__device__ inline void classify(Impact &I)
+{
+    double E1[3], E2[3], W[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) { E1[k] = I.v1[k] - I.v0[k]; E2[k] = I.v2[k] - I.v0[k]; W[k] = I.v0[k] - I.p[k]; }
+    const double a = dot3(E1, E1), b = dot3(E2, E1), c = dot3(E2, E2), d = dot3(W, E1), e = dot3(W, E2);
+    const double det = a * c - b * b;
+    double s = b * e - c * d, t = b * d - a * e;
+    int br;
+    if (s + t <= det) {
+        if (s < 0) {
+            if (t < 0 && d < 0) {
+                t = 0;
+                if (-d >= a) { br = 1; s = 1; setup_vv(I, 0, 2); }
+                else { br = 5; s = -d / a; setup_ve(I, 2, 1); }
+            } else {
+                s = 0;
+                if (e >= 0) { br = 3; t = 0; setup_vv(I, 0, 1); }
+                else if (-e >= c) { br = 2; t = 1; setup_vv(I, 0, 3); }
+                else { br = 4; t = -e / c; setup_ve(I, 3, 1); }
+            }
+        } else if (t < 0) {
+            t = 0;
+            if (d >= 0) { br = 3; s = 0; setup_vv(I, 0, 1); }
+            else if (-d >= a) { br = 1; s = 1; setup_vv(I, 0, 2); }
+            else { br = 5; s = -d / a; setup_ve(I, 1, 2); }
+        } else {
+            br = 0;
+            const double inv = 1. / det;
+            s *= inv;
+            t *= inv;
+            setup_pp(I);
+        }
+    } else {
+        if (s < 0) {
+            const double t0 = b + d, t1 = c + e;
+            if (t1 > t0) {
+                const double num = t1 - t0, den = a - 2 * b + c;
+                if (num >= den) { br = 1; s = 1; t = 0; setup_vv(I, 0, 2); }
+                else { br = 6; s = num / den; t = 1 - s; setup_ve(I, 2, 3); }
+            } else {
+                s = 0;
+                if (t1 <= 0) { br = 2; t = 1; setup_vv(I, 0, 3); }
+                else if (e >= 0) { br = 3; t = 0; setup_vv(I, 0, 1); }
+                else { br = 4; t = -e / c; setup_ve(I, 1, 3); }
+            }
+        } else if (t < 0) {
+            const double t0 = b + e, t1 = a + d;
+            if (t1 > t0) {
+                const double num = t1 - t0, den = a - 2 * b + c;
+                if (num >= den) { br = 2; t = 1; s = 0; setup_vv(I, 0, 3); }
+                else { br = 7; t = num / den; s = 1 - t; setup_ve(I, 2, 3); }
+            } else {
+                t = 0;
+                if (t1 <= 0) { br = 1; s = 1; setup_vv(I, 0, 2); }
+                else if (d >= 0) { br = 3; s = 0; setup_vv(I, 0, 1); }
+                else { br = 5; s = -d / a; setup_ve(I, 1, 2); }
+            }
+        } else {
+            const double num = c + e - b - d;
+            if (num <= 0) { br = 2; s = 0; t = 1; setup_vv(I, 0, 3); }
+            else {
+                const double den = a - 2 * b + c;
+                if (num >= den) { br = 1; s = 1; t = 0; setup_vv(I, 0, 2); }
+                else { br = 6; s = num / den; t = 1 - s; setup_ve(I, 2, 3); }
+            }
+        }
+    }
+    I.branch = br;
+    I.s = s;
+    I.t = t;
+}
+
+// ---- vertex-vertex: |x_i0 - x_i1|^2 ----
+__device__ inline double vv_grad(const Impact &I, int r)
+{
+    const int nd = r / 3, k = r % 3;
+    const double dd = xnode(I, I.i0, k) - xnode(I, I.i1, k);
+    return nd == I.i0 ? 2 * dd : (nd == I.i1 ? -(2 * dd) : 0.0);
+}
+__device__ inline double vv_hess(const Impact &I, int r, int c)
+{
+    if (r % 3 != c % 3) return 0;
+    const int nr = r / 3, nc = c / 3;
+    const bool ir = nr == I.i0 || nr == I.i1, ic = nc == I.i0 || nc == I.i1;
+    if (!ir || !ic) return 0;
+    return nr == nc ? 2.0 : -2.0;
+}
+
+// ---- vertex-edge: derivatives of f1 - g/f2 on (y0, y1, y2) ----
+__device__ inline int ve_slot(const Impact &I, int nd) { return nd == 0 ? 0 : (nd == I.ea ? 1 : (nd == I.eb ? 2 : -1)); }
+struct VeD { double qd, f1d, f2d; };
+__device__ inline VeD ve_d1(const Impact &I, int P, int k)
+{
+    const double a = sel3(I.y0, k), b = sel3(I.y1, k), c = sel3(I.y2, k);
+    VeD D;
+    D.qd = P == 0 ? b - c : (P == 1 ? a - 2 * b + c : b - a);
+    const double e1 = b - a, e2 = c - b;
+    D.f1d = P == 0 ? -2 * e1 : (P == 1 ? 2 * e1 : 0.0);
+    D.f2d = P == 0 ? 0.0 : (P == 1 ? -2 * e2 : 2 * e2);
+    return D;
+}
+__device__ inline double ve_grad(const Impact &I, int r)
+{
+    const int P = ve_slot(I, r / 3);
+    if (P < 0) return 0;
+    const VeD D = ve_d1(I, P, r % 3);
+    const double gd = 2 * I.q * D.qd;
+    return D.f1d + (I.g * D.f2d - I.f2 * gd) * I.if2sq;
+}
+__device__ inline double ve_hess(const Impact &I, int r, int c)
+{
+    const int Pr = ve_slot(I, r / 3), Pc = ve_slot(I, c / 3);
+    if (Pr < 0 || Pc < 0) return 0;
+    const VeD A = ve_d1(I, Pr, r % 3), B = ve_d1(I, Pc, c % 3);
+    const bool same = (r % 3) == (c % 3);
+    // node patterns of q = (y1-y0).(y2-y1), f1 = |y1-y0|^2, f2 = |y2-y1|^2
+    const int pc = Pr * 3 + Pc;
+    const double qdd = same ? (pc == 1 || pc == 3 || pc == 5 || pc == 7 ? 1.0 : (pc == 4 ? -2.0 : (pc == 2 || pc == 6 ? -1.0 : 0.0))) : 0.0;
+    const double f1dd = same ? (pc == 0 || pc == 4 ? 2.0 : (pc == 1 || pc == 3 ? -2.0 : 0.0)) : 0.0;
+    const double f2dd = same ? (pc == 4 || pc == 8 ? 2.0 : (pc == 5 || pc == 7 ? -2.0 : 0.0)) : 0.0;
+    const double gdI = 2 * I.q * A.qd, gdJ = 2 * I.q * B.qd;
+    const double gdd = 2 * (A.qd * B.qd + I.q * qdd);
+    return -2 * I.g * A.f2d * B.f2d * I.if2cu + (gdI * B.f2d + gdJ * A.f2d) * I.if2sq + f1dd + I.g * f2dd * I.if2sq -
+           gdd * I.if2;
+}
+
+// ---- interior: chain rule through s(x), t(x) ----
+struct PpD { double a1, b1, c1, d1, e1, dt1, s1, t1, u1; };
+__device__ inline PpD pp_d1(const Impact &I, int i)
+{
+    const int nd = i / 3, k = i % 3;
+    const double e1k = sel3(I.E1, k), e2k = sel3(I.E2, k), wk = sel3(I.W, k);
+    PpD D;
+    if (nd == 0) { D.a1 = 0; D.b1 = 0; D.c1 = 0; D.d1 = -e1k; D.e1 = -e2k; }
+    else if (nd == 1) { D.a1 = -2 * e1k; D.b1 = -e1k - e2k; D.c1 = -2 * e2k; D.d1 = e1k - wk; D.e1 = e2k - wk; }
+    else if (nd == 2) { D.a1 = 2 * e1k; D.b1 = e2k; D.c1 = 0; D.d1 = wk; D.e1 = 0; }
+    else { D.a1 = 0; D.b1 = e1k; D.c1 = 2 * e2k; D.d1 = 0; D.e1 = wk; }
+    D.dt1 = I.c * D.a1 + I.a * D.c1 - 2 * I.b * D.b1;
+    D.s1 = ((I.c * I.d - I.b * I.e) * D.dt1) * I.idet2 + ((I.e * D.b1 + I.b * D.e1) - (I.d * D.c1 + I.c * D.d1)) * I.idet;
+    D.t1 = ((I.a * I.e - I.b * I.d) * D.dt1) * I.idet2 + ((I.d * D.b1 + I.b * D.d1) - (I.a * D.e1 + I.e * D.a1)) * I.idet;
+    D.u1 = -(D.s1 + D.t1);
+    return D;
+}
+// d r_k / d x_i
+__device__ inline double pp_dr(const Impact &I, const PpD &D, int i, int k)
+{
+    double v = sel3(I.v1, k) * D.s1 + sel3(I.v2, k) * D.t1 + sel3(I.v0, k) * D.u1;
+    const int nd = i / 3;
+    if (i % 3 == k) v += (nd == 0 ? -1.0 : nd == 1 ? I.pu : nd == 2 ? I.ps : I.pt);
+    return v;
+}
+__device__ inline double pp_grad(const Impact &I, int i)
+{
+    const PpD D = pp_d1(I, i);
+    return 2 * (I.r[0] * pp_dr(I, D, i, 0) + I.r[1] * pp_dr(I, D, i, 1) + I.r[2] * pp_dr(I, D, i, 2));
+}
+// constant node patterns of the second derivatives of a..e (point_triange_v2.h:298-371)
+__device__ inline void pp_patterns(int ni, int nj, double &a2, double &b2, double &c2, double &d2, double &e2)
+{
+    const int k = ni * 4 + nj;
+    a2 = (k == 5 || k == 10) ? 2.0 : ((k == 6 || k == 9) ? -2.0 : 0.0);
+    b2 = k == 5 ? 2.0 : ((k == 6 || k == 9 || k == 7 || k == 13) ? -1.0 : ((k == 11 || k == 14) ? 1.0 : 0.0));
+    c2 = (k == 5 || k == 15) ? 2.0 : ((k == 7 || k == 13) ? -2.0 : 0.0);
+    d2 = (k == 1 || k == 4 || k == 6 || k == 9) ? 1.0 : ((k == 2 || k == 8) ? -1.0 : (k == 5 ? -2.0 : 0.0));
+    e2 = (k == 1 || k == 4 || k == 7 || k == 13) ? 1.0 : ((k == 3 || k == 12) ? -1.0 : (k == 5 ? -2.0 : 0.0));
+}
+__device__ inline double pp_hess(const Impact &I, const PpD &Di, int i, int j)
+{
+    const PpD Dj = pp_d1(I, j);
+    const int ni = i / 3, nj = j / 3;
+    double a2 = 0, b2 = 0, c2 = 0, d2 = 0, e2 = 0;
+    if ((i % 3) == (j % 3)) pp_patterns(ni, nj, a2, b2, c2, d2, e2);
+    const double a = I.a, b = I.b, c = I.c, d = I.d, e = I.e;
+    const double ddet = -2 * Di.b1 * Dj.b1 + Dj.a1 * Di.c1 + Di.a1 * Dj.c1 + c * a2 - 2 * b * b2 + a * c2;
+    const double s2 = (-(Dj.c1 * Di.d1) - Di.c1 * Dj.d1 + Dj.b1 * Di.e1 + Di.b1 * Dj.e1 + e * b2 - d * c2 - c * d2 + b * e2) * I.idet -
+                      ((Dj.dt1 * (e * Di.b1 - d * Di.c1 - c * Di.d1 + b * Di.e1)) +
+                       (Di.dt1 * (e * Dj.b1 - d * Dj.c1 - c * Dj.d1 + b * Dj.e1)) + (I.Ns * ddet)) * I.idet2 +
+                      (2 * I.Ns * Di.dt1 * Dj.dt1) * I.idet3;
+    const double t2 = (Dj.b1 * Di.d1 + Di.b1 * Dj.d1 - Dj.a1 * Di.e1 - Di.a1 * Dj.e1 - e * a2 + d * b2 + b * d2 - a * e2) * I.idet -
+                      ((Dj.dt1 * (-(e * Di.a1) + d * Di.b1 + b * Di.d1 - a * Di.e1)) +
+                       (Di.dt1 * (-(e * Dj.a1) + d * Dj.b1 + b * Dj.d1 - a * Dj.e1)) + (I.Nt * ddet)) * I.idet2 +
+                      (2 * I.Nt * Di.dt1 * Dj.dt1) * I.idet3;
+    const double u2 = -(s2 + t2);
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double v = sel3(I.v1, k) * s2 + sel3(I.v2, k) * t2 + sel3(I.v0, k) * u2;
+        if (i % 3 == k) v += (ni == 1 ? Dj.u1 : ni == 2 ? Dj.s1 : ni == 3 ? Dj.t1 : 0);
+        if (j % 3 == k) v += (nj == 1 ? Di.u1 : nj == 2 ? Di.s1 : nj == 3 ? Di.t1 : 0);
+        acc += pp_dr(I, Di, i, k) * pp_dr(I, Dj, j, k) + I.r[k] * v;
+    }
+    return 2 * acc;
+}
+
+__device__ inline double grad_at(const Impact &I, int r)
+{
+    if (I.branch == 0) return pp_grad(I, r);
+    if (I.branch <= 3) return vv_grad(I, r);
+    return ve_grad(I, r);
+}
+
+// Phase 1 for impacts: record = 78 upper entries of (k/2) Hess d^2, then 12 forces
+// f = [-fp, w0 fp, w1 fp, w2 fp] with fp = (k/2) d(d^2)/dp (collision_response.h:89-114).
+// Rigid-rigid impacts and d^2 <= eps^2 stage zeros.
+__global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ icr, const double *__restrict__ dn,
+                                                        int n, int cs, int ns, double kpen, double eps,
+                                                        double *__restrict__ st)
+{
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / CT_GROUP;
+    const int lane = threadIdx.x % CT_GROUP;
+    if (gid >= n || lane >= 6) return;
+    const int u = gid;
+    double *rec = st + (size_t)u * CT_REC;
+    bool active = false;
+#pragma unroll
+    for (int i = 0; i < 4; i++) active |= icr[u + cs * (CR_ROWS + i)] >= 0;
+    Impact I;
+    if (active) {
+        const int n0 = icr[u + cs * CR_NODES], n1 = icr[u + cs * (CR_NODES + 1)];
+        const int n2 = icr[u + cs * (CR_NODES + 2)], n3 = icr[u + cs * (CR_NODES + 3)];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            I.p[j] = dn[n0 + ns * (ND_XC + j)];
+            I.v0[j] = dn[n1 + ns * (ND_XC + j)];
+            I.v1[j] = dn[n2 + ns * (ND_XC + j)];
+            I.v2[j] = dn[n3 + ns * (ND_XC + j)];
+        }
+        classify(I);
+        active = I.dsq > eps * eps;
+    }
+    const double hk = kpen / 2;
+    for (int rr = 0; rr < 2; rr++) {
+        const int r = rr == 0 ? lane : 11 - lane;
+        if (!active) {
+            for (int c = r; c < 12; c++) rec[upper12(r, c)] = 0.;
+            rec[CT_F_OFF + r] = 0.;
+            continue;
+        }
+        if (I.branch == 0) {
+            const PpD Di = pp_d1(I, r);
+            for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * pp_hess(I, Di, r, c);
+        } else if (I.branch <= 3) {
+            for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * vv_hess(I, r, c);
+        } else {
+            for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * ve_hess(I, r, c);
+        }
+        // force of row r: the point gets -fp, triangle vertex m gets w_m fp
+        const double fpc = kpen * 0.5 * grad_at(I, r % 3);
+        const int nd = r / 3;
+        const double w = nd == 0 ? -1.0 : (nd == 1 ? 1 - (I.s + I.t) : (nd == 2 ? I.s : I.t));
+        rec[CT_F_OFF + r] = nd == 0 ? -fpc : w * fpc;
+    }
+}
+
+}  // namespace icf

@@ -230,179 +230,6 @@ __device__ inline void cz_frame(const double p[9], double R[9], double &area)
     R[3] = r0 / lr; R[4] = r1 / lr; R[5] = r2 / lr;
 }
 
-// Phase 1 for CZs: law, history, flags and averages written straight to dcz/icz (as the
-// reference does); rhs (18, FP64) and Q (18, FP64 or FP32) staged.  A CZ that is currently
-// failed or fails now stages zeros, so its slots receive +0.
-template <bool MIXED>
-__global__ __launch_bounds__(256) void k_cz_phase1(double *__restrict__ dcz, int *__restrict__ icz,
-                                                   const double *__restrict__ dn, int n, int cs, int ns,
-                                                   double *__restrict__ st_rhs, void *__restrict__ st_tan)
-{
-    const int u = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (u >= n) return;
-    double *orhs = st_rhs + (size_t)u * CZ_RHS_REC;
-    auto write_zero = [&]() {
-#pragma unroll
-        for (int i = 0; i < CZ_RHS_REC; i++) orhs[i] = 0.;
-        if (MIXED) {
-            float *t = (float *)st_tan + (size_t)u * CZ_TAN_REC;
-#pragma unroll
-            for (int i = 0; i < CZ_TAN_REC; i++) t[i] = 0.f;
-        } else {
-            double *t = (double *)st_tan + (size_t)u * CZ_TAN_REC;
-#pragma unroll
-            for (int i = 0; i < CZ_TAN_REC; i++) t[i] = 0.;
-        }
-    };
-    if (icz[u + cs * CZI_FAILED] != 0) {  // cz_kernels.h:219
-        write_zero();
-        return;
-    }
-    double xc[18];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int v = icz[u + cs * (CZI_VRTS + i)];
-#pragma unroll
-        for (int k = 0; k < 3; k++) xc[3 * i + k] = dn[v + ns * (ND_X0 + k)] + dn[v + ns * (ND_UN + k)];
-    }
-    double pmax[3], tmax[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        pmax[i] = dcz[u + cs * (CZD_PMAX + i)];
-        tmax[i] = dcz[u + cs * (CZD_TMAX + i)];
-    }
-    double R[9], area;
-    {
-#pragma clang fp contract(off)
-        double mid[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) mid[i] = (xc[i] + xc[i + 9]) * 0.5;
-        cz_frame(mid, R, area);
-    }
-    // local coordinates and their top-bottom differences (cz_kernels.h:259-290)
-    double dx[9];
-    {
-#pragma clang fp contract(off)
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                double top = xc[3 * i] * R[r * 3] + xc[3 * i + 1] * R[r * 3 + 1] + xc[3 * i + 2] * R[r * 3 + 2];
-                double bot = xc[3 * i + 9] * R[r * 3] + xc[3 * i + 10] * R[r * 3 + 1] + xc[3 * i + 11] * R[r * 3 + 2];
-                dx[3 * i + r] = top - bot;
-            }
-    }
-
-    double rhs[18];
-#pragma unroll
-    for (int i = 0; i < 18; i++) rhs[i] = 0;
-    double Q[18];
-    bool anyContact = false, anyFailed = false;
-    double avgDn = 0, avgDt = 0, avgTn = 0, avgTt = 0;
-    const double wa = (1.0 / 3.0) * area;
-
-    for (int gp = 0; gp < 3; gp++) {
-        double dt1, dt2, dnn;
-        {
-#pragma clang fp contract(off)
-            dt1 = 0; dt2 = 0; dnn = 0;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const double s = g_c.sf[i * 3 + gp];
-                dt1 += dx[3 * i + 0] * s;
-                dt2 += dx[3 * i + 1] * s;
-                dnn += dx[3 * i + 2] * s;
-            }
-        }
-        const double opn = dnn, opt = sqrt(dt1 * dt1 + dt2 * dt2);
-        bool gc = false, gf = false;
-        double L[5];
-        ppr_law(gc, gf, pmax[gp], tmax[gp], opn, opt, L);
-        anyContact |= gc;
-        anyFailed |= gf;
-        const double Tn = L[0], Tt = L[1], Dnn = L[2], Dtt = L[3], Dnt = L[4];
-        avgDn += opn / 3; avgDt += opt / 3; avgTn += Tn / 3; avgTt += Tt / 3;
-
-        double T0, T1, T2 = Tn, Td[9];
-        if (opt < 1e-20) {
-            T0 = T1 = 0;
-            Td[0] = Dtt; Td[1] = 0; Td[2] = Dnt;
-            Td[3] = 0; Td[4] = Dtt; Td[5] = Dnt;
-            Td[6] = Dnt; Td[7] = Dnt; Td[8] = Dnn;
-        } else {
-            T0 = Tt * dt1 / opt;
-            T1 = Tt * dt2 / opt;
-            const double o2 = opt * opt, o3 = o2 * opt;
-            const double d00 = dt1 * dt1, d10 = dt2 * dt1, d11 = dt2 * dt2;
-            Td[0] = Dtt * d00 / o2 + Tt * d11 / o3;
-            Td[4] = Dtt * d11 / o2 + Tt * d00 / o3;
-            Td[8] = Dnn;
-            Td[1] = Td[3] = Dtt * d10 / o2 - Tt * d10 / o3;
-            Td[2] = Td[6] = Dnt * dt1 / opt;
-            Td[5] = Td[7] = Dnt * dt2 / opt;
-        }
-        // global traction R^T T, scattered with -sgn_a s_a w a_J (cz_kernels.h:343-364)
-        double Tg[3];
-#pragma unroll
-        for (int r = 0; r < 3; r++) Tg[r] = R[r] * T0 + R[3 + r] * T1 + R[6 + r] * T2;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const double c = g_c.sf[a * 3 + gp] * wa;
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                rhs[3 * a + r] -= c * Tg[r];
-                rhs[9 + 3 * a + r] += c * Tg[r];
-            }
-        }
-        // Q_gp = w a_J R^T Td R (symmetric), cz_kernels.h:366-401
-        double TR[9];
-#pragma unroll
-        for (int m = 0; m < 3; m++)
-#pragma unroll
-            for (int l = 0; l < 3; l++) TR[m * 3 + l] = Td[m * 3] * R[l] + Td[m * 3 + 1] * R[3 + l] + Td[m * 3 + 2] * R[6 + l];
-        double q[9];
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-#pragma unroll
-            for (int l = 0; l < 3; l++) q[k * 3 + l] = wa * (R[k] * TR[l] + R[3 + k] * TR[3 + l] + R[6 + k] * TR[6 + l]);
-        Q[6 * gp + 0] = q[0]; Q[6 * gp + 1] = q[1]; Q[6 * gp + 2] = q[2];
-        Q[6 * gp + 3] = q[4]; Q[6 * gp + 4] = q[5]; Q[6 * gp + 5] = q[8];
-    }
-
-    // tentative history = max over Gauss points in all three slots; flags (cz_kernels.h:404-422)
-    int damaged = 0;
-    const double pm = fmax(fmax(pmax[0], pmax[1]), pmax[2]);
-    const double tm = fmax(fmax(tmax[0], tmax[1]), tmax[2]);
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        dcz[u + cs * (CZD_TPMAX + i)] = pm;
-        dcz[u + cs * (CZD_TTMAX + i)] = tm;
-        if (pmax[i] >= g_c.deln * g_c.rn || tmax[i] >= g_c.delt * g_c.rt) damaged = 1;
-    }
-    icz[u + cs * CZI_TCONTACT] = anyContact ? 1 : 0;
-    icz[u + cs * CZI_TFAILED] = anyFailed ? 1 : 0;
-    icz[u + cs * CZI_TDAMAGED] = anyFailed ? 0 : damaged;
-    if (anyFailed) {
-        write_zero();
-        return;
-    }
-    dcz[u + cs * CZD_DN] = avgDn;
-    dcz[u + cs * CZD_DT] = avgDt;
-    dcz[u + cs * CZD_TN] = avgTn;
-    dcz[u + cs * CZD_TT] = avgTt;
-#pragma unroll
-    for (int i = 0; i < 18; i++) orhs[i] = rhs[i];
-    if (MIXED) {
-        float *t = (float *)st_tan + (size_t)u * CZ_TAN_REC;
-#pragma unroll
-        for (int i = 0; i < 18; i++) t[i] = (float)Q[i];
-    } else {
-        double *t = (double *)st_tan + (size_t)u * CZ_TAN_REC;
-#pragma unroll
-        for (int i = 0; i < 18; i++) t[i] = Q[i];
-    }
-}
-
 // Phase 2: add tangent block (a,b), a,b in 0..5, of CZ record `rec` to acc.
 template <typename T>
 __device__ inline void cz_block(double acc[9], const T *__restrict__ rec, int a, int b)

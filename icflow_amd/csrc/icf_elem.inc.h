@@ -278,6 +278,41 @@ __device__ inline void tet_block_iso(double acc[9], const T *__restrict__ rec, i
         }
 }
 
+// The same block as tet_block_iso, split into a load step and an accumulate step so that a
+// caller can issue the loads of several contributions before using any of them.
+template <typename T>
+struct TetIsoLoad {
+    T gi[3], gj[3], V;
+    int i, j;
+    __device__ inline void load(const T *__restrict__ base, uint32_t code)
+    {
+        const uint32_t q = code_q(code);
+        i = q >> 2;
+        j = q & 3;
+        const T *rec = base + (size_t)code_unit(code) * TET_ISO_REC;
+        gi[0] = rec[3 * i]; gi[1] = rec[3 * i + 1]; gi[2] = rec[3 * i + 2];
+        gj[0] = rec[3 * j]; gj[1] = rec[3 * j + 1]; gj[2] = rec[3 * j + 2];
+        V = rec[12];
+    }
+    __device__ inline void add(double acc[9], T stiff, T mcoef) const
+    {
+        const T lam = sizeof(T) == 4 ? (T)g_c.lamf : (T)g_c.lam;
+        const T mu = sizeof(T) == 4 ? (T)g_c.muf : (T)g_c.mu;
+        const T c1 = stiff * V, cm = mcoef * V;
+        const T dg = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                const int mi = (3 * i + k) * 12 + 3 * j + l;
+                const T m = sizeof(T) == 4 ? (T)g_c.Mf[mi] : (T)g_c.M[mi];
+                T v = c1 * (lam * gi[k] * gj[l] + mu * gj[k] * gi[l]) + cm * m;
+                if (k == l) v += c1 * mu * dg;
+                acc[3 * k + l] += (double)v;
+            }
+    }
+};
+
 template <typename T>
 __device__ inline void tet_block_gen(double acc[9], const T *__restrict__ rec, int i, int j, T stiff, T mcoef)
 {

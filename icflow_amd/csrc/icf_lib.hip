@@ -22,7 +22,9 @@ __constant__ DevConsts g_c;
 }
 #include "icf_elem.inc.h"
 #include "icf_cz.inc.h"
+#include "icf_cz_phase1.inc.h"
 #include "icf_contact.inc.h"
+#include "icf_contact_resp.inc.h"
 #include "icf_reduce.inc.h"
 
 using namespace icf;
@@ -54,6 +56,8 @@ struct SlotMap {
     int ncodes = 0;
     int *off = nullptr;            // nkeys + 1
     uint32_t *codes = nullptr;     // ncodes
+    int *keys = nullptr;           // keys with contributions (accumulate-mode maps only)
+    int ntouched = 0;
 };
 
 std::atomic<int> g_next_ctx_id{1};
@@ -74,6 +78,8 @@ struct icf_ctx {
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     std::string err;
     double last_ms = -1;
+    bool blocking = true;          // icf_set_blocking
+    bool capturing = false;        // between icf_graph_begin and icf_graph_end
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
     struct Pending {
@@ -105,6 +111,7 @@ int fail(icf_ctx *ctx, int code, const std::string &msg)
 int grow(icf_ctx *ctx, DevBuf &b, size_t bytes)
 {
     if (b.bytes >= bytes) return 0;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "an internal buffer must grow: run the call once uncaptured");
     if (b.p) {
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         HIPCHK(ctx, hipFree(b.p));
@@ -121,8 +128,10 @@ void free_map(SlotMap &m)
 {
     if (m.off) (void)hipFree(m.off);
     if (m.codes) (void)hipFree(m.codes);
+    if (m.keys) (void)hipFree(m.keys);
     m.off = nullptr;
     m.codes = nullptr;
+    m.keys = nullptr;
 }
 
 // Drop cached maps whose key fields overlap [lo, hi).
@@ -160,7 +169,7 @@ unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs);
 
 hipEvent_t prof_begin(icf_ctx *ctx)
 {
-    if (!ctx->profiling) return nullptr;
+    if (!ctx->profiling || ctx->capturing) return nullptr;
     hipEvent_t e = nullptr;
     if (!ctx->ev_pool.empty()) {
         e = ctx->ev_pool.back();
@@ -221,6 +230,8 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
             *out = &m;
             return 0;
         }
+    if (ctx->capturing)
+        return fail(ctx, ICF_E_CAPTURE, "a slot map would be built during capture: run the call once uncaptured");
     for (const SrcKey &s : srcs)
         if ((uint32_t)s.n > CODE_UNIT_MAX + 1u)
             return fail(ctx, ICF_E_CAPACITY, "more than 2^24 units in one pcsr array");
@@ -273,6 +284,26 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         free_map(m);
         return fail(ctx, ICF_E_INDEX, "a slot/row/node index exceeds the declared size");
     }
+    if (fixed_nkeys < 0 && nkeys > 0) {
+        // accumulate-mode maps list the keys that have contributions, so that their reduction
+        // launches over those keys only
+        int *flag = nullptr, *pos = nullptr;
+        HIPCHK(ctx, hipMalloc(&flag, sizeof(int) * ((size_t)nkeys + 1)));
+        HIPCHK(ctx, hipMalloc(&pos, sizeof(int) * ((size_t)nkeys + 1)));
+        HIPCHK(ctx, hipMemsetAsync(flag + nkeys, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_map_flag, dim3(blocks_for(nkeys, 256)), dim3(256), 0, ctx->stream, d_cnt, nkeys, flag);
+        HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp_bytes, flag, pos, nkeys + 1, ctx->stream));
+        int nt = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&nt, pos + nkeys, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMalloc(&m.keys, sizeof(int) * ((size_t)nt + 1)));
+        hipLaunchKernelGGL(k_map_compact, dim3(blocks_for(nkeys, 256)), dim3(256), 0, ctx->stream, flag, pos, nkeys,
+                           m.keys);
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipFree(flag));
+        HIPCHK(ctx, hipFree(pos));
+        m.ntouched = nt;
+    }
     m.ncodes = total;
     HIPCHK(ctx, hipMalloc(&m.codes, sizeof(uint32_t) * ((size_t)total + 1)));
     HIPCHK(ctx, hipMemsetAsync(d_cnt, 0, sizeof(int) * ((size_t)nkeys + 1), ctx->stream));
@@ -302,13 +333,18 @@ int check_ctx(icf_ctx *ctx)
 
 struct Timer {
     icf_ctx *ctx;
-    explicit Timer(icf_ctx *c) : ctx(c)
+    bool sync;
+    explicit Timer(icf_ctx *c) : ctx(c), sync(c->blocking && !c->capturing)
     {
         ctx->last_ms = -1;
-        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        if (sync) (void)hipEventRecord(ctx->ev0, ctx->stream);
     }
     int finish()
     {
+        if (!sync) {
+            HIPCHK(ctx, hipGetLastError());
+            return 0;
+        }
         HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
         float ms = 0;
@@ -345,7 +381,8 @@ int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int 
     if (int r = grow(ctx, ctx->cz_rhs, sizeof(double) * CZ_RHS_REC * (size_t)(n > 0 ? n : 1))) return r;
     if (int r = grow(ctx, ctx->cz_tan, (mixed ? 4 : 8) * CZ_TAN_REC * (size_t)(n > 0 ? n : 1))) return r;
     if (n == 0) return 0;
-    dim3 g(blocks_for(n, 256)), b(256);
+    const long long waves = (n + CZ_PER_WAVE - 1) / CZ_PER_WAVE;   // three lanes per CZ
+    dim3 g(blocks_for(waves * 64, 256)), b(256);
     if (mixed)
         LAUNCH(ctx, "k_cz_phase1<mixed>", (k_cz_phase1<true>), g, b, dcz, icz, dn, n, cs, ns, (double *)ctx->cz_rhs.p,
                ctx->cz_tan.p);
@@ -371,21 +408,26 @@ Stage make_stage(icf_ctx *ctx, double h)
 
 void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, const Stage &sv)
 {
-    if (m->nkeys == 0) return;
+    // accumulate: only the keys with contributions; overwrite: every key
+    const int *keys = accumulate ? m->keys : nullptr;
+    const int n = accumulate ? m->ntouched : m->nkeys;
+    if (n == 0) return;
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
-    dim3 g(blocks_for(m->nkeys, 256)), b(256);
-    if (mixed && iso) LAUNCH(ctx, "k_reduce_vals<mixed,iso>", (k_reduce_vals<true, true>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
-    else if (mixed) LAUNCH(ctx, "k_reduce_vals<mixed,gen>", (k_reduce_vals<true, false>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
-    else if (iso) LAUNCH(ctx, "k_reduce_vals<fp64,iso>", (k_reduce_vals<false, true>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
-    else LAUNCH(ctx, "k_reduce_vals<fp64,gen>", (k_reduce_vals<false, false>), g, b, m->off, m->codes, m->nkeys, A, accumulate, sv);
+    dim3 g(blocks_for(n, 256)), b(256);
+    if (mixed && iso) LAUNCH(ctx, "k_reduce_vals<mixed,iso>", (k_reduce_vals<true, true>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
+    else if (mixed) LAUNCH(ctx, "k_reduce_vals<mixed,gen>", (k_reduce_vals<true, false>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
+    else if (iso) LAUNCH(ctx, "k_reduce_vals<fp64,iso>", (k_reduce_vals<false, true>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
+    else LAUNCH(ctx, "k_reduce_vals<fp64,gen>", (k_reduce_vals<false, false>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
 }
 
 void launch_vec3(icf_ctx *ctx, const SlotMap *m, double *out, int comp_stride, int elem_stride, int accumulate,
                  const Stage &sv)
 {
-    if (m->nkeys == 0) return;
+    const int *keys = accumulate ? m->keys : nullptr;
+    const int n = accumulate ? m->ntouched : m->nkeys;
+    if (n == 0) return;
     LAUNCH(ctx, elem_stride == 1 ? "k_reduce_vec3<force>" : "k_reduce_vec3<rhs>", k_reduce_vec3,
-           dim3(blocks_for(m->nkeys, 256)), dim3(256), m->off, m->codes, m->nkeys, out, comp_stride, elem_stride,
+           dim3(blocks_for(n, 256)), dim3(256), m->off, m->codes, keys, n, out, comp_stride, elem_stride,
            accumulate, sv);
 }
 
@@ -529,6 +571,7 @@ int icf_free(icf_ctx *ctx, void *dptr)
 {
     if (!ctx) return ICF_E_INVALID;
     if (!dptr) return 0;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_free during capture");
     if (int r = check_ctx(ctx)) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     size_t sz = 0;
@@ -544,6 +587,7 @@ int icf_free(icf_ctx *ctx, void *dptr)
 int icf_h2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t bytes)
 {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return ICF_E_INVALID;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "host copies cannot be captured");
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
     char *d = (char *)dst + dst_off;
@@ -556,6 +600,7 @@ int icf_h2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t byt
 int icf_d2h(icf_ctx *ctx, void *dst, const void *src, size_t src_off, size_t bytes)
 {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return ICF_E_INVALID;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "host copies cannot be captured");
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
     HIPCHK(ctx, hipMemcpyAsync(dst, (const char *)src + src_off, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -566,6 +611,7 @@ int icf_d2h(icf_ctx *ctx, void *dst, const void *src, size_t src_off, size_t byt
 int icf_memset(icf_ctx *ctx, void *dst, int value, size_t bytes)
 {
     if (!ctx || (!dst && bytes)) return ICF_E_INVALID;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_memset during capture (clear buffers with a kernel)");
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
     invalidate_range(ctx, (uintptr_t)dst, (uintptr_t)dst + bytes);
@@ -577,13 +623,16 @@ int icf_memset(icf_ctx *ctx, void *dst, int value, size_t bytes)
 int icf_synchronize(icf_ctx *ctx)
 {
     if (int r = check_ctx(ctx)) return r;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_synchronize during capture");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_flush(ctx);
     return 0;
 }
 
 int icf_pattern_changed(icf_ctx *ctx)
 {
     if (int r = check_ctx(ctx)) return r;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_pattern_changed during capture");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     for (SlotMap &m : ctx->maps) free_map(m);
     ctx->maps.clear();
@@ -598,9 +647,89 @@ int icf_d2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t src
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
     char *d = (char *)dst + dst_off;
-    invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
-    HIPCHK(ctx, hipMemcpyAsync(d, (const char *)src + src_off, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (!ctx->capturing) invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
+    Timer tm(ctx);
+    LAUNCH(ctx, "k_copy", k_copy, dim3(blocks_for((long long)((bytes + 15) / 16), 256)), dim3(256), d,
+           (const char *)src + src_off, bytes);
+    return tm.finish();
+}
+
+int icf_set_blocking(icf_ctx *ctx, int blocking)
+{
+    if (!ctx) return ICF_E_INVALID;
+    ctx->blocking = blocking != 0;
+    return 0;
+}
+
+int icf_cz_counts(icf_ctx *ctx, int *nCZDamaged, int *nCZFailed)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_cz_counts during capture");
+    int sums[2] = {0, 0};
+    HIPCHK(ctx, hipMemcpyAsync(sums, ctx->small + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (nCZDamaged) *nCZDamaged = sums[0];
+    if (nCZFailed) *nCZFailed = sums[1];
+    return 0;
+}
+
+struct icf_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+};
+
+int icf_graph_begin(icf_ctx *ctx)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "capture already in progress");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_flush(ctx);
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return 0;
+}
+
+int icf_graph_end(icf_ctx *ctx, icf_graph **out)
+{
+    if (!ctx || !out) return ICF_E_INVALID;
+    if (!ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "no capture in progress");
+    *out = nullptr;
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    HIPCHK(ctx, hipStreamEndCapture(ctx->stream, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(ctx, (int)e, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    icf_graph *gr = new icf_graph();
+    gr->graph = g;
+    gr->exec = ex;
+    gr->device = ctx->device;
+    *out = gr;
+    return 0;
+}
+
+int icf_graph_launch(icf_ctx *ctx, icf_graph *g)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (!g || g->device != ctx->device) return fail(ctx, ICF_E_INVALID, "icf_graph_launch: bad graph");
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_graph_launch during capture");
+    if (int r = upload_consts(ctx)) return r;
+    Timer tm(ctx);
+    HIPCHK(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return tm.finish();
+}
+
+int icf_graph_destroy(icf_graph *g)
+{
+    if (!g) return ICF_E_INVALID;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return 0;
 }
 
@@ -701,7 +830,7 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
         return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: invalid argument");
     if (nPairs == 0) return 0;
     Timer tm(ctx);
-    LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for(nPairs, 256)), dim3(256), nPairs, tet_stride,
+    LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256), nPairs, tet_stride,
            nl2, dn, ie, faces, nd_stride, el_all_stride, fc_stride);
     return tm.finish();
 }
@@ -723,7 +852,7 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
     if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
     if (int r = grow(ctx, ctx->ct, sizeof(double) * CT_REC * (size_t)nImpacts)) return r;
     Timer tm(ctx);
-    LAUNCH(ctx, "k_contact_phase1", k_contact_phase1, dim3(blocks_for(nImpacts, 128)), dim3(128), icr, dn,
+    LAUNCH(ctx, "k_contact_phase1", k_contact_phase1, dim3(blocks_for((long long)nImpacts * CT_GROUP, 256)), dim3(256), icr, dn,
            nImpacts, cr_stride, nd_stride, k, distanceEpsilon, (double *)ctx->ct.p);
     Stage st = make_stage(ctx, 1.0);
     launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
@@ -736,11 +865,13 @@ int icf_sum_int(icf_ctx *ctx, const int *d, int n, int *out)
 {
     if (int r = check_ctx(ctx)) return r;
     if (n < 0 || !out || (n && !d)) return fail(ctx, ICF_E_INVALID, "icf_sum_int: invalid argument");
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_sum_int returns a host value");
     *out = 0;
     if (n == 0) return 0;
-    HIPCHK(ctx, hipMemsetAsync(ctx->small + 2, 0, sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(k_sum_int, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, d, n, ctx->small + 2);
-    HIPCHK(ctx, hipMemcpyAsync(out, ctx->small + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, ctx->small + 4, 1, 0);
+    hipLaunchKernelGGL(k_sum_int, dim3(n < 64 * 256 ? blocks_for(n, 256) : 64), dim3(256), 0, ctx->stream, d, n,
+                       ctx->small + 4);
+    HIPCHK(ctx, hipMemcpyAsync(out, ctx->small + 4, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -776,20 +907,19 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
     if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
     launch_vals(ctx, mv, A, 0, st);
     launch_vec3(ctx, mr, b, 1, 3, 0, st);
-    if (nCZs) {
-        HIPCHK(ctx, hipMemsetAsync(ctx->small + 2, 0, 2 * sizeof(int), ctx->stream));
-        LAUNCH(ctx, "k_sum_int", k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256),
-               icz + (size_t)cz_stride * CZI_TDAMAGED, nCZs, ctx->small + 2);
-        LAUNCH(ctx, "k_sum_int", k_sum_int, dim3(blocks_for(nCZs, 256)), dim3(256),
-               icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 3);
-    }
+    // the two counts of GPU_Functionality.cs:474-475 (kernel-cleared, so the call can be captured)
+    LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(64), ctx->small + 2, 2, 0);
+    if (nCZs)
+        LAUNCH(ctx, "k_count_flags", k_count_flags, dim3(nCZs < 64 * 256 ? blocks_for(nCZs, 256) : 64), dim3(256),
+               icz + (size_t)cz_stride * CZI_TDAMAGED, icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 2);
     if (int r = tm.finish()) return r;
-    int sums[2] = {0, 0};
-    if (nCZs) {
-        HIPCHK(ctx, hipMemcpy(sums, ctx->small + 2, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if ((nCZDamaged || nCZFailed) && ctx->blocking && !ctx->capturing) {
+        int sums[2] = {0, 0};
+        HIPCHK(ctx, hipMemcpyAsync(sums, ctx->small + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (nCZDamaged) *nCZDamaged = sums[0];
+        if (nCZFailed) *nCZFailed = sums[1];
     }
-    if (nCZDamaged) *nCZDamaged = sums[0];
-    if (nCZFailed) *nCZFailed = sums[1];
     return 0;
 }
 

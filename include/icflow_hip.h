@@ -35,6 +35,7 @@ extern "C" {
 #define ICF_E_CAPACITY (-3)     /* more units than the 24-bit unit field of the slot maps   */
 #define ICF_E_CONSTANTS (-4)    /* constants outside the supported form (see icf_consts)    */
 #define ICF_E_INDEX (-5)        /* a slot / row / node index beyond the declared size       */
+#define ICF_E_CAPTURE (-6)      /* not allowed while a graph is being captured (see below)  */
 
 #define ICF_FP64 0              /* plain FP64 throughout                                    */
 #define ICF_MIXED 1             /* FP32 tangent blocks, FP64 residual, law and accumulation */
@@ -138,6 +139,26 @@ double icf_last_device_ms(const icf_ctx *ctx);
  * on the context's stream and its duration is accumulated under the kernel's name.
  * icf_profile_entry fills name (NUL-terminated, truncated to cap), total milliseconds and
  * launch count of entry i < icf_profile_count(). */
+/* --- asynchronous use and iteration graphs ---
+ * icf_set_blocking(ctx, 0): kernel entry points return once their work is enqueued on the
+ * context's stream (the reference's RunAsync, GPU_Functionality.cs:451); icf_synchronize()
+ * waits.  The fused call then leaves its two CZ counts on the device; icf_cz_counts() reads
+ * them (and waits).
+ * icf_graph_begin / icf_graph_end capture every call made in between into one HIP graph;
+ * icf_graph_launch replays it (blocking unless icf_set_blocking(ctx, 0)).  A replay repeats
+ * the captured calls exactly -- same buffers, sizes, time step and penalty parameters -- so
+ * capture again when any of them changes.  The slot maps of the captured calls must already
+ * exist (run the iteration once uncaptured); calls that would build one, grow an internal
+ * buffer, or copy to or from the host return ICF_E_CAPTURE during capture.  icf_d2d is
+ * allowed and is recorded as a copy kernel. */
+typedef struct icf_graph icf_graph;
+int icf_set_blocking(icf_ctx *ctx, int blocking);
+int icf_cz_counts(icf_ctx *ctx, int *nCZDamaged, int *nCZFailed);
+int icf_graph_begin(icf_ctx *ctx);
+int icf_graph_end(icf_ctx *ctx, icf_graph **out);
+int icf_graph_launch(icf_ctx *ctx, icf_graph *g);
+int icf_graph_destroy(icf_graph *g);
+
 int icf_set_profiling(icf_ctx *ctx, int on);
 int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);

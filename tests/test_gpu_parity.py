@@ -134,3 +134,54 @@ def test_mixed_precision_tolerances(col, orc, specimen_consts):
     assert errs["vals"] > 0          # really ran in FP32
     _cz_fields(p.gf, got, ref, len(col.czs))
     p.gf.close()
+
+
+def test_graph_replays_match_eager(orc, specimen_consts):
+    """One captured iteration graph (fused assembly + contact response), replayed three times
+    with eager calls in between; every replay must equal the eager result bit for bit."""
+    col = small_specimen(cells=(5, 4, 4), size=(0.05, 0.04, 0.04), grains=5, seed=7)
+    st = make_state(col, specimen_consts, seed=8, platen_dz_frac=0.3)
+    p = Pipeline(col, specimen_consts, st, contact=True)
+    gf, lib, ctx = p.gf, p.gf.lib, p.gf.ctx
+    eager = p.assemble()
+    ns = gf.nd_stride
+    zeroF = np.zeros(3 * ns)
+    from workloads import DIST_EPS, PENALTY_K
+
+    def calls():
+        _lib.check(lib.icf_assemble_elems_and_czs(
+            ctx, gf.g_ie_pcsr.p, len(col.elastic), gf.el_elastic_stride, gf.g_dcz.p, gf.g_icz.p, len(col.czs),
+            gf.cz_stride, gf.g_dn.p, ns, H, gf.g_dvals.p, gf.csr.nnz, gf.g_drhs.p, gf.csr.N, None, None), ctx, "asm")
+        _lib.check(lib.icf_collision_response(ctx, gf.g_icr.p, gf.g_dn.p, H, gf.g_dvals.p, gf.g_drhs.p, gf.nImpacts,
+                                              gf.cr_stride, ns, PENALTY_K, DIST_EPS), ctx, "contact")
+
+    _lib.check(lib.icf_set_blocking(ctx, 0))
+    _lib.check(lib.icf_graph_begin(ctx), ctx, "begin")
+    calls()
+    g = ctypes.c_void_p()
+    _lib.check(lib.icf_graph_end(ctx, ctypes.byref(g)), ctx, "end")
+    _lib.check(lib.icf_set_blocking(ctx, 1))
+    for rep in range(3):
+        gf.g_dn.upload(zeroF, offset_elems=12 * ns)                 # eager work between replays
+        gf.g_dvals.zero()
+        _lib.check(lib.icf_graph_launch(ctx, g), ctx, "launch")
+        vals, rhs = gf.TransferLinearSystemToHost()
+        F = gf.TransferUpdatedStateToHost()["F"]
+        assert np.array_equal(vals, eager["vals"]), rep
+        assert np.array_equal(rhs, eager["rhs"]), rep
+        assert np.array_equal(F, eager["F"]), rep
+        d, f = ctypes.c_int(), ctypes.c_int()
+        _lib.check(lib.icf_cz_counts(ctx, ctypes.byref(d), ctypes.byref(f)), ctx, "counts")
+        assert (d.value, f.value) == eager["counts"]
+    # building a slot map is refused during capture
+    _lib.check(lib.icf_set_blocking(ctx, 0))
+    _lib.check(lib.icf_graph_begin(ctx), ctx, "begin")
+    r = lib.icf_elem_force(ctx, gf.g_ie_pcsr.p, gf.g_dn.p, H, gf.g_dvals.p, gf.g_drhs.p, len(col.elastic),
+                           gf.el_elastic_stride, ns)
+    g2 = ctypes.c_void_p()
+    lib.icf_graph_end(ctx, ctypes.byref(g2))
+    assert r == -6
+    if g2.value:
+        lib.icf_graph_destroy(g2)
+    lib.icf_graph_destroy(g)
+    gf.close()
