@@ -92,6 +92,12 @@ class Runner:
         gf.c_icr, gf.cr_stride = gf.pack_icr(impacts)
         self.icr = gf.buf("icr", len(gf.c_icr), np.int32)
         self.icr.upload(gf.c_icr)
+        cs, nT = gf.cr_stride, self.nT
+        fld = gf.c_icr.reshape(-1, cs)[:, :nT]
+        self.contact_touched = {
+            "vals": int(np.unique(fld[4:20][fld[4:20] >= 0]).size),
+            "rhs": int(np.unique(fld[20:24][fld[20:24] >= 0]).size),
+            "force": int(np.unique(fld[0:4]).size)}
         nnz, N = gf.csr.nnz, gf.csr.N
         self.vals = gf.buf("vals", 2 * nnz * 9, np.float64)
         self.rhs = gf.buf("rhs", N * 3, np.float64)
@@ -185,7 +191,9 @@ def timed(runner, steps, warmup, how):
 
 def algorithmic_bytes(r):
     """Compulsory HBM bytes per launch of each kernel (every byte counted once; staging records
-    and slot maps are implementation traffic and are not counted)."""
+    and slot maps are implementation traffic and are not counted), keyed by the profile name
+    without its template arguments; accumulate-mode launches (the contact reductions) key as
+    "<name>+acc"."""
     col, gf = r.col, r.gf
     T, C, P, Q = len(col.elastic), len(col.czs), r.nT, r.Q
     nnzb, N = gf.csr.nnz, gf.csr.N
@@ -193,6 +201,7 @@ def algorithmic_bytes(r):
     n_cz = int(np.unique(col.czs).size) if C else 0
     n_surf = int(np.unique(col.surf).size)
     adj = gf.maxAdjacentFaces
+    touched = r.contact_touched
     return {
         # ids 16 B/tet + x0,un,vn,an 96 B per node
         "k_tet_phase1": 16 * T + 96 * n_def,
@@ -207,9 +216,37 @@ def algorithmic_bytes(r):
         "k_narrow_phase": 12 * Q + 16 * len(col.surf) + 24 * n_surf,
         # tuple in/out 24 B + adjacency count and ids + face ids
         "k_closest_face": 24 * P + 4 * P * (1 + adj),
+        # impact int record (96 B) + 4 nodes' x_current (96 B)
         "k_contact_phase1": 96 * P + 96 * P,
-        "k_sum_int": 4 * C,
+        # read-modify-write of the touched blocks + the 16 slot fields per impact
+        "k_reduce_vals+acc": 144 * touched["vals"] + 64 * P,
+        "k_reduce_vec3<rhs+acc>": 48 * touched["rhs"] + 16 * P,
+        "k_reduce_vec3<force+acc>": 48 * touched["force"] + 16 * P,
+        # the two restores of a step (8Q and 8 tet_stride bytes), read + write, averaged
+        "k_copy": 8 * Q + 8 * gf.tet_stride,
+        "k_count_flags": 8 * C,
+        "k_fill_int": 8,
     }
+
+
+def iteration_bytes(r):
+    """Compulsory HBM bytes of one whole Newton iteration's assembly (FP64 reference buffers,
+    every byte once): unit records, node fields in, node force out, BCSR values and rhs out,
+    narrow-phase lists.  Divided by the step time this is the iteration's HBM roofline."""
+    col, gf = r.col, r.gf
+    T, C, P, Q = len(col.elastic), len(col.czs), r.nT, r.Q
+    Nn = len(col.x0)
+    return (96 * T + (196 + 48 + 80 + 12) * C + 96 * P + 144 * Nn + 72 * gf.csr.nnz + 24 * gf.csr.N
+            + 12 * Q + 24 * P)
+
+
+def kernel_bytes(alg, name):
+    base = name.split("<")[0]
+    if name.endswith(",acc>"):
+        if base == "k_reduce_vec3":
+            return alg.get(name.replace(",acc>", "+acc>"))
+        return alg.get(base + "+acc")
+    return alg.get(name, alg.get(base))
 
 
 def cpu_baseline(runner, seconds):
@@ -249,6 +286,10 @@ def main():
     ap.add_argument("--platen-frac", type=float, default=0.3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed", help="precision modes to time (fp64, mixed)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run icf_d2d over this many MiB (a plain copy of known size, for "
+                         "calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}).",
@@ -273,7 +314,10 @@ def main():
     log(f"mixed vs fp64 normwise rel err: {errs}")
 
     out = {}
+    modes = [m for m in args.modes.split(",") if m]
     for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
+        if name not in modes:
+            continue
         r.set_mode(mode)
         log(f"timing {name}: {args.warmup} warmup + {args.steps} steps each way")
         res = {how: timed(r, args.steps, args.warmup, how) for how in ("dropin", "graph", "profile")}
@@ -282,22 +326,30 @@ def main():
         for kname, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
             log(f"    {kname:28s} {1e3 * ms / max(n, 1):9.1f} us/launch  x{n}")
 
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        a, b = r.gf.buf("probe_a", nb // 8, np.float64), r.gf.buf("probe_b", nb // 8, np.float64)
+        a.zero()
+        for _ in range(3):
+            r.check(r.lib.icf_d2d(r.gf.ctx, b.p, 0, a.p, 0, nb), r.gf.ctx, "icf_d2d")
+        log(f"copy probe: 3 x icf_d2d of {nb} bytes")
+
     T, C, P = len(col.elastic), len(col.czs), r.nT
     active_C = C - r.failed_czs()
     units = T + active_C + P
-    sec = out["mixed"]["graph"][0]
-    prof = out["mixed"]["profile"][1]
+    head = "mixed" if "mixed" in out else next(iter(out))
+    sec = out[head]["graph"][0]
+    prof = out[head]["profile"][1]
     alg = algorithmic_bytes(r)
-    # dominant kernel of the mixed step and its roofline
-    dom = max(prof.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dn) = dom
-    base = dname.split("<")[0] if not dname.startswith("k_reduce_vec3") else dname
-    dbytes = alg.get(base, alg.get(dname))
+    # dominant kernel of the step (largest total time) and its roofline
+    dname, (dms, dn) = max(prof.items(), key=lambda kv: kv[1][0])
+    dbytes = kernel_bytes(alg, dname)
     avg_s = dms / dn / 1e3
     achieved = dbytes / avg_s / 1e9
     roof = dict(kernel=dname, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes=int(dbytes),
                 avg_us=round(avg_s * 1e6, 2))
+    it_bytes = iteration_bytes(r)
 
     cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds)
     if cpu:
@@ -309,17 +361,20 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(sec * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic",
+        "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": f"compression with contact: cube {args.cells}^3 hex x 6 tets, {args.grains} "
                                f"grains (seed {args.seed}), PPR CZs on inter-grain faces, 2 rigid 32x32 platens",
                    "tets": T, "czs": C, "active_czs": active_C, "impacts": P, "candidate_pairs": r.Q,
-                   "nodes": len(col.x0), "rows": r.gf.csr.N, "nnzb": r.gf.csr.nnz, "mode": "mixed"},
+                   "nodes": len(col.x0), "rows": r.gf.csr.N, "nnzb": r.gf.csr.nnz, "mode": head},
         "timing": "one captured HIP graph per Newton iteration, replayed; per-kernel times from a separate "
                   "pass with HIP events around each kernel on the library's stream",
         "dropin_ms_per_step": {k: round(out[k]["dropin"][0] * 1e3, 4) for k in out},
-        "fp64": {"value": round(units / out["fp64"]["graph"][0] / 1e6, 3),
-                 "ms_per_step": round(out["fp64"]["graph"][0] * 1e3, 4)},
-        "mixed_over_fp64": round(out["fp64"]["graph"][0] / sec, 3),
+        "fp64": ({"value": round(units / out["fp64"]["graph"][0] / 1e6, 3),
+                  "ms_per_step": round(out["fp64"]["graph"][0] * 1e3, 4)} if "fp64" in out else None),
+        "mixed_over_fp64": (round(out["fp64"]["graph"][0] / out["mixed"]["graph"][0], 3)
+                            if "fp64" in out and "mixed" in out else None),
+        "iteration_roofline": {"alg_bytes": it_bytes, "achieved_GBs": round(it_bytes / sec / 1e9, 1),
+                               "frac": round(it_bytes / sec / 1e9 / HBM_PEAK_GBS, 4)},
         "mixed_rel_err_vs_fp64": errs,
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
         "roofline": roof,

@@ -11,7 +11,7 @@
 
 namespace icf {
 
-constexpr int CT_GROUP = 8;
+constexpr int CT_GROUP = 4;
 
 __device__ inline double sel3(const double v[3], int c) { return c == 0 ? v[0] : (c == 1 ? v[1] : v[2]); }
 
@@ -325,16 +325,158 @@ __device__ inline double grad_at(const Impact &I, int r)
     return ve_grad(I, r);
 }
 
+// ---- interior, closed form ----
+// In the interior region the squared distance is that to the plane of the triangle,
+// f = h^2 / m with N = E1 x E2, W = p - v0, h = N.W, m = N.N (E1 = v1 - v0, E2 = v2 - v0);
+// the reference's chain rule through s(x), t(x) (point_triange_v2.h:449-582) evaluates the
+// same function.  Here
+//   grad f = (2h/m) gh - (h/m)^2 gm,
+//   Hess f = (2/m) u u^T + (2h/m) Hh - (h/m)^2 Hm,     u = gh - (h/m) gm,
+// with gh = grad h = [N, -(sum), E2 x W, W x E1], gm = grad m = [0, -(sum), 2 E2 x N, 2 N x E1],
+//   Hh_ij = A T(W) + B T(E2) + C T(E1),   Hm_ij = 2 c_i.c_j + 2 A T(N),
+// where i = (node na, axis ka), j = (nb, kb), T(V) = eps_{ka kb c} V_c, c_i = dN/dx_i, and with
+// per-node coefficients a1 = dE1/dx, a2 = dE2/dx, w = dW/dx (p: 0,0,1; v0: -1,-1,-1; v1: 1,0,0;
+// v2: 0,1,0): A = a1(na) a2(nb) - a1(nb) a2(na), B = a1(nb) w(na) - a1(na) w(nb),
+// C = a2(na) w(nb) - a2(nb) w(na).  Each entry costs about 15 operations instead of the
+// chain rule's ~200.
+struct PpClosed {
+    double N[3], W[3], E1[3], E2[3];
+    double u[12];
+    double cu, chh, chm, gp;   // 2/m, 2h/m, (h/m)^2, 2h/m (gradient w.r.t. p = gp N)
+};
+
+__device__ inline void cross3(const double a[3], const double b[3], double c[3])
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __attribute__((always_inline)) inline void pp_closed_setup(const Impact &I, PpClosed &P)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        P.W[k] = I.p[k] - I.v0[k];
+        P.E1[k] = I.v1[k] - I.v0[k];
+        P.E2[k] = I.v2[k] - I.v0[k];
+    }
+    cross3(P.E1, P.E2, P.N);
+    const double h = dot3(P.N, P.W), m = dot3(P.N, P.N), im = 1.0 / m, r = h * im;
+    double g2[3], g3[3], m2[3], m3[3];
+    cross3(P.E2, P.W, g2);
+    cross3(P.W, P.E1, g3);
+    cross3(P.E2, P.N, m2);
+    cross3(P.N, P.E1, m3);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        m2[k] *= 2;
+        m3[k] *= 2;
+        const double g1 = -(P.N[k] + g2[k] + g3[k]), m1 = -(m2[k] + m3[k]);
+        P.u[k] = P.N[k];
+        P.u[3 + k] = g1 - r * m1;
+        P.u[6 + k] = g2[k] - r * m2[k];
+        P.u[9 + k] = g3[k] - r * m3[k];
+    }
+    P.cu = 2 * im;
+    P.chh = 2 * h * im;
+    P.chm = r * r;
+    P.gp = 2 * h * im;
+}
+
+__device__ inline double sel12(const double v[12], int i)
+{
+    double r = 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) r += (i == k ? 1.0 : 0.0) * v[k];
+    return r;
+}
+
+// V[k] for a runtime k as a masked sum: a select between elements of a private array is
+// turned into a load through a selected address, which puts the array in scratch memory.
+__device__ inline double pick3(const double V[3], int k)
+{
+    return (k == 0 ? 1.0 : 0.0) * V[0] + (k == 1 ? 1.0 : 0.0) * V[1] + (k == 2 ? 1.0 : 0.0) * V[2];
+}
+
+// eps_{a b c} V_c
+__device__ inline double tperm(int a, int b, const double V[3])
+{
+    const int d = (b - a + 3) % 3;   // 1: (a, a+1), 2: (a, a+2) = -(a+2, a)
+    const double vc = pick3(V, 3 - a - b);
+    return a == b ? 0.0 : (d == 1 ? vc : -vc);
+}
+
+// Row i of (k/2) Hess f, entries c >= i, into rec.
+__device__ __attribute__((always_inline)) inline void pp_closed_row(const PpClosed &P, int i, double hk, double *__restrict__ rec)
+{
+    const int na = i / 3, ka = i % 3;
+    const double a1i = na == 1 ? -1.0 : (na == 2 ? 1.0 : 0.0);
+    const double a2i = na == 1 ? -1.0 : (na == 3 ? 1.0 : 0.0);
+    const double wi = na == 0 ? 1.0 : (na == 1 ? -1.0 : 0.0);
+    const double ui = sel12(P.u, i);
+    // c_i = a1 (e_ka x E2) - a2 (e_ka x E1)
+    double ek2[3], ek1[3];
+    ek2[0] = 0; ek2[1] = 0; ek2[2] = 0;
+    ek1[0] = 0; ek1[1] = 0; ek1[2] = 0;
+    {
+        const int k1 = (ka + 1) % 3, k2 = (ka + 2) % 3;
+        // e_k x V = (component k1) -V_k2 ... : (e_k x V)_k1 = -V_k2, (e_k x V)_k2 = V_k1
+        const double e2k1 = pick3(P.E2, k1), e2k2 = pick3(P.E2, k2);
+        const double e1k1 = pick3(P.E1, k1), e1k2 = pick3(P.E1, k2);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ek2[c] = c == k1 ? -e2k2 : (c == k2 ? e2k1 : 0.0);
+            ek1[c] = c == k1 ? -e1k2 : (c == k2 ? e1k1 : 0.0);
+        }
+    }
+    double ci[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) ci[c] = a1i * ek2[c] - a2i * ek1[c];
+    // T(ka, kb; V) for kb = 0, 1, 2
+    double tw[3], t2[3], t1[3], tn[3];
+#pragma unroll
+    for (int kb = 0; kb < 3; kb++) {
+        tw[kb] = tperm(ka, kb, P.W);
+        t2[kb] = tperm(ka, kb, P.E2);
+        t1[kb] = tperm(ka, kb, P.E1);
+        tn[kb] = tperm(ka, kb, P.N);
+    }
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+        if (j < i) continue;
+        const int nb = j / 3, kb = j % 3;
+        const double a1j = nb == 1 ? -1.0 : (nb == 2 ? 1.0 : 0.0);
+        const double a2j = nb == 1 ? -1.0 : (nb == 3 ? 1.0 : 0.0);
+        const double wj = nb == 0 ? 1.0 : (nb == 1 ? -1.0 : 0.0);
+        const double A = a1i * a2j - a1j * a2i, B = a1j * wi - a1i * wj, C = a2i * wj - a2j * wi;
+        // c_j = a1j (e_kb x E2) - a2j (e_kb x E1), constant-coefficient in E1, E2
+        double cj[3];
+        {
+            const int k1 = (kb + 1) % 3, k2 = (kb + 2) % 3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double x2 = c == k1 ? -P.E2[k2] : (c == k2 ? P.E2[k1] : 0.0);
+                const double x1 = c == k1 ? -P.E1[k2] : (c == k2 ? P.E1[k1] : 0.0);
+                cj[c] = a1j * x2 - a2j * x1;
+            }
+        }
+        const double hh = A * tw[kb] + B * t2[kb] + C * t1[kb];
+        const double hm = 2 * (ci[0] * cj[0] + ci[1] * cj[1] + ci[2] * cj[2]) + 2 * A * tn[kb];
+        rec[upper12(i, j)] = hk * (P.cu * ui * P.u[j] + P.chh * hh - P.chm * hm);
+    }
+}
+
 // Phase 1 for impacts: record = 78 upper entries of (k/2) Hess d^2, then 12 forces
 // f = [-fp, w0 fp, w1 fp, w2 fp] with fp = (k/2) d(d^2)/dp (collision_response.h:89-114).
-// Rigid-rigid impacts and d^2 <= eps^2 stage zeros.
+// Rigid-rigid impacts and d^2 <= eps^2 stage zeros.  CT_GROUP lanes per impact; lane l
+// produces rows l, 7 - l and 8 + l (20, 20, 19 and 19 entries for l = 0..3).
 __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ icr, const double *__restrict__ dn,
                                                         int n, int cs, int ns, double kpen, double eps,
                                                         double *__restrict__ st)
 {
     const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / CT_GROUP;
     const int lane = threadIdx.x % CT_GROUP;
-    if (gid >= n || lane >= 6) return;
+    if (gid >= n) return;
     const int u = gid;
     double *rec = st + (size_t)u * CT_REC;
     bool active = false;
@@ -355,23 +497,42 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
         active = I.dsq > eps * eps;
     }
     const double hk = kpen / 2;
-    for (int rr = 0; rr < 2; rr++) {
-        const int r = rr == 0 ? lane : 11 - lane;
-        if (!active) {
+    const int rows[3] = {lane, 7 - lane, 8 + lane};
+    if (!active) {
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++) {
+            const int r = rows[rr];
             for (int c = r; c < 12; c++) rec[upper12(r, c)] = 0.;
             rec[CT_F_OFF + r] = 0.;
-            continue;
         }
-        if (I.branch == 0) {
-            const PpD Di = pp_d1(I, r);
-            for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * pp_hess(I, Di, r, c);
-        } else if (I.branch <= 3) {
-            for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * vv_hess(I, r, c);
-        } else {
-            for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * ve_hess(I, r, c);
+        return;
+    }
+    double gpk[3];    // (k/2) d(d^2)/dp
+    if (I.branch == 0) {
+        PpClosed P;
+        pp_closed_setup(I, P);
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++) pp_closed_row(P, rows[rr], hk, rec);
+#pragma unroll
+        for (int k = 0; k < 3; k++) gpk[k] = hk * P.gp * P.N[k];
+    } else {
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++) {
+            const int r = rows[rr];
+            if (I.branch <= 3) {
+                for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * vv_hess(I, r, c);
+            } else {
+                for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * ve_hess(I, r, c);
+            }
         }
-        // force of row r: the point gets -fp, triangle vertex m gets w_m fp
-        const double fpc = kpen * 0.5 * grad_at(I, r % 3);
+#pragma unroll
+        for (int k = 0; k < 3; k++) gpk[k] = hk * (I.branch <= 3 ? vv_grad(I, k) : ve_grad(I, k));
+    }
+    // force of row r: the point gets -fp, triangle vertex m gets w_m fp
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++) {
+        const int r = rows[rr];
+        const double fpc = pick3(gpk, r % 3);
         const int nd = r / 3;
         const double w = nd == 0 ? -1.0 : (nd == 1 ? 1 - (I.s + I.t) : (nd == 2 ? I.s : I.t));
         rec[CT_F_OFF + r] = nd == 0 ? -fpc : w * fpc;

@@ -55,12 +55,14 @@ struct DevConsts {
     double p_m, p_n, deln, delt, pMtn, pMnt, gam_n, gam_t;
     double sf[9];           // sf[node*3 + gp]
     double lam, mu;         // isotropic split of E (valid when iso != 0)
+    double m4[16];          // M[(3i)*12 + 3j]: the mass block (i,j) is m4[4i+j] I3 when iso != 0
     int iso;
     // powers at zero opening: (1 - 0)^e = 1 and (p/a + 0)^e, used by Dnn_(0, .) etc.
     double bn0[3], bt0[3];  // (p_m/alpha)^{p_m, p_m-1, p_m-2}, (p_n/beta)^{p_n, p_n-1, p_n-2}
     float Ef[36];
     float Mf[144];
     float lamf, muf;
+    float m4f[16];
 };
 
 // ---- small vector helpers --------------------------------------------------------------------

@@ -219,18 +219,20 @@ __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, 
 #pragma unroll
             for (int k = 0; k < 3; k++)
                 gt[3 * i + k] = R[k * 3 + 0] * g[3 * i] + R[k * 3 + 1] * g[3 * i + 1] + R[k * 3 + 2] * g[3 * i + 2];
+        // record: four (g~_i, V) quadruples, so that phase 2 fetches node i's gradient and the
+        // volume with one 16-byte (FP32) or two 16-byte (FP64) loads
         if (MIXED) {
-            float *t = (float *)st_tan + (size_t)u * TET_ISO_REC;
+            float4 *t = (float4 *)st_tan + (size_t)u * 4;
 #pragma unroll
-            for (int i = 0; i < 12; i++) t[i] = (float)gt[i];
-            t[12] = (float)V;
-            t[13] = t[14] = t[15] = 0.f;
+            for (int i = 0; i < 4; i++)
+                t[i] = make_float4((float)gt[3 * i], (float)gt[3 * i + 1], (float)gt[3 * i + 2], (float)V);
         } else {
-            double *t = (double *)st_tan + (size_t)u * TET_ISO_REC;
+            double2 *t = (double2 *)st_tan + (size_t)u * 8;
 #pragma unroll
-            for (int i = 0; i < 12; i++) t[i] = gt[i];
-            t[12] = V;
-            t[13] = t[14] = t[15] = 0.;
+            for (int i = 0; i < 4; i++) {
+                t[2 * i] = make_double2(gt[3 * i], gt[3 * i + 1]);
+                t[2 * i + 1] = make_double2(gt[3 * i + 2], V);
+            }
         }
     } else {
         if (MIXED) {
@@ -253,62 +255,70 @@ __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, 
     }
 }
 
-// Phase 2: add LHS block (i,j) of tet record `rec` to acc.  LHS = stiff * R K R^T + mcoef V M
+// Phase 2: LHS block (i,j) of an isotropic tet record.  LHS = stiff * R K R^T + mcoef V M
 // with stiff = 1 + dampK gamma / (h beta), mcoef = (1/h^2 + dampM gamma / h) / beta
-// (element_kernels2.h:239-249).
+// (element_kernels2.h:239-249); with isotropic E the stiffness block is
+// V (lam g~_i g~_j^T + mu g~_j g~_i^T + mu (g~_i . g~_j) I), and the mass block is m_ij I
+// (the supported form of M).
 template <typename T>
-__device__ inline void tet_block_iso(double acc[9], const T *__restrict__ rec, int i, int j, T stiff, T mcoef)
-{
-    const T lam = sizeof(T) == 4 ? (T)g_c.lamf : (T)g_c.lam;
-    const T mu = sizeof(T) == 4 ? (T)g_c.muf : (T)g_c.mu;
-    T gi[3] = {rec[3 * i], rec[3 * i + 1], rec[3 * i + 2]};
-    T gj[3] = {rec[3 * j], rec[3 * j + 1], rec[3 * j + 2]};
-    const T V = rec[12];
-    const T c1 = stiff * V, cm = mcoef * V;
-    const T dg = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int l = 0; l < 3; l++) {
-            const int mi = (3 * i + k) * 12 + 3 * j + l;
-            const T m = sizeof(T) == 4 ? (T)g_c.Mf[mi] : (T)g_c.M[mi];
-            T v = c1 * (lam * gi[k] * gj[l] + mu * gj[k] * gi[l]) + cm * m;
-            if (k == l) v += c1 * mu * dg;
-            acc[3 * k + l] += (double)v;
-        }
-}
+struct TetIsoLoad;
 
-// The same block as tet_block_iso, split into a load step and an accumulate step so that a
-// caller can issue the loads of several contributions before using any of them.
-template <typename T>
-struct TetIsoLoad {
-    T gi[3], gj[3], V;
-    int i, j;
-    __device__ inline void load(const T *__restrict__ base, uint32_t code)
+template <>
+struct TetIsoLoad<float> {
+    float4 gi, gj;   // (g~, V)
+    int ij;
+    __device__ inline void load(const float *__restrict__ base, uint32_t code)
     {
         const uint32_t q = code_q(code);
-        i = q >> 2;
-        j = q & 3;
-        const T *rec = base + (size_t)code_unit(code) * TET_ISO_REC;
-        gi[0] = rec[3 * i]; gi[1] = rec[3 * i + 1]; gi[2] = rec[3 * i + 2];
-        gj[0] = rec[3 * j]; gj[1] = rec[3 * j + 1]; gj[2] = rec[3 * j + 2];
-        V = rec[12];
+        const float4 *rec = (const float4 *)base + (size_t)code_unit(code) * 4;
+        ij = (int)q;
+        gi = rec[q >> 2];
+        gj = rec[q & 3];
     }
-    __device__ inline void add(double acc[9], T stiff, T mcoef) const
+    __device__ inline void values(float w[9], float stiff, float mcoef) const
     {
-        const T lam = sizeof(T) == 4 ? (T)g_c.lamf : (T)g_c.lam;
-        const T mu = sizeof(T) == 4 ? (T)g_c.muf : (T)g_c.mu;
-        const T c1 = stiff * V, cm = mcoef * V;
-        const T dg = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+        const float V = gi.w, c1 = stiff * V, cl = c1 * g_c.lamf, cu = c1 * g_c.muf;
+        const float dg = gi.x * gj.x + gi.y * gj.y + gi.z * gj.z;
+        const float dd = cu * dg + mcoef * V * g_c.m4f[ij];
+        const float a[3] = {gi.x, gi.y, gi.z}, b[3] = {gj.x, gj.y, gj.z};
 #pragma unroll
         for (int k = 0; k < 3; k++)
 #pragma unroll
             for (int l = 0; l < 3; l++) {
-                const int mi = (3 * i + k) * 12 + 3 * j + l;
-                const T m = sizeof(T) == 4 ? (T)g_c.Mf[mi] : (T)g_c.M[mi];
-                T v = c1 * (lam * gi[k] * gj[l] + mu * gj[k] * gi[l]) + cm * m;
-                if (k == l) v += c1 * mu * dg;
-                acc[3 * k + l] += (double)v;
+                float v = cl * a[k] * b[l] + cu * b[k] * a[l];
+                if (k == l) v += dd;
+                w[3 * k + l] = v;
+            }
+    }
+};
+
+template <>
+struct TetIsoLoad<double> {
+    double2 i0, i1, j0, j1;   // (g~x, g~y), (g~z, V)
+    int ij;
+    __device__ inline void load(const double *__restrict__ base, uint32_t code)
+    {
+        const uint32_t q = code_q(code);
+        const double2 *rec = (const double2 *)base + (size_t)code_unit(code) * 8;
+        ij = (int)q;
+        i0 = rec[2 * (q >> 2)];
+        i1 = rec[2 * (q >> 2) + 1];
+        j0 = rec[2 * (q & 3)];
+        j1 = rec[2 * (q & 3) + 1];
+    }
+    __device__ inline void values(double w[9], double stiff, double mcoef) const
+    {
+        const double V = i1.y, c1 = stiff * V, cl = c1 * g_c.lam, cu = c1 * g_c.mu;
+        const double a[3] = {i0.x, i0.y, i1.x}, b[3] = {j0.x, j0.y, j1.x};
+        const double dg = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+        const double dd = cu * dg + mcoef * V * g_c.m4[ij];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                double v = cl * a[k] * b[l] + cu * b[k] * a[l];
+                if (k == l) v += dd;
+                w[3 * k + l] = v;
             }
     }
 };

@@ -56,8 +56,16 @@ struct SlotMap {
     int ncodes = 0;
     int *off = nullptr;            // nkeys + 1
     uint32_t *codes = nullptr;     // ncodes
-    int *keys = nullptr;           // keys with contributions (accumulate-mode maps only)
-    int ntouched = 0;
+    int *keys = nullptr;           // accumulate mode: the keys with contributions (null: all keys)
+    int nlaunch = 0;               // keys a reduction visits
+    int *chunks = nullptr;         // BCSR-value maps: chunk starts (list positions), nchunks + 1
+    int nchunks = 0;
+    int id = 0;                    // unique per map built
+    // BCSR-value maps: the rhs map (by id) whose rows this map's diagonal slots were checked
+    // against, the outcome, and slot -> rhs row (-1 off the diagonal) when it holds
+    int fused_with = 0;
+    bool fuse_ok = false;
+    int *slot_row = nullptr;
 };
 
 std::atomic<int> g_next_ctx_id{1};
@@ -76,6 +84,7 @@ struct icf_ctx {
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
     int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
+    int next_map_id = 0;
     std::string err;
     double last_ms = -1;
     bool blocking = true;          // icf_set_blocking
@@ -129,9 +138,13 @@ void free_map(SlotMap &m)
     if (m.off) (void)hipFree(m.off);
     if (m.codes) (void)hipFree(m.codes);
     if (m.keys) (void)hipFree(m.keys);
+    if (m.slot_row) (void)hipFree(m.slot_row);
+    if (m.chunks) (void)hipFree(m.chunks);
+    m.chunks = nullptr;
     m.off = nullptr;
     m.codes = nullptr;
     m.keys = nullptr;
+    m.slot_row = nullptr;
 }
 
 // Drop cached maps whose key fields overlap [lo, hi).
@@ -284,6 +297,8 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         free_map(m);
         return fail(ctx, ICF_E_INDEX, "a slot/row/node index exceeds the declared size");
     }
+    int *list = nullptr;           // accumulate mode: the keys with contributions
+    int nlist = nkeys;
     if (fixed_nkeys < 0 && nkeys > 0) {
         // accumulate-mode maps list the keys that have contributions, so that their reduction
         // launches over those keys only
@@ -296,15 +311,48 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         int nt = 0;
         HIPCHK(ctx, hipMemcpyAsync(&nt, pos + nkeys, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        HIPCHK(ctx, hipMalloc(&m.keys, sizeof(int) * ((size_t)nt + 1)));
+        HIPCHK(ctx, hipMalloc(&list, sizeof(int) * ((size_t)nt + 1)));
         hipLaunchKernelGGL(k_map_compact, dim3(blocks_for(nkeys, 256)), dim3(256), 0, ctx->stream, flag, pos, nkeys,
-                           m.keys);
+                           list);
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         HIPCHK(ctx, hipFree(flag));
         HIPCHK(ctx, hipFree(pos));
-        m.ntouched = nt;
+        nlist = nt;
     }
+    m.keys = list;                 // null when overwriting: every key, in order
+    if (target == TGT_VALS && nlist > 0) {
+        // chunks of consecutive list positions for the staged reduction of BCSR values: at
+        // most FLAT_KEYS keys and FLAT_CAP contributions each (a single key with more
+        // contributions gets a chunk of its own, reduced without staging)
+        std::vector<int> h_off((size_t)nkeys + 1), h_list(list ? (size_t)nlist : 0);
+        HIPCHK(ctx, hipMemcpyAsync(h_off.data(), m.off, sizeof(int) * ((size_t)nkeys + 1), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        if (list)
+            HIPCHK(ctx, hipMemcpyAsync(h_list.data(), list, sizeof(int) * (size_t)nlist, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<int> ch(1, 0);
+        int cnt = 0, nk = 0;
+        for (int p = 0; p < nlist; p++) {
+            const int k = list ? h_list[p] : p;
+            const int c = h_off[k + 1] - h_off[k];
+            if (nk > 0 && (nk == FLAT_KEYS || cnt + c > FLAT_CAP)) {
+                ch.push_back(p);
+                cnt = 0;
+                nk = 0;
+            }
+            cnt += c;
+            nk++;
+        }
+        ch.push_back(nlist);
+        m.nchunks = (int)ch.size() - 1;
+        HIPCHK(ctx, hipMalloc(&m.chunks, sizeof(int) * ch.size()));
+        HIPCHK(ctx, hipMemcpyAsync(m.chunks, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    m.nlaunch = nlist;
     m.ncodes = total;
+    m.id = ++ctx->next_map_id;
     HIPCHK(ctx, hipMalloc(&m.codes, sizeof(uint32_t) * ((size_t)total + 1)));
     HIPCHK(ctx, hipMemsetAsync(d_cnt, 0, sizeof(int) * ((size_t)nkeys + 1), ctx->stream));
     for (const SrcKey &s : srcs) {
@@ -321,6 +369,43 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
     HIPCHK(ctx, hipFree(d_cnt));
     ctx->maps.push_back(m);
     *out = &ctx->maps.back();
+    return 0;
+}
+
+// Whether the rhs rows of map mr can be reduced by the diagonal-slot threads of map mv (see
+// k_fuse_check); decided once per pair of maps.  *slot_row = the slot -> row table, or null.
+int get_fusion(icf_ctx *ctx, SlotMap *mv, const SlotMap *mr, const MapSrc &tet, const MapSrc &cz,
+               const int **slot_row)
+{
+    *slot_row = nullptr;
+    if (mv->fused_with != mr->id) {
+        if (ctx->capturing)
+            return fail(ctx, ICF_E_CAPTURE, "a slot map would be built during capture: run the call once uncaptured");
+        if (mv->slot_row) (void)hipFree(mv->slot_row);
+        mv->slot_row = nullptr;
+        mv->fuse_ok = false;
+        mv->fused_with = mr->id;
+        if (mr->nkeys > 0 && mv->nkeys > 0) {
+            HIPCHK(ctx, hipMalloc(&mv->slot_row, sizeof(int) * (size_t)mv->nkeys));
+            int *d_err = ctx->small + 1;
+            hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(mv->nkeys, 256)), dim3(256), 0, ctx->stream,
+                               mv->slot_row, mv->nkeys, -1);
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+            hipLaunchKernelGGL(k_fuse_check, dim3(blocks_for(mr->nkeys, 256)), dim3(256), 0, ctx->stream, mr->off,
+                               mr->codes, mr->nkeys, mv->off, mv->codes, mv->nkeys, tet, cz, mv->slot_row, d_err);
+            int err = 1;
+            HIPCHK(ctx, hipGetLastError());
+            HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            if (err) {
+                HIPCHK(ctx, hipFree(mv->slot_row));
+                mv->slot_row = nullptr;
+            } else {
+                mv->fuse_ok = true;
+            }
+        }
+    }
+    if (mv->fuse_ok) *slot_row = mv->slot_row;
     return 0;
 }
 
@@ -406,29 +491,57 @@ Stage make_stage(icf_ctx *ctx, double h)
     return sv;
 }
 
-void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, const Stage &sv)
+template <bool MX, bool IS, bool ACC, bool RHS, typename L>
+void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, const Stage &sv, const int *slot_row,
+                 double *b)
 {
-    // accumulate: only the keys with contributions; overwrite: every key
-    const int *keys = accumulate ? m->keys : nullptr;
-    const int n = accumulate ? m->ntouched : m->nkeys;
-    if (n == 0) return;
+    LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, RHS, L>), dim3(m->nchunks), dim3(FLAT_KEYS), m->off, m->codes,
+           m->keys, m->chunks, m->nchunks, A, sv, slot_row, b);
+}
+
+// Phase 2 of BCSR values for map m.  accumulate: add into A over the keys with contributions
+// (single-kernel entry points); otherwise overwrite every block, with the rhs rows reduced
+// alongside when slot_row is given.  contact: the map's units are impacts, whose FP64
+// records are staged as FP64 in both modes.
+void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, const Stage &sv,
+                 const int *slot_row = nullptr, double *b = nullptr, bool contact = false)
+{
+    if (m->nchunks == 0) return;
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
-    dim3 g(blocks_for(n, 256)), b(256);
-    if (mixed && iso) LAUNCH(ctx, "k_reduce_vals<mixed,iso>", (k_reduce_vals<true, true>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
-    else if (mixed) LAUNCH(ctx, "k_reduce_vals<mixed,gen>", (k_reduce_vals<true, false>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
-    else if (iso) LAUNCH(ctx, "k_reduce_vals<fp64,iso>", (k_reduce_vals<false, true>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
-    else LAUNCH(ctx, "k_reduce_vals<fp64,gen>", (k_reduce_vals<false, false>), g, b, m->off, m->codes, keys, n, A, accumulate, sv);
+    if (contact) {
+        vals_kernel<false, false, true, false, double>(ctx, "k_reduce_vals<contact,acc>", m, A, sv, nullptr, nullptr);
+        return;
+    }
+#define ICF_VALS(MX, IS, L, NAME)                                                                              \
+    do {                                                                                                       \
+        if (accumulate)                                                                                        \
+            vals_kernel<MX, IS, true, false, L>(ctx, "k_reduce_vals<" NAME ",acc>", m, A, sv, nullptr, nullptr); \
+        else if (slot_row)                                                                                     \
+            vals_kernel<MX, IS, false, true, L>(ctx, "k_reduce_vals<" NAME ",rhs>", m, A, sv, slot_row, b);    \
+        else                                                                                                   \
+            vals_kernel<MX, IS, false, false, L>(ctx, "k_reduce_vals<" NAME ">", m, A, sv, nullptr, nullptr);  \
+    } while (0)
+    if (mixed && iso) ICF_VALS(true, true, float, "mixed,iso");
+    else if (mixed) ICF_VALS(true, false, float, "mixed,gen");
+    else if (iso) ICF_VALS(false, true, double, "fp64,iso");
+    else ICF_VALS(false, false, double, "fp64,gen");
+#undef ICF_VALS
 }
 
 void launch_vec3(icf_ctx *ctx, const SlotMap *m, double *out, int comp_stride, int elem_stride, int accumulate,
                  const Stage &sv)
 {
-    const int *keys = accumulate ? m->keys : nullptr;
-    const int n = accumulate ? m->ntouched : m->nkeys;
+    const int *keys = m->keys;
+    const int n = m->nlaunch;
     if (n == 0) return;
-    LAUNCH(ctx, elem_stride == 1 ? "k_reduce_vec3<force>" : "k_reduce_vec3<rhs>", k_reduce_vec3,
-           dim3(blocks_for(n, 256)), dim3(256), m->off, m->codes, keys, n, out, comp_stride, elem_stride,
-           accumulate, sv);
+    dim3 g(blocks_for(n, 256)), b(256);
+    const bool force = elem_stride == 1;
+    if (accumulate)
+        LAUNCH(ctx, force ? "k_reduce_vec3<force,acc>" : "k_reduce_vec3<rhs,acc>", k_reduce_vec3<true>, g, b, m->off,
+               m->codes, keys, n, out, comp_stride, elem_stride, sv);
+    else
+        LAUNCH(ctx, force ? "k_reduce_vec3<force>" : "k_reduce_vec3<rhs>", k_reduce_vec3<false>, g, b, m->off,
+               m->codes, keys, n, out, comp_stride, elem_stride, sv);
 }
 
 bool bad_units(int n, int stride) { return n < 0 || stride < 0 || (n > 0 && stride < n); }
@@ -531,6 +644,17 @@ int icf_set_constants(icf_ctx *ctx, const icf_consts *c)
                 else iso = (v == 0);
             }
         if (iso) iso = fabs(dg - (lam + 2 * mu)) <= 1e-12 * fabs(dg);
+        // the isotropic path also takes the mass as m_ij I3 per node pair (ModelPrms.cs:239-248
+        // builds it so); any other M takes the general path
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                const double mij = c->M[(3 * i) * 12 + 3 * j];
+                d.m4[4 * i + j] = mij;
+                d.m4f[4 * i + j] = (float)mij;
+                for (int k = 0; k < 3 && iso; k++)
+                    for (int l = 0; l < 3 && iso; l++)
+                        iso = c->M[(3 * i + k) * 12 + 3 * j + l] == (k == l ? mij : 0.0);
+            }
         d.iso = iso ? 1 : 0;
         d.lam = lam;
         d.mu = mu;
@@ -856,7 +980,7 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
            nImpacts, cr_stride, nd_stride, k, distanceEpsilon, (double *)ctx->ct.p);
     Stage st = make_stage(ctx, 1.0);
     launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
-    launch_vals(ctx, mv, A, 1, st);
+    launch_vals(ctx, mv, A, 1, st, nullptr, nullptr, true);
     launch_vec3(ctx, mr, b, 1, 3, 1, st);
     return tm.finish();
 }
@@ -900,13 +1024,19 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
     if (int r = get_map(ctx, TGT_RHS, N, sr, &mr)) return r;
     if (nElems)
         if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
+    const int *slot_row = nullptr;
+    {
+        MapSrc tet{nElems ? ie : nullptr, nElems, el_stride, EL_SLOTS, 16, KIND_TET};
+        MapSrc cz{nCZs ? icz : nullptr, nCZs, cz_stride, CZI_SLOTS, 36, KIND_CZ};
+        if (int r = get_fusion(ctx, mv, mr, tet, cz, &slot_row)) return r;
+    }
     Timer tm(ctx);
     if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
     if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
     Stage st = make_stage(ctx, h);
     if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
-    launch_vals(ctx, mv, A, 0, st);
-    launch_vec3(ctx, mr, b, 1, 3, 0, st);
+    launch_vals(ctx, mv, A, 0, st, slot_row, b);
+    if (!slot_row) launch_vec3(ctx, mr, b, 1, 3, 0, st);
     // the two counts of GPU_Functionality.cs:474-475 (kernel-cleared, so the call can be captured)
     LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(64), ctx->small + 2, 2, 0);
     if (nCZs)

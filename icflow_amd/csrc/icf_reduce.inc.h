@@ -97,6 +97,11 @@ __global__ __launch_bounds__(256) void k_map_compact(const int *__restrict__ fla
     if (k < nkeys && flag[k]) keys[pos[k]] = k;
 }
 
+// Chunk limits of the staged BCSR-value reduction: keys per workgroup (one thread each) and
+// contributions staged in LDS (FLAT_CAP * 9 values: 27 KB FP32, 54 KB FP64).
+constexpr int FLAT_KEYS = 256;
+constexpr int FLAT_CAP = 768;
+
 // Staged per-unit records of the current call.
 struct Stage {
     const double *tet_rhs;
@@ -107,66 +112,6 @@ struct Stage {
     double stiff, mcoef;   // tet LHS coefficients (element_kernels2.h:239-240)
 };
 
-// Phase 2 for BCSR values: one thread per slot.  accumulate != 0 adds to A (reference
-// semantics for a single kernel call); accumulate == 0 overwrites, zero for slots without
-// contributions (the fused AssembleElemsAndCZs, whose memset is folded in here).
-// keys != nullptr: thread t handles key keys[t] (t < nkeys), the keys that have contributions.
-template <bool MIXED, bool ISO>
-__global__ __launch_bounds__(256) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
-                                                     const int *__restrict__ keys, int nkeys, double *__restrict__ A,
-                                                     int accumulate, Stage sv)
-{
-    using T = typename std::conditional<MIXED, float, double>::type;
-    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= nkeys) return;
-    const int s = keys ? keys[t] : t;
-    const int b = off[s], e = off[s + 1];
-    double acc[9];
-    double *dst = A + (size_t)s * 9;
-    if (accumulate) {
-        if (b == e) return;
-#pragma unroll
-        for (int i = 0; i < 9; i++) acc[i] = dst[i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < 9; i++) acc[i] = 0.;
-    }
-    const T stiff = (T)sv.stiff, mcoef = (T)sv.mcoef;
-    int c = b;
-    if (ISO) {
-        // Tets come first in a slot's list (codes are sorted by kind).  Four at a time: all
-        // loads issued before the in-order accumulation, for memory-level parallelism.
-        const T *tt = (const T *)sv.tet_tan;
-        for (; c + 4 <= e; c += 4) {
-            const uint32_t k0 = codes[c], k1 = codes[c + 1], k2 = codes[c + 2], k3 = codes[c + 3];
-            if (code_kind(k3) != KIND_TET) break;
-            TetIsoLoad<T> l0, l1, l2, l3;
-            l0.load(tt, k0);
-            l1.load(tt, k1);
-            l2.load(tt, k2);
-            l3.load(tt, k3);
-            l0.add(acc, stiff, mcoef);
-            l1.add(acc, stiff, mcoef);
-            l2.add(acc, stiff, mcoef);
-            l3.add(acc, stiff, mcoef);
-        }
-    }
-    for (; c < e; c++) {
-        const uint32_t code = codes[c];
-        const uint32_t kind = code_kind(code), u = code_unit(code), q = code_q(code);
-        if (kind == KIND_TET) {
-            if (ISO) tet_block_iso<T>(acc, (const T *)sv.tet_tan + (size_t)u * TET_ISO_REC, q >> 2, q & 3, stiff, mcoef);
-            else tet_block_gen<T>(acc, (const T *)sv.tet_tan + (size_t)u * TET_GEN_REC, q >> 2, q & 3, stiff, mcoef);
-        } else if (kind == KIND_CZ) {
-            cz_block<T>(acc, (const T *)sv.cz_tan + (size_t)u * CZ_TAN_REC, q / 6, q % 6);
-        } else {
-            contact_block(acc, sv.ct + (size_t)u * CT_REC, q >> 2, q & 3);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 9; i++) dst[i] = acc[i];
-}
-
 __device__ inline const double *rhs_of(const Stage &sv, uint32_t code)
 {
     const uint32_t kind = code_kind(code), u = code_unit(code), q = code_q(code);
@@ -175,18 +120,182 @@ __device__ inline const double *rhs_of(const Stage &sv, uint32_t code)
     return sv.ct + (size_t)u * CT_REC + CT_F_OFF + 3 * q;
 }
 
+// Residual part of a diagonal-block code (local block (a, a) of a tet or CZ).
+__device__ inline const double *rhs_of_diag(const Stage &sv, uint32_t code)
+{
+    const uint32_t u = code_unit(code), q = code_q(code);
+    if (code_kind(code) == KIND_TET) return sv.tet_rhs + (size_t)u * TET_RHS_REC + 3 * (q >> 2);
+    return sv.cz_rhs + (size_t)u * CZ_RHS_REC + 3 * (q / 6);
+}
+
+// For each rhs row r: its contribution list must equal, code for code, the list of the
+// diagonal slot its first contributor names (local block (a, a)); then slot_row[slot] = r.
+// Any row without contributions, any mismatch, or two rows on one slot sets *err.
+__global__ __launch_bounds__(256) void k_fuse_check(const int *__restrict__ roff, const uint32_t *__restrict__ rcodes,
+                                                    int N, const int *__restrict__ soff,
+                                                    const uint32_t *__restrict__ scodes, int nnzb, MapSrc tet,
+                                                    MapSrc cz, int *__restrict__ slot_row, int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int b = roff[r], e = roff[r + 1];
+    if (b == e) {
+        atomicOr(err, 1);
+        return;
+    }
+    const uint32_t c0 = rcodes[b];
+    const int u = (int)code_unit(c0), a = (int)code_q(c0);
+    int s = -1;
+    if (code_kind(c0) == KIND_TET && tet.base && u < tet.n && a < 4)
+        s = tet.base[u + (size_t)tet.stride * (tet.field0 + 5 * a)];
+    else if (code_kind(c0) == KIND_CZ && cz.base && u < cz.n && a < 6)
+        s = cz.base[u + (size_t)cz.stride * (cz.field0 + 7 * a)];
+    if (s < 0 || s >= nnzb || soff[s + 1] - soff[s] != e - b) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int sb = soff[s];
+    for (int k = 0; k < e - b; k++) {
+        const uint32_t rc = rcodes[b + k], sc = scodes[sb + k];
+        const uint32_t kind = code_kind(rc), side = kind == KIND_TET ? 5u : 7u;
+        if (kind == KIND_CONTACT || code_kind(sc) != kind || code_unit(sc) != code_unit(rc) ||
+            code_q(sc) != code_q(rc) * side) {
+            atomicOr(err, 1);
+            return;
+        }
+    }
+    if (atomicCAS(&slot_row[s], -1, r) != -1) atomicOr(err, 1);
+}
+
+// One contribution's 3x3 block, as the values that are added into the slot's FP64
+// accumulator (FP32 values in mixed mode for tets and CZs).
+template <bool MIXED, bool ISO, typename L>
+__device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
+{
+    using T = typename std::conditional<MIXED, float, double>::type;
+    const uint32_t kind = code_kind(code), u = code_unit(code), q = code_q(code);
+    const T stiff = (T)sv.stiff, mcoef = (T)sv.mcoef;
+    if (kind == KIND_TET && ISO) {
+        TetIsoLoad<T> l;
+        l.load((const T *)sv.tet_tan, code);
+        T w[9];
+        l.values(w, stiff, mcoef);
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = (L)w[k];
+        return;
+    }
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (kind == KIND_TET)
+        tet_block_gen<T>(acc, (const T *)sv.tet_tan + (size_t)u * TET_GEN_REC, q >> 2, q & 3, stiff, mcoef);
+    else if (kind == KIND_CZ)
+        cz_block<T>(acc, (const T *)sv.cz_tan + (size_t)u * CZ_TAN_REC, q / 6, q % 6);
+    else
+        contact_block(acc, sv.ct + (size_t)u * CT_REC, q >> 2, q & 3);
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = (L)acc[k];
+}
+
+// Phase 2 for BCSR values, one workgroup per chunk of consecutive keys (at most FLAT_KEYS
+// keys and FLAT_CAP contributions; the chunk table is built with the map).  The chunk's
+// contributions are contiguous in `codes`: first every thread evaluates contributions
+// (uniform work across lanes) into LDS, then one thread per key adds its contributions from
+// LDS in list order into an FP64 accumulator and writes the block.  The values and their
+// order are those of a sequential loop over the key's list, so the result does not depend
+// on the chunking.
+//   keys   null: chunk positions are keys; else keys[position] (accumulate mode)
+//   ACC    add to A (single-kernel calls); otherwise overwrite, zero where nothing contributes
+//   RHS    (overwrite only) rows reduced by their diagonal slots: the contributions to rhs
+//          row r are the same units, in the same order, as those to block (r, r) -- checked
+//          when the maps are built (k_fuse_check) -- so the thread of slot s with
+//          slot_row[s] = r >= 0 also sums the units' residual parts into b[3r .. 3r+2].
+// L is the staging type: float for tet and CZ blocks in mixed mode, double otherwise.
+template <bool MIXED, bool ISO, bool ACC, bool RHS, typename L>
+__global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
+                                                           const int *__restrict__ keys, const int *__restrict__ chunks,
+                                                           int nchunks, double *__restrict__ A, Stage sv,
+                                                           const int *__restrict__ slot_row, double *__restrict__ bvec)
+{
+    __shared__ L lds[FLAT_CAP * 9];
+    const int ch = xcd_remap(blockIdx.x, gridDim.x);
+    if (ch >= nchunks) return;
+    const int p0 = chunks[ch], p1 = chunks[ch + 1];
+    const int kfirst = keys ? keys[p0] : p0, klast = keys ? keys[p1 - 1] : p1 - 1;
+    const int cb = off[kfirst], ce = off[klast + 1];
+    const bool staged = ce - cb <= FLAT_CAP;
+    if (staged) {
+        for (int i = threadIdx.x; i < ce - cb; i += FLAT_KEYS) {
+            L v[9];
+            contrib9<MIXED, ISO, L>(sv, codes[cb + i], v);
+#pragma unroll
+            for (int k = 0; k < 9; k++) lds[i * 9 + k] = v[k];
+        }
+        __syncthreads();
+    }
+    const int p = p0 + threadIdx.x;
+    if (p >= p1) return;
+    const int s = keys ? keys[p] : p;
+    const int b = off[s], e = off[s + 1];
+    double acc[9];
+    double *dst = A + (size_t)s * 9;
+    if (ACC) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) acc[i] = dst[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) acc[i] = 0.;
+    }
+    if (staged) {
+        for (int c = b - cb; c < e - cb; c++) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) acc[k] += (double)lds[c * 9 + k];
+        }
+    } else {
+        for (int c = b; c < e; c++) {
+            L v[9];
+            contrib9<MIXED, ISO, L>(sv, codes[c], v);
+#pragma unroll
+            for (int k = 0; k < 9; k++) acc[k] += (double)v[k];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) dst[i] = acc[i];
+    if (RHS) {
+        const int row = slot_row[s];
+        if (row < 0) return;
+        double a0 = 0., a1 = 0., a2 = 0.;
+        int c = b;
+        for (; c + 4 <= e; c += 4) {      // loads of four contributions in flight, added in order
+            const double *r0 = rhs_of_diag(sv, codes[c]), *r1 = rhs_of_diag(sv, codes[c + 1]);
+            const double *r2 = rhs_of_diag(sv, codes[c + 2]), *r3 = rhs_of_diag(sv, codes[c + 3]);
+            const double x0 = r0[0], y0 = r0[1], z0 = r0[2], x1 = r1[0], y1 = r1[1], z1 = r1[2];
+            const double x2 = r2[0], y2 = r2[1], z2 = r2[2], x3 = r3[0], y3 = r3[1], z3 = r3[2];
+            a0 += x0; a1 += y0; a2 += z0;
+            a0 += x1; a1 += y1; a2 += z1;
+            a0 += x2; a1 += y2; a2 += z2;
+            a0 += x3; a1 += y3; a2 += z3;
+        }
+        for (; c < e; c++) {
+            const double *r = rhs_of_diag(sv, codes[c]);
+            a0 += r[0]; a1 += r[1]; a2 += r[2];
+        }
+        double *o = bvec + (size_t)row * 3;
+        o[0] = a0; o[1] = a1; o[2] = a2;
+    }
+}
+
 // Phase 2 for rhs rows (b[row*3+k]) or node forces (dn[node + ns*(F+k)]): one thread per key.
+template <bool ACC>
 __global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off, const uint32_t *__restrict__ codes,
                                                      const int *__restrict__ keys, int nkeys, double *__restrict__ out,
-                                                     int comp_stride, int elem_stride, int accumulate, Stage sv)
+                                                     int comp_stride, int elem_stride, Stage sv)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= nkeys) return;
     const int s = keys ? keys[t] : t;
     const int b = off[s], e = off[s + 1];
     double *dst = out + (size_t)s * elem_stride;
     double a0, a1, a2;
-    if (accumulate) {
+    if (ACC) {
         if (b == e) return;
         a0 = dst[0]; a1 = dst[comp_stride]; a2 = dst[2 * comp_stride];
     } else {

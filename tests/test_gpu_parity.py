@@ -185,3 +185,70 @@ def test_graph_replays_match_eager(orc, specimen_consts):
         lib.icf_graph_destroy(g2)
     lib.icf_graph_destroy(g)
     gf.close()
+
+
+def test_slot_with_more_contributions_than_one_chunk(orc, specimen_consts):
+    """1000 copies of one tet on the same pcsr slots: every slot then has more contributions
+    than the staged reduction holds in LDS (768), which takes its unstaged path.  Checked in
+    both the overwrite (fused) and the accumulate (single-kernel) modes."""
+    from icflow_amd.gpu_functionality import stride_of
+    lib = _lib.load()
+    ctx = ctypes.c_void_p()
+    _lib.check(lib.icf_create(0, ctypes.byref(ctx)))
+    try:
+        _lib.check(lib.icf_set_constants(ctx, ctypes.byref(specimen_consts)), ctx, "consts")
+        rng = np.random.default_rng(11)
+        nn, ns = 4, stride_of(4)
+        dn = np.zeros(18 * ns)
+        x0 = np.array([[0, 0, 0], [0.01, 0, 0], [0, 0.01, 0], [0, 0, 0.01]]) + rng.normal(0, 1e-4, (4, 3))
+        un = rng.normal(0, 1e-5, (4, 3))
+        for k in range(3):
+            dn[k * ns:k * ns + nn] = x0[:, k]
+            dn[(3 + k) * ns:(3 + k) * ns + nn] = un[:, k]
+            dn[(6 + k) * ns:(6 + k) * ns + nn] = rng.normal(0, 1e-3, 4)
+            dn[(9 + k) * ns:(9 + k) * ns + nn] = rng.normal(0, 1e-2, 4)
+            dn[(15 + k) * ns:(15 + k) * ns + nn] = x0[:, k] + un[:, k]
+        T = 1000
+        es = stride_of(T)
+        ie = np.zeros(28 * es, np.int32)
+        up = {}
+        for i in range(4):
+            for j in range(i, 4):
+                up[(i, j)] = len(up)
+        for t in range(T):
+            for i in range(4):
+                ie[t + es * i] = i
+                ie[t + es * (20 + i)] = i
+                for j in range(4):
+                    ie[t + es * (4 + 4 * i + j)] = up[(i, j)] if j >= i else -1
+        nnz, N = len(up), 4
+        ref_A, ref_b = np.zeros(nnz * 9), np.zeros(N * 3)
+        orc.set_constants(specimen_consts)
+        dref = dn.copy()
+        orc.elem_force(ie, dref, H, ref_A, ref_b, T, es, ns)
+
+        def dev(a):
+            p = ctypes.c_void_p()
+            _lib.check(lib.icf_alloc(ctx, a.nbytes, ctypes.byref(p)), ctx, "alloc")
+            _lib.check(lib.icf_h2d(ctx, p, 0, _lib.ptr(a), a.nbytes), ctx, "h2d")
+            return p
+
+        d_dn, d_ie = dev(dn), dev(ie)
+        d_A, d_b = dev(np.zeros(nnz * 9)), dev(np.zeros(N * 3))
+        for fused in (True, False):
+            _lib.check(lib.icf_h2d(ctx, d_dn, 0, _lib.ptr(dn), dn.nbytes), ctx, "h2d")
+            _lib.check(lib.icf_memset(ctx, d_A, 0, nnz * 72), ctx, "memset")
+            _lib.check(lib.icf_memset(ctx, d_b, 0, N * 24), ctx, "memset")
+            if fused:
+                _lib.check(lib.icf_assemble_elems_and_czs(ctx, d_ie, T, es, None, None, 0, 0, d_dn, ns, H, d_A,
+                                                          nnz, d_b, N, None, None), ctx, "fused")
+            else:
+                _lib.check(lib.icf_elem_force(ctx, d_ie, d_dn, H, d_A, d_b, T, es, ns), ctx, "elem")
+            A, b = np.zeros(nnz * 9), np.zeros(N * 3)
+            _lib.check(lib.icf_d2h(ctx, _lib.ptr(A), d_A, 0, A.nbytes), ctx, "d2h")
+            _lib.check(lib.icf_d2h(ctx, _lib.ptr(b), d_b, 0, b.nbytes), ctx, "d2h")
+            print("fused" if fused else "single", rel_err(A, ref_A), rel_err(b, ref_b), flush=True)
+            assert rel_err(A, ref_A) <= TOL64
+            assert rel_err(b, ref_b) <= TOL64
+    finally:
+        lib.icf_destroy(ctx)
