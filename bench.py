@@ -249,6 +249,24 @@ def kernel_bytes(alg, name):
     return alg.get(name, alg.get(base))
 
 
+def broad_phase_timing(r, reps=5):
+    """icf_broad_phase over the bench's surface tets: pair count (equal to the host k-DOP
+    grid's) and milliseconds per call (blocking, includes its host count read-back)."""
+    gf, lib, ctx = r.gf, r.lib, r.gf.ctx
+    nS = len(r.col.surf)
+    d = gf.buf("bp_bench", 2 * r.Q, np.int32)
+    n = ctypes.c_longlong()
+    r.check(lib.icf_broad_phase(ctx, gf.g_dn.p, gf.nd_stride, gf.g_ie.p, gf.el_all_stride, nS, 1, d.p, r.Q,
+                                ctypes.byref(n)), ctx, "icf_broad_phase")
+    same = bool(n.value == r.Q and np.array_equal(d.download(2 * r.Q), r.pairs.ravel()))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.icf_broad_phase(ctx, gf.g_dn.p, gf.nd_stride, gf.g_ie.p, gf.el_all_stride, nS, 1, d.p, r.Q,
+                            ctypes.byref(n))
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    return {"pairs": int(n.value), "same_as_host": same, "ms_per_call": round(ms, 3)}
+
+
 def cpu_baseline(runner, seconds):
     """The CPU oracle (plain C, one core) over the same iteration's element, CZ and contact
     passes, repeated for about `seconds`."""
@@ -287,6 +305,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="precision modes to time (fp64, mixed)")
+    ap.add_argument("--newton-steps", type=int, default=10,
+                    help="steps of the FP64-vs-mixed three-point-bend Newton run (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run icf_d2d over this many MiB (a plain copy of known size, for "
                          "calibrating hardware byte counters)")
@@ -351,6 +371,17 @@ def main():
                 avg_us=round(avg_s * 1e6, 2))
     it_bytes = iteration_bytes(r)
 
+    # device broad phase over the same surface tets (reported beside the step; the reference's
+    # broad phase is a host BVH outside the assembly path)
+    bp = broad_phase_timing(r)
+    log(f"device broad phase: {bp}")
+    study = None
+    if args.newton_steps > 0:
+        from icflow_amd import newton
+        t0 = time.time()
+        study = newton.precision_study(consts, steps=args.newton_steps, log=log)
+        study["wall_s"] = round(time.time() - t0, 1)
+        log(f"precision study: {study}")
     cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -378,6 +409,8 @@ def main():
         "mixed_rel_err_vs_fp64": errs,
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
         "roofline": roof,
+        "device_broad_phase": bp,
+        "precision_study": study,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

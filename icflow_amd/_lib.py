@@ -15,6 +15,7 @@ c_int, c_double, c_void_p, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_vo
 P_int = ctypes.POINTER(ctypes.c_int)
 
 ICF_FP64, ICF_MIXED = 0, 1
+ICF_E_INVALID, ICF_E_NOCONST, ICF_E_CAPACITY, ICF_E_CONSTANTS, ICF_E_INDEX, ICF_E_CAPTURE = -1, -2, -3, -4, -5, -6
 ERRORS = {-1: "ICF_E_INVALID", -2: "ICF_E_NOCONST", -3: "ICF_E_CAPACITY", -4: "ICF_E_CONSTANTS",
           -5: "ICF_E_INDEX", -6: "ICF_E_CAPTURE"}
 
@@ -92,6 +93,17 @@ _SIGS = {
     "icfh_nl2set": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int, P_int]),
     "icfh_kdop_pairs": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_longlong,
                                 ctypes.POINTER(ctypes.c_longlong)]),
+    "icf_nodes_infer": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double]),
+    "icf_nodes_add_dx": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "icf_nodes_accept": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double]),
+    "icf_build_pattern": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                  c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, ctypes.POINTER(c_int)]),
+    "icf_pack_pcsr": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int,
+                              c_void_p, c_int, c_int, c_void_p, c_int, c_int]),
+    "icf_broad_phase": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_longlong,
+                                ctypes.POINTER(ctypes.c_longlong)]),
+    "icf_nl2set": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
+                           ctypes.POINTER(c_int)]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -37,7 +37,7 @@ struct PprPow {
     double b0, b1, b2;  // (p/a + D/delta)^{p, p-1, p-2}
 };
 
-__device__ inline PprPow ppr_pow_n(double Dn)
+__device__ __attribute__((always_inline)) inline PprPow ppr_pow_n(double Dn)
 {
     PprPow r;
     if (Dn == 0.0) {
@@ -47,11 +47,11 @@ __device__ inline PprPow ppr_pow_n(double Dn)
     }
     double d = Dn / g_c.deln;
     pow3(1.0 - d, g_c.alpha, r.a0, r.a1, r.a2);
-    pow3(g_c.p_m / g_c.alpha + d, g_c.p_m, r.b0, r.b1, r.b2);
+    pow3(g_c.pm_alpha + d, g_c.p_m, r.b0, r.b1, r.b2);
     return r;
 }
 
-__device__ inline PprPow ppr_pow_t(double Dt)
+__device__ __attribute__((always_inline)) inline PprPow ppr_pow_t(double Dt)
 {
     PprPow r;
     if (Dt == 0.0) {
@@ -61,7 +61,7 @@ __device__ inline PprPow ppr_pow_t(double Dt)
     }
     double d = Dt / g_c.delt;
     pow3(1.0 - d, g_c.beta, r.a0, r.a1, r.a2);
-    pow3(g_c.p_n / g_c.beta + d, g_c.p_n, r.b0, r.b1, r.b2);
+    pow3(g_c.pn_beta + d, g_c.p_n, r.b0, r.b1, r.b2);
     return r;
 }
 
@@ -76,23 +76,23 @@ __device__ inline double pprZ(const PprPow &w, double p, double a)
 }
 __device__ inline double Tn_at(const PprPow &n, const PprPow &t)
 {
-    return (g_c.gam_n / g_c.deln) * pprX(n, g_c.p_m, g_c.alpha) * (g_c.gam_t * t.a0 * t.b0 + g_c.pMtn);
+    return g_c.k_tn * pprX(n, g_c.p_m, g_c.alpha) * (g_c.gam_t * t.a0 * t.b0 + g_c.pMtn);
 }
 __device__ inline double Tt_at(const PprPow &n, const PprPow &t)
 {
-    return (g_c.gam_t / g_c.delt) * pprX(t, g_c.p_n, g_c.beta) * (g_c.gam_n * n.a0 * n.b0 + g_c.pMnt);
+    return g_c.k_tt * pprX(t, g_c.p_n, g_c.beta) * (g_c.gam_n * n.a0 * n.b0 + g_c.pMnt);
 }
 __device__ inline double Dnn_at(const PprPow &n, const PprPow &t)
 {
-    return (g_c.gam_n / (g_c.deln * g_c.deln)) * pprZ(n, g_c.p_m, g_c.alpha) * (g_c.gam_t * t.a0 * t.b0 + g_c.pMtn);
+    return g_c.k_dnn * pprZ(n, g_c.p_m, g_c.alpha) * (g_c.gam_t * t.a0 * t.b0 + g_c.pMtn);
 }
 __device__ inline double Dtt_at(const PprPow &n, const PprPow &t)
 {
-    return (g_c.gam_t / (g_c.delt * g_c.delt)) * pprZ(t, g_c.p_n, g_c.beta) * (g_c.gam_n * n.a0 * n.b0 + g_c.pMnt);
+    return g_c.k_dtt * pprZ(t, g_c.p_n, g_c.beta) * (g_c.gam_n * n.a0 * n.b0 + g_c.pMnt);
 }
 __device__ inline double Dnt_at(const PprPow &n, const PprPow &t)
 {
-    return (g_c.gam_n * g_c.gam_t / (g_c.deln * g_c.delt)) * pprX(n, g_c.p_m, g_c.alpha) * pprX(t, g_c.p_n, g_c.beta);
+    return g_c.k_dnt * pprX(n, g_c.p_m, g_c.alpha) * pprX(t, g_c.p_n, g_c.beta);
 }
 
 // PPR law with history, one Gauss point.  Same branches and failure tests as
@@ -109,27 +109,31 @@ __device__ inline void ppr_law(bool &contact, bool &failed, double &pmax, double
     }
     contact = (opn < 0);
     const double eps = -1e-9;
-    const double thr_t = g_c.f_tt * 0.05, thr_n = g_c.f_tn * 0.05;
-    const PprPow n0 = ppr_pow_n(0.0);
+    const double thr_t = g_c.thr_t, thr_n = g_c.thr_n;
+    // Every branch below draws on the powers at these openings only: evaluated once, up
+    // front, so that a wavefront whose lanes take different branches does not evaluate
+    // them once per branch.  (At zero opening the powers are constants.)
+    const PprPow n0 = ppr_pow_n(0.0), t0 = ppr_pow_t(0.0);
+    const PprPow n_opn = ppr_pow_n(contact ? 0.0 : opn);
+    const PprPow n_pm = ppr_pow_n(pmax);
+    const PprPow t_opt = ppr_pow_t(opt);
+    const PprPow t_tm = ppr_pow_t(tmax);
     if (contact) {
-        const PprPow t_opt = ppr_pow_t(opt);
-        const PprPow t_tmax = ppr_pow_t(tmax);
         if (pmax != 0) {
-            const PprPow n_pm = ppr_pow_n(pmax);
-            double peak = Tn_at(n_pm, t_tmax);
+            double peak = Tn_at(n_pm, t_tm);
             Tn = peak * opn / pmax;
             Dnn = peak / pmax;
         } else {
-            Dnn = Dnn_at(n0, t_tmax);
+            Dnn = Dnn_at(n0, t_tm);
             Tn = Dnn * opn;
         }
         Tt = Tt_at(n0, t_opt);
-        if (Tt >= eps && !(opt > g_c.delt * g_c.rt / 5 && Tt < thr_t)) {
+        if (Tt >= eps && !(opt > g_c.lim_t && Tt < thr_t)) {
             if (opt >= tmax) {
                 tmax = opt;
                 Dtt = Dtt_at(n0, t_opt);
             } else {
-                double peak = Tt_at(n0, t_tmax);
+                double peak = Tt_at(n0, t_tm);
                 Tt = peak * opt / tmax;
                 Dtt = peak / tmax;
             }
@@ -139,12 +143,9 @@ __device__ inline void ppr_law(bool &contact, bool &failed, double &pmax, double
             failed = true;
         }
     } else {
-        const PprPow n_opn = ppr_pow_n(opn);
-        const PprPow t_opt = ppr_pow_t(opt);
         Tt = Tt_at(n_opn, t_opt);
         Tn = Tn_at(n_opn, t_opt);
-        if (Tt >= eps && Tn >= eps && !(opt > g_c.delt * g_c.rt / 5 && Tt < thr_t) &&
-            !(opn > g_c.deln * g_c.rn / 5 && Tn < thr_n)) {
+        if (Tt >= eps && Tn >= eps && !(opt > g_c.lim_t && Tt < thr_t) && !(opn > g_c.lim_n && Tn < thr_n)) {
             const bool tsoft = (opt >= tmax), nsoft = (opn >= pmax);
             if (tsoft && nsoft) {
                 tmax = opt;
@@ -153,37 +154,32 @@ __device__ inline void ppr_law(bool &contact, bool &failed, double &pmax, double
                 Dnt = Dnt_at(n_opn, t_opt);
                 Dtt = Dtt_at(n_opn, t_opt);
             } else if (tsoft && !nsoft) {
-                const PprPow n_pm = ppr_pow_n(pmax);
                 Dnt = 0;
                 if (pmax != 0) {
-                    double peak = Tn_at(n_pm, ppr_pow_t(tmax));
+                    double peak = Tn_at(n_pm, t_tm);
                     Tn = peak * opn / pmax;
                     Dnn = peak / pmax;
                 } else {
                     Tn = 0;
-                    Dnn = Dnn_at(n0, ppr_pow_t(tmax));
+                    Dnn = Dnn_at(n0, t_tm);
                 }
                 tmax = opt;
                 Tt = Tt_at(n_pm, t_opt);
                 Dtt = Dtt_at(n_pm, t_opt);
             } else if (!tsoft && nsoft) {
-                const PprPow n_pm = ppr_pow_n(pmax);
                 Dnt = 0;
                 if (tmax != 0) {
-                    double peak = Tt_at(n_pm, ppr_pow_t(tmax));
+                    double peak = Tt_at(n_pm, t_tm);
                     Tt = peak * opt / tmax;
                     Dtt = peak / tmax;
                 } else {
                     Tt = 0;
-                    Dtt = Dtt_at(n_pm, ppr_pow_t(0.0));
+                    Dtt = Dtt_at(n_pm, t0);
                 }
                 pmax = opn;
-                const PprPow t_tm = ppr_pow_t(tmax);
                 Tn = Tn_at(n_opn, t_tm);
                 Dnn = Dnn_at(n_opn, t_tm);
             } else {
-                const PprPow n_pm = ppr_pow_n(pmax);
-                const PprPow t_tm = ppr_pow_t(tmax);
                 Dnt = 0;
                 double peakN = Tn_at(n_pm, t_tm);
                 if (pmax != 0) {
@@ -199,7 +195,7 @@ __device__ inline void ppr_law(bool &contact, bool &failed, double &pmax, double
                     Dtt = peakT / tmax;
                 } else {
                     Tt = 0;
-                    Dtt = Dtt_at(n_pm, ppr_pow_t(0.0));
+                    Dtt = Dtt_at(n_pm, t0);
                 }
             }
         } else {

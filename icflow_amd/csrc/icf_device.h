@@ -57,6 +57,13 @@ struct DevConsts {
     double lam, mu;         // isotropic split of E (valid when iso != 0)
     double m4[16];          // M[(3i)*12 + 3j]: the mass block (i,j) is m4[4i+j] I3 when iso != 0
     int iso;
+    // constant factors of the PPR expressions, formed on the host with the same operations
+    // and order as in PPR_math.h / cz_kernels.h, so they are the same doubles
+    double k_tn, k_tt;              // gam_n / deln, gam_t / delt
+    double k_dnn, k_dtt, k_dnt;     // gam_n / (deln deln), gam_t / (delt delt), gam_n gam_t / (deln delt)
+    double pm_alpha, pn_beta;       // p_m / alpha, p_n / beta
+    double thr_n, thr_t;            // f_tn * 0.05, f_tt * 0.05
+    double lim_n, lim_t;            // deln * rn / 5, delt * rt / 5
     // powers at zero opening: (1 - 0)^e = 1 and (p/a + 0)^e, used by Dnn_(0, .) etc.
     double bn0[3], bt0[3];  // (p_m/alpha)^{p_m, p_m-1, p_m-2}, (p_n/beta)^{p_n, p_n-1, p_n-2}
     float Ef[36];

@@ -13,6 +13,7 @@
 #include <list>
 #include <type_traits>
 #include <atomic>
+#include <algorithm>
 
 #include "../../include/icflow_hip.h"
 
@@ -26,6 +27,9 @@ __constant__ DevConsts g_c;
 #include "icf_contact.inc.h"
 #include "icf_contact_resp.inc.h"
 #include "icf_reduce.inc.h"
+#include "icf_nodes.inc.h"
+#include "icf_pattern.inc.h"
+#include "icf_broad.inc.h"
 
 using namespace icf;
 
@@ -66,6 +70,10 @@ struct SlotMap {
     int fused_with = 0;
     bool fuse_ok = false;
     int *slot_row = nullptr;
+    // copy of the key fields the map was built from; after a write that may have touched them
+    // (suspect) the map is kept if they still compare equal
+    int *snap = nullptr;
+    bool suspect = false;
 };
 
 std::atomic<int> g_next_ctx_id{1};
@@ -82,6 +90,8 @@ struct icf_ctx {
     int precision = ICF_FP64;
     DevConsts hc;
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
+    DevBuf pat_a, pat_b;           // pattern build / broad phase / NL2set: 64-bit keys
+    DevBuf bp_f, bp_k, bp_i, bp_l;   // broad phase and NL2set scratch (doubles, keys, ints, int64)
     int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     int next_map_id = 0;
@@ -140,6 +150,8 @@ void free_map(SlotMap &m)
     if (m.keys) (void)hipFree(m.keys);
     if (m.slot_row) (void)hipFree(m.slot_row);
     if (m.chunks) (void)hipFree(m.chunks);
+    if (m.snap) (void)hipFree(m.snap);
+    m.snap = nullptr;
     m.chunks = nullptr;
     m.off = nullptr;
     m.codes = nullptr;
@@ -147,8 +159,9 @@ void free_map(SlotMap &m)
     m.slot_row = nullptr;
 }
 
-// Drop cached maps whose key fields overlap [lo, hi).
-void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi)
+// Maps whose key fields overlap [lo, hi): dropped when the memory goes away (erase), else
+// marked suspect, to be compared against their snapshot before the next use.
+void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi, bool erase = false)
 {
     for (auto it = ctx->maps.begin(); it != ctx->maps.end();) {
         bool hit = false;
@@ -157,11 +170,12 @@ void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi)
             uintptr_t b = (uintptr_t)(s.base + (size_t)s.stride * (s.field0 + s.nq));
             if (a < hi && lo < b) hit = true;
         }
-        if (hit) {
+        if (hit && erase) {
             (void)hipStreamSynchronize(ctx->stream);
             free_map(*it);
             it = ctx->maps.erase(it);
         } else {
+            if (hit) it->suspect = true;
             ++it;
         }
     }
@@ -238,11 +252,37 @@ void prof_flush(icf_ctx *ctx)
 // Build (or fetch) the slot map for `srcs` with keys of `target`.
 int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey> &srcs, SlotMap **out)
 {
-    for (SlotMap &m : ctx->maps)
-        if (m.target == target && m.fixed_nkeys == fixed_nkeys && m.srcs == srcs) {
-            *out = &m;
-            return 0;
+    for (auto it = ctx->maps.begin(); it != ctx->maps.end(); ++it) {
+        SlotMap &m = *it;
+        if (m.target != target || m.fixed_nkeys != fixed_nkeys || !(m.srcs == srcs)) continue;
+        if (m.suspect) {
+            if (ctx->capturing)
+                return fail(ctx, ICF_E_CAPTURE, "a slot map must be checked during capture: run the call once uncaptured");
+            int *d_diff = ctx->small + 1;
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_diff, 1, 0);
+            long long off = 0;
+            for (const SrcKey &k : srcs) {
+                if (k.n == 0) continue;
+                MapSrc ms{k.base, k.n, k.stride, k.field0, k.nq, k.kind};
+                const long long tot = (long long)k.n * k.nq;
+                hipLaunchKernelGGL(k_snap_cmp, dim3(blocks_for(tot, 256)), dim3(256), 0, ctx->stream, ms,
+                                   m.snap + off, d_diff);
+                off += tot;
+            }
+            int diff = 1;
+            HIPCHK(ctx, hipGetLastError());
+            HIPCHK(ctx, hipMemcpyAsync(&diff, d_diff, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            if (diff) {
+                free_map(m);
+                ctx->maps.erase(it);
+                break;
+            }
+            m.suspect = false;
         }
+        *out = &m;
+        return 0;
+    }
     if (ctx->capturing)
         return fail(ctx, ICF_E_CAPTURE, "a slot map would be built during capture: run the call once uncaptured");
     for (const SrcKey &s : srcs)
@@ -364,6 +404,19 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
     }
     if (nkeys > 0)
         hipLaunchKernelGGL(k_map_sort, dim3(blocks_for(nkeys, 256)), dim3(256), 0, ctx->stream, m.off, m.codes, nkeys);
+    {
+        long long snap_n = 0;
+        for (const SrcKey &s : srcs) snap_n += (long long)s.n * s.nq;
+        HIPCHK(ctx, hipMalloc(&m.snap, sizeof(int) * (size_t)(snap_n + 1)));
+        long long off = 0;
+        for (const SrcKey &s : srcs) {
+            if (s.n == 0) continue;
+            MapSrc ms{s.base, s.n, s.stride, s.field0, s.nq, s.kind};
+            const long long tot = (long long)s.n * s.nq;
+            hipLaunchKernelGGL(k_snap_copy, dim3(blocks_for(tot, 256)), dim3(256), 0, ctx->stream, ms, m.snap + off);
+            off += tot;
+        }
+    }
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx, hipFree(d_cnt));
@@ -584,7 +637,8 @@ int icf_destroy(icf_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (SlotMap &m : ctx->maps) free_map(m);
-    DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct, &ctx->scan_tmp};
+    DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
+                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->small) (void)hipFree(ctx->small);
@@ -661,6 +715,17 @@ int icf_set_constants(icf_ctx *ctx, const icf_consts *c)
         d.lamf = (float)lam;
         d.muf = (float)mu;
     }
+    d.k_tn = c->gam_n / c->deln;
+    d.k_tt = c->gam_t / c->delt;
+    d.k_dnn = c->gam_n / (c->deln * c->deln);
+    d.k_dtt = c->gam_t / (c->delt * c->delt);
+    d.k_dnt = c->gam_n * c->gam_t / (c->deln * c->delt);
+    d.pm_alpha = c->p_m / c->alpha;
+    d.pn_beta = c->p_n / c->beta;
+    d.thr_n = c->f_tn * 0.05;
+    d.thr_t = c->f_tt * 0.05;
+    d.lim_n = c->deln * c->rn / 5;
+    d.lim_t = c->delt * c->rt / 5;
     for (int e = 0; e < 3; e++) {
         d.bn0[e] = pow(c->p_m / c->alpha, c->p_m - e);
         d.bt0[e] = pow(c->p_n / c->beta, c->p_n - e);
@@ -701,9 +766,9 @@ int icf_free(icf_ctx *ctx, void *dptr)
     size_t sz = 0;
     void *basep = nullptr;
     if (hipMemGetAddressRange(&basep, &sz, dptr) == hipSuccess && sz)
-        invalidate_range(ctx, (uintptr_t)basep, (uintptr_t)basep + sz);
+        invalidate_range(ctx, (uintptr_t)basep, (uintptr_t)basep + sz, true);
     else
-        invalidate_range(ctx, (uintptr_t)dptr, (uintptr_t)dptr + 1);
+        invalidate_range(ctx, (uintptr_t)dptr, (uintptr_t)dptr + 1, true);
     HIPCHK(ctx, hipFree(dptr));
     return 0;
 }
@@ -757,9 +822,7 @@ int icf_pattern_changed(icf_ctx *ctx)
 {
     if (int r = check_ctx(ctx)) return r;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_pattern_changed during capture");
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    for (SlotMap &m : ctx->maps) free_map(m);
-    ctx->maps.clear();
+    for (SlotMap &m : ctx->maps) m.suspect = true;
     return 0;
 }
 
@@ -1050,6 +1113,258 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
         if (nCZDamaged) *nCZDamaged = sums[0];
         if (nCZFailed) *nCZFailed = sums[1];
     }
+    return 0;
+}
+
+int icf_nodes_infer(icf_ctx *ctx, const double *st, int st_stride, double *dn, int nd_stride, int nNodes, double h,
+                    double beta, double gamma)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nNodes, st_stride) || bad_units(nNodes, nd_stride) || (nNodes && (!st || !dn)) || !(h > 0) ||
+        !(beta > 0))
+        return fail(ctx, ICF_E_INVALID, "icf_nodes_infer: invalid argument");
+    if (nNodes == 0) return 0;
+    Timer tm(ctx);
+    LAUNCH(ctx, "k_nodes_infer", k_nodes_infer, dim3(blocks_for(3LL * nNodes, 256)), dim3(256), st, st_stride, dn,
+           nd_stride, nNodes, h, beta, gamma);
+    return tm.finish();
+}
+
+int icf_nodes_add_dx(icf_ctx *ctx, double *st, int st_stride, const int *altId, int nNodes, const double *dx)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nNodes, st_stride) || (nNodes && (!st || !altId || !dx)))
+        return fail(ctx, ICF_E_INVALID, "icf_nodes_add_dx: invalid argument");
+    if (nNodes == 0) return 0;
+    Timer tm(ctx);
+    LAUNCH(ctx, "k_nodes_add_dx", k_nodes_add_dx, dim3(blocks_for(3LL * nNodes, 256)), dim3(256), st, st_stride,
+           altId, nNodes, dx);
+    return tm.finish();
+}
+
+int icf_nodes_accept(icf_ctx *ctx, double *st, int st_stride, const double *dn, int nd_stride, int nNodes, double h)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nNodes, st_stride) || bad_units(nNodes, nd_stride) || (nNodes && (!st || !dn)) || !(h > 0))
+        return fail(ctx, ICF_E_INVALID, "icf_nodes_accept: invalid argument");
+    if (nNodes == 0) return 0;
+    Timer tm(ctx);
+    LAUNCH(ctx, "k_nodes_accept", k_nodes_accept, dim3(blocks_for(3LL * nNodes, 256)), dim3(256), st, st_stride, dn,
+           nd_stride, nNodes, h);
+    return tm.finish();
+}
+
+int icf_build_pattern(icf_ctx *ctx, const int *altId, int nNodes, const int *ie, int nElems, int el_stride,
+                      const int *icz, int nCZs, int cz_stride, const int *icr, int nImpacts, int cr_stride, int N,
+                      int *csr_rows, int *csr_cols, int max_nnz, int *nnz)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (!nnz || nNodes < 0 || N < 0 || max_nnz < 0 || bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) ||
+        bad_units(nImpacts, cr_stride) || (nNodes && !altId) || (nElems && !ie) || (nCZs && !icz) ||
+        (nImpacts && !icr) || !csr_rows || (max_nnz && !csr_cols))
+        return fail(ctx, ICF_E_INVALID, "icf_build_pattern: invalid argument");
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_build_pattern returns a host value");
+    *nnz = 0;
+    PatSrc src[3] = {{ie, nElems, el_stride, EL_NODES, 4, 0, 0},
+                     {icz, nCZs, cz_stride, CZI_VRTS, 6, 0, 0},
+                     {icr, nImpacts, cr_stride, CR_NODES, 4, 1, 0}};
+    long long M = 0;
+    for (PatSrc &p : src) {
+        p.out0 = M;
+        M += (long long)p.n * pat_pairs(p.K, p.strict);
+    }
+    if (M >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: more than 2^31 candidate pairs");
+    int *d_err = ctx->small + 1, *d_num = ctx->small + 5;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+    int num = 0;
+    unsigned long long last = PAT_NONE;
+    if (M > 0) {
+        if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)M)) return r;
+        if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)M)) return r;
+        auto *a = (unsigned long long *)ctx->pat_a.p, *b = (unsigned long long *)ctx->pat_b.p;
+        for (const PatSrc &p : src)
+            if (p.n)
+                hipLaunchKernelGGL(k_pat_emit, dim3(blocks_for((long long)p.n * pat_pairs(p.K, p.strict), 256)),
+                                   dim3(256), 0, ctx->stream, p, altId, nNodes, N, a, d_err);
+        size_t sb = 0, ub = 0;
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sb, a, b, (int)M, 0, 64, ctx->stream));
+        HIPCHK(ctx, hipcub::DeviceSelect::Unique(nullptr, ub, b, a, d_num, (int)M, ctx->stream));
+        if (int r = grow(ctx, ctx->scan_tmp, sb > ub ? sb : ub)) return r;
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, sb, a, b, (int)M, 0, 64, ctx->stream));
+        HIPCHK(ctx, hipcub::DeviceSelect::Unique(ctx->scan_tmp.p, ub, b, a, d_num, (int)M, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(&num, d_num, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (num > 0)
+            HIPCHK(ctx, hipMemcpyAsync(&last, a + num - 1, sizeof(last), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    int err = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return fail(ctx, ICF_E_INDEX, "icf_build_pattern: a node id or altId is out of range");
+    const int n_nz = num - (last == PAT_NONE && num > 0 ? 1 : 0);
+    *nnz = n_nz;
+    if (n_nz > max_nnz) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: more blocks than max_nnz");
+    const unsigned long long *u = (const unsigned long long *)ctx->pat_a.p;
+    if (n_nz > 0)
+        hipLaunchKernelGGL(k_pat_cols, dim3(blocks_for(n_nz, 256)), dim3(256), 0, ctx->stream, u, n_nz, csr_cols);
+    hipLaunchKernelGGL(k_pat_rows, dim3(blocks_for((long long)N + 1, 256)), dim3(256), 0, ctx->stream, u, n_nz, N,
+                       csr_rows);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_pack_pcsr(icf_ctx *ctx, const int *altId, int nNodes, const int *csr_rows, const int *csr_cols, int N,
+                  int *ie, int nElems, int el_stride, int *icz, int nCZs, int cz_stride, int *icr, int nImpacts,
+                  int cr_stride)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (nNodes < 0 || N < 0 || bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) ||
+        bad_units(nImpacts, cr_stride) || (nNodes && !altId) || !csr_rows || (nElems && !ie) || (nCZs && !icz) ||
+        (nImpacts && !icr))
+        return fail(ctx, ICF_E_INVALID, "icf_pack_pcsr: invalid argument");
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_pack_pcsr reports index errors to the host");
+    int *d_err = ctx->small + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+    PackSrc ps[3] = {{ie, nElems, el_stride, EL_NODES, EL_SLOTS, EL_ROWS, 4},
+                     {icz, nCZs, cz_stride, CZI_VRTS, CZI_SLOTS, CZI_ROWS, 6},
+                     {icr, nImpacts, cr_stride, CR_NODES, CR_SLOTS, CR_ROWS, 4}};
+    for (const PackSrc &p : ps) {
+        if (!p.n) continue;
+        hipLaunchKernelGGL(k_pack_pcsr, dim3(blocks_for((long long)p.n * p.K, 256)), dim3(256), 0, ctx->stream, p,
+                           altId, nNodes, csr_rows, csr_cols, N, d_err);
+        // slot and row fields changed: maps built from them are checked before their next use
+        invalidate_range(ctx, (uintptr_t)(p.base + (size_t)p.stride * p.slot0),
+                         (uintptr_t)(p.base + (size_t)p.stride * (p.row0 + p.K)));
+    }
+    if (nCZs)
+        hipLaunchKernelGGL(k_pack_cz_flags, dim3(blocks_for(nCZs, 256)), dim3(256), 0, ctx->stream, icz, nCZs,
+                           cz_stride);
+    int err = 0;
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return fail(ctx, ICF_E_INDEX, "icf_pack_pcsr: a node id is out of range or a pair is not in the pattern");
+    return 0;
+}
+
+int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie, int el_all_stride, int nTets,
+                    int exclude_shared, int *pairs, long long max_pairs, long long *nOut)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (!nOut || bad_units(nTets, el_all_stride) || nd_stride < 0 || max_pairs < 0 || (nTets && (!dn || !ie)) ||
+        (max_pairs && !pairs))
+        return fail(ctx, ICF_E_INVALID, "icf_broad_phase: invalid argument");
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_broad_phase returns a host value");
+    *nOut = 0;
+    if (nTets == 0) return 0;
+    const int nT = nTets;
+    // scratch: lo, hi (12 nT each), ext (nT), mn (3 nT), red (4)
+    if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)28 * nT + 4))) return r;
+    if (int r = grow(ctx, ctx->bp_k, sizeof(unsigned long long) * 2 * (size_t)nT)) return r;
+    if (int r = grow(ctx, ctx->bp_i, sizeof(int) * 2 * (size_t)nT)) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2 * ((size_t)nT + 1))) return r;
+    double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT, *ext = hi + (size_t)12 * nT,
+           *mn = ext + nT, *red = mn + (size_t)3 * nT;
+    auto *key = (unsigned long long *)ctx->bp_k.p, *skey = key + nT;
+    int *idx = (int *)ctx->bp_i.p, *sidx = idx + nT;
+    long long *cnt = (long long *)ctx->bp_l.p, *off = cnt + nT + 1;
+    int *d_err = ctx->small + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+    hipLaunchKernelGGL(k_bp_kdop, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, dn, nd_stride, ie,
+                       el_all_stride, nT, lo, hi, ext, mn, d_err, nd_stride);
+    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    HIPCHK(ctx, hipcub::DeviceReduce::Max(nullptr, b1, ext, red, nT, ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceReduce::Min(nullptr, b2, mn, red + 1, nT, ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b3, key, skey, idx, sidx, nT, 0, 3 * BP_CELL_BITS,
+                                                   ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, cnt, off, nT + 1, ctx->stream));
+    size_t tb = std::max(std::max(b1, b2), std::max(b3, b4));
+    if (int r = grow(ctx, ctx->scan_tmp, tb)) return r;
+    HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b1, ext, red, nT, ctx->stream));
+    for (int k = 0; k < 3; k++)
+        HIPCHK(ctx, hipcub::DeviceReduce::Min(ctx->scan_tmp.p, b2, mn + (size_t)k * nT, red + 1 + k, nT, ctx->stream));
+    hipLaunchKernelGGL(k_bp_cell, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, nT, red, key, idx, d_err);
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, b3, key, skey, idx, sidx, nT, 0,
+                                                   3 * BP_CELL_BITS, ctx->stream));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)(cnt + nT), 2, 0);
+    hipLaunchKernelGGL((k_bp_pairs<false>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, skey,
+                       sidx, ie, el_all_stride, exclude_shared, red, cnt, nullptr, nullptr);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b4, cnt, off, nT + 1, ctx->stream));
+    long long total = 0;
+    int err = 0;
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(&total, off + nT, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
+    *nOut = total;
+    if (total > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
+    if (total == 0) return 0;
+    if (total >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 pairs");
+    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)total)) return r;
+    if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)total)) return r;
+    auto *pa = (unsigned long long *)ctx->pat_a.p, *pb = (unsigned long long *)ctx->pat_b.p;
+    hipLaunchKernelGGL((k_bp_pairs<true>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, skey,
+                       sidx, ie, el_all_stride, exclude_shared, red, nullptr, off, pa);
+    size_t b5 = 0;
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, b5, pa, pb, (int)total, 0, 64, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, b5, pa, pb, (int)total, 0, 64, ctx->stream));
+    hipLaunchKernelGGL(k_bp_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, pb, total, pairs);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int icf_nl2set(icf_ctx *ctx, const int *masks, const int *pairs, int nPairs, const int *ie, int el_all_stride,
+               int *nl2, int tet_stride, int *nOut)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (!nOut || nPairs < 0 || el_all_stride < 0 || tet_stride < 0 || (nPairs && (!masks || !pairs || !ie)) ||
+        (tet_stride && !nl2))
+        return fail(ctx, ICF_E_INVALID, "icf_nl2set: invalid argument");
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_nl2set returns a host value");
+    *nOut = 0;
+    if (nPairs == 0) return 0;
+    if (int r = grow(ctx, ctx->bp_i, sizeof(int) * 2 * ((size_t)nPairs + 1))) return r;
+    int *cnt = (int *)ctx->bp_i.p, *off = cnt + nPairs + 1;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, cnt + nPairs, 1, 0);
+    hipLaunchKernelGGL(k_nl2_count, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, masks, nPairs, cnt);
+    size_t b1 = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b1, cnt, off, nPairs + 1, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, b1)) return r;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b1, cnt, off, nPairs + 1, ctx->stream));
+    int total = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&total, off + nPairs, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (total == 0) return 0;
+    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * 2 * (size_t)total)) return r;
+    if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * 2 * (size_t)total)) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(int) * 4 * (size_t)total + sizeof(int))) return r;
+    auto *k0 = (unsigned long long *)ctx->pat_a.p, *k1 = k0 + total;
+    auto *k2 = (unsigned long long *)ctx->pat_b.p, *k3 = k2 + total;
+    int *p0 = (int *)ctx->bp_l.p, *p1 = p0 + total, *p2 = p1 + total, *p3 = p2 + total, *d_num = p3 + total;
+    hipLaunchKernelGGL(k_nl2_emit, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, masks, pairs, nPairs, ie,
+                       el_all_stride, off, k0, p0);
+    size_t s1 = 0, s2 = 0, s3 = 0;
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, s1, k0, k1, p0, p1, total, 0, 64, ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceSelect::UniqueByKey(nullptr, s2, k1, p1, k2, p2, d_num, total, ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, s3, p2, p3, k2, k3, total, 0, 32, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, std::max(s1, std::max(s2, s3)))) return r;
+    // by key, stable (equal keys keep insertion order), then the first of each key, then back
+    // into insertion order
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, s1, k0, k1, p0, p1, total, 0, 64, ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceSelect::UniqueByKey(ctx->scan_tmp.p, s2, k1, p1, k2, p2, d_num, total, ctx->stream));
+    int num = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&num, d_num, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *nOut = num;
+    if (num > tet_stride) return fail(ctx, ICF_E_CAPACITY, "icf_nl2set: more tuples than tet_stride");
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, s3, p2, p3, k2, k3, num, 0, 32, ctx->stream));
+    hipLaunchKernelGGL(k_nl2_write, dim3(blocks_for(num, 256)), dim3(256), 0, ctx->stream, k3, num, nl2, tet_stride);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

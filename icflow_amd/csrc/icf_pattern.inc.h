@@ -1,0 +1,153 @@
+// icf_pattern.inc.h -- BCSR pattern and pcsr slot packing on the device (included by
+// icf_lib.hip).
+//
+// Restates, for device-resident unit buffers:
+//  - the static structure of MeshCollection.UpdateStaticStructureData
+//    (v3Library/Geometry/Mesh/MeshCollection.cs:91-109): for every CZ and element, every pair
+//    of non-anchored vertices with altId_2 >= altId_1;
+//  - the dynamic structure of ImplicitModel3._addCollidingNodesToStructure
+//    (ImplicitModel3.cs:177-196): every pair of an impact's non-anchored nodes with
+//    altId_2 > altId_1;
+//  - CSRDictionary.CreateStructure (CSRDictionary.cs:64-97): rows 0..N-1, each row's columns
+//    the sorted union of its static and dynamic neighbours, slot = position;
+//  - the slot / row packing of GPU_Functionality.TransferPCSR (GPU_Functionality.cs:345-416)
+//    and collisionResponse (:627-665): slot(i,j) = position of (altId_i, altId_j) when both
+//    nodes are non-anchored and altId_j >= altId_i, else -1; row_i = altId_i or -1.
+// Pairs are emitted as 64-bit keys (row << 32 | column), sorted and de-duplicated with
+// hipCUB, which gives the same rows and sorted columns as the dictionary.
+#pragma once
+
+namespace icf {
+
+constexpr unsigned long long PAT_NONE = ~0ULL;
+
+// One unit source of the pattern: node ids base[u + stride * (field0 + i)], i < K.
+struct PatSrc {
+    const int *base;
+    int n, stride, field0, K;
+    int strict;            // 1: only altId_j > altId_i (impacts)
+    long long out0;        // first output position of this source
+};
+
+// Number of candidate pairs per unit: K(K+1)/2, or K(K-1)/2 when strict.
+__host__ __device__ inline int pat_pairs(int K, int strict) { return strict ? K * (K - 1) / 2 : K * (K + 1) / 2; }
+
+__global__ __launch_bounds__(256) void k_pat_emit(PatSrc s, const int *__restrict__ alt, int nNodes, int N,
+                                                  unsigned long long *__restrict__ keys, int *__restrict__ err)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = pat_pairs(s.K, s.strict);
+    if (t >= (long long)s.n * P) return;
+    const int u = (int)(t / P);
+    int p = (int)(t % P);
+    // local pair (i, j), i <= j (i < j when strict), in row-major order
+    int i = 0, row_len = s.strict ? s.K - 1 : s.K;
+    while (p >= row_len) {
+        p -= row_len;
+        i++;
+        row_len--;
+    }
+    const int j = s.strict ? i + 1 + p : i + p;
+    const int ni = s.base[u + (size_t)s.stride * (s.field0 + i)];
+    const int nj = s.base[u + (size_t)s.stride * (s.field0 + j)];
+    unsigned long long key = PAT_NONE;
+    if (ni < 0 || ni >= nNodes || nj < 0 || nj >= nNodes) {
+        atomicOr(err, 1);
+    } else {
+        const int a = alt[ni], b = alt[nj];
+        if (a >= 0 && b >= 0 && !(s.strict && a == b)) {
+            const int r = a < b ? a : b, c = a < b ? b : a;
+            if (c >= N) atomicOr(err, 1);
+            else key = ((unsigned long long)r << 32) | (unsigned)c;
+        }
+    }
+    keys[s.out0 + t] = key;
+}
+
+// csr_cols[k] = column of unique key k (k < nnz); csr_rows[r] = first k with row >= r.
+__global__ __launch_bounds__(256) void k_pat_cols(const unsigned long long *__restrict__ ukeys, int nnz,
+                                                  int *__restrict__ cols)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) cols[k] = (int)(ukeys[k] & 0xffffffffULL);
+}
+
+__global__ __launch_bounds__(256) void k_pat_rows(const unsigned long long *__restrict__ ukeys, int nnz, int N,
+                                                  int *__restrict__ rows)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > N) return;
+    int lo = 0, hi = nnz;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)(ukeys[mid] >> 32) < r) lo = mid + 1;
+        else hi = mid;
+    }
+    rows[r] = lo;
+}
+
+__device__ inline int pat_lookup(const int *__restrict__ rows, const int *__restrict__ cols, int r, int c)
+{
+    int lo = rows[r], hi = rows[r + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = cols[mid];
+        if (v == c) return mid;
+        if (v < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return -1;
+}
+
+// Slot and row fields of one unit kind; one thread per (unit, local row i).
+struct PackSrc {
+    int *base;
+    int n, stride, node0, slot0, row0, K;
+};
+
+__global__ __launch_bounds__(256) void k_pack_pcsr(PackSrc s, const int *__restrict__ alt, int nNodes,
+                                                   const int *__restrict__ rows, const int *__restrict__ cols, int N,
+                                                   int *__restrict__ err)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)s.n * s.K) return;
+    const int i = (int)(t / s.n), u = (int)(t % s.n);     // i-major: adjacent threads, adjacent units
+    const int ni = s.base[u + (size_t)s.stride * (s.node0 + i)];
+    if (ni < 0 || ni >= nNodes) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int ai = alt[ni];
+    if (ai >= N) {
+        atomicOr(err, 1);
+        return;
+    }
+    s.base[u + (size_t)s.stride * (s.row0 + i)] = ai >= 0 ? ai : -1;
+    for (int j = 0; j < s.K; j++) {
+        const int nj = s.base[u + (size_t)s.stride * (s.node0 + j)];
+        int slot = -1;
+        if (nj < 0 || nj >= nNodes) {
+            atomicOr(err, 1);
+        } else {
+            const int aj = alt[nj];
+            if (ai >= 0 && aj >= 0 && aj >= ai) {
+                slot = aj < N ? pat_lookup(rows, cols, ai, aj) : -1;
+                if (slot < 0) atomicOr(err, 1);
+            }
+        }
+        s.base[u + (size_t)s.stride * (s.slot0 + i * s.K + j)] = slot;
+    }
+}
+
+// CZ flags that TransferPCSR resets (GPU_Functionality.cs:384-386): current failed,
+// tentative failed, tentative damaged.
+__global__ __launch_bounds__(256) void k_pack_cz_flags(int *__restrict__ icz, int n, int cs)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    icz[u + (size_t)cs * CZI_FAILED] = 0;
+    icz[u + (size_t)cs * CZI_TFAILED] = 0;
+    icz[u + (size_t)cs * CZI_TDAMAGED] = 0;
+}
+
+}  // namespace icf

@@ -82,6 +82,23 @@ __global__ __launch_bounds__(256) void k_map_sort(const int *__restrict__ off, u
     }
 }
 
+// Copy of a source's key fields (q-major), and its comparison with the current fields.
+__global__ __launch_bounds__(256) void k_snap_copy(MapSrc s, int *__restrict__ snap)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)s.n * s.nq) return;
+    const int q = (int)(i / s.n), u = (int)(i % s.n);
+    snap[i] = s.base[u + (size_t)s.stride * (s.field0 + q)];
+}
+
+__global__ __launch_bounds__(256) void k_snap_cmp(MapSrc s, const int *__restrict__ snap, int *__restrict__ diff)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)s.n * s.nq) return;
+    const int q = (int)(i / s.n), u = (int)(i % s.n);
+    if (snap[i] != s.base[u + (size_t)s.stride * (s.field0 + q)]) atomicOr(diff, 1);
+}
+
 // flag[k] = 1 if key k has contributions
 __global__ __launch_bounds__(256) void k_map_flag(const int *__restrict__ cnt, int nkeys, int *__restrict__ flag)
 {

@@ -164,6 +164,72 @@ int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);
 int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches);
 
+/* --- node state kept on the device between iterations ---
+ * The reference keeps u, v, a, du per node on the host (v3Library/Geometry/Node.cs) and packs
+ * and uploads the whole node buffer every iteration (TransferNodesToGPU,
+ * GPU_Functionality.cs:307-340).  With these calls the state stays on the device in
+ * st: double SoA 12 x st_stride per node (u 0-2, v 3-5, a 6-8, du 9-11), and only the solver's
+ * dx crosses the bus.  Results are the reference's doubles (same expressions, no contraction). */
+/* Node.InferTentativeValues (Node.cs:79-97) for every node, written into dn as
+ * TransferNodesToGPU packs it: un, vn, an, x_current = x0 + un (x0 read from dn), F = 0. */
+int icf_nodes_infer(icf_ctx *ctx, const double *st, int st_stride, double *dn, int nd_stride,
+                    int nNodes, double h, double beta, double gamma);
+/* ImplicitModel3._XtoDU (ImplicitModel3.cs:247-256): du += dx[3 altId + k] for altId >= 0;
+ * dx is the solver's increment (3 per active row) in device memory. */
+int icf_nodes_add_dx(icf_ctx *ctx, double *st, int st_stride, const int *altId, int nNodes,
+                     const double *dx);
+/* Node.AcceptTentativeValues (Node.cs:62-77) with un as last inferred into dn:
+ * v' = (un - u)/h, a = (v' - v)/h, v = v', u = un, du = 0. */
+int icf_nodes_accept(icf_ctx *ctx, double *st, int st_stride, const double *dn, int nd_stride,
+                     int nNodes, double h);
+
+/* --- BCSR pattern and pcsr packing on the device ---
+ * The reference builds the pattern on the host (CSRDictionary.CreateStructure,
+ * CSRDictionary.cs:64-97, from MeshCollection.UpdateStaticStructureData,
+ * MeshCollection.cs:91-109, and ImplicitModel3._addCollidingNodesToStructure,
+ * ImplicitModel3.cs:177-196) and packs every unit's slots and rows on the host before each
+ * upload (TransferPCSR, GPU_Functionality.cs:345-416; collisionResponse, :627-665).
+ * altId: per node (all nodes), -1 for anchored; node ids are read from the unit buffers
+ * (ie fields 0-3, icz 4-9, icr 0-3).  All arrays are device memory. */
+/* Rows 0..N-1; row r's columns are the sorted union of c >= r over pairs (altId_i, altId_j)
+ * of non-anchored vertices of each element and CZ, and c > r over each impact's nodes.
+ * Writes csr_rows[N+1], csr_cols[*nnz].  Returns ICF_E_CAPACITY (with *nnz set) if the
+ * pattern has more than max_nnz blocks, ICF_E_INDEX for a node id outside [0, nNodes) or an
+ * altId >= N.  Any unit count may be 0 (its pointer may then be NULL). */
+int icf_build_pattern(icf_ctx *ctx, const int *altId, int nNodes,
+                      const int *ie, int nElems, int el_stride,
+                      const int *icz, int nCZs, int cz_stride,
+                      const int *icr, int nImpacts, int cr_stride,
+                      int N, int *csr_rows, int *csr_cols, int max_nnz, int *nnz);
+/* Slot fields (position of (altId_i, altId_j) if both >= 0 and altId_j >= altId_i, else -1)
+ * and row fields (altId_i or -1) of every element, CZ and impact, written in place; for CZs
+ * also the current-failed, tentative-failed and tentative-damaged flags are reset to 0, as
+ * TransferPCSR does.  ICF_E_INDEX if a needed pair is not in the pattern. */
+int icf_pack_pcsr(icf_ctx *ctx, const int *altId, int nNodes, const int *csr_rows,
+                  const int *csr_cols, int N, int *ie, int nElems, int el_stride,
+                  int *icz, int nCZs, int cz_stride, int *icr, int nImpacts, int cr_stride);
+
+/* --- broad phase and NL2set on the device ---
+ * The reference finds candidate pairs with a host k-DOP BVH (BVHT.ConstructAndTraverse,
+ * v3Library/Collisions/BVH/BVHT.cs:29-77; BVHN.cs:166-196; kDOP24.cs:84-140) and de-duplicates
+ * the narrow phase's (node, tet) tuples in a host HashSet (GPU_Functionality.cs:565-600). */
+/* Every pair (a < b) of the first nTets surface tets of ie whose 12-direction k-DOPs on
+ * x_current (dn fields 15-17) overlap, sorted by (a, b), into pairs[2k], pairs[2k+1] (device).
+ * exclude_shared != 0 drops pairs that share a node (the narrow phase zeroes them anyway,
+ * narrowphase.h:99-103).  ICF_E_CAPACITY with *nOut = the full count if it exceeds
+ * max_pairs. */
+int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie,
+                    int el_all_stride, int nTets, int exclude_shared, int *pairs,
+                    long long max_pairs, long long *nOut);
+/* From the narrow phase's masks (narrowList[2i] after icf_narrow_phase) and the candidate
+ * pairs it was given (pairs[2i], pairs[2i+1]): the (node, tet) tuples -- vertex k of tet B
+ * inside A for mask bit k < 4, vertex k-4 of A inside B for bit k >= 4 -- each kept once, in
+ * the order of first occurrence, written to rows 0 (node) and 1 (tet) of nl2 (SoA,
+ * tet_stride), ready for icf_find_closest_face.  ICF_E_CAPACITY (with *nOut set) if there are
+ * more than tet_stride tuples. */
+int icf_nl2set(icf_ctx *ctx, const int *masks, const int *pairs, int nPairs, const int *ie,
+               int el_all_stride, int *nl2, int tet_stride, int *nOut);
+
 #ifdef __cplusplus
 }
 #endif

@@ -984,3 +984,54 @@ void orc_collision_response(const int *icr, double *dn, double h, double *A, dou
         scatter4(pcsr, cr_stride, A, b, H, fi);
     }
 }
+
+/* ---------------------------------------------------------------------------------------
+ * Node state between iterations.
+ * ------------------------------------------------------------------------------------- */
+
+/* Node.cs:79-86 (InferTentativeUVA), per component. */
+static void infer_uva(double du, double u, double v, double a, double h, double beta, double gamma,
+                      double *un, double *vn, double *an)
+{
+    *un = u + du;
+    *an = a * (1.0 - 1.0 / (2 * beta)) + du / (h * h * beta) + v * (-1.0 / (h * beta));
+    *vn = v + h * ((1.0 - gamma) * a + gamma * *an);
+}
+
+/* Node.cs:88-97 with the packing of GPU_Functionality.cs:313-337. */
+void orc_nodes_infer(const double *st, int ss, double *dn, int ns, int n, double h, double beta, double gamma)
+{
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            double un, vn, an;
+            infer_uva(st[i + ss * (9 + k)], st[i + ss * k], st[i + ss * (3 + k)], st[i + ss * (6 + k)], h, beta,
+                      gamma, &un, &vn, &an);
+            dn[i + ns * (3 + k)] = un;
+            dn[i + ns * (6 + k)] = vn;
+            dn[i + ns * (9 + k)] = an;
+            dn[i + ns * (12 + k)] = 0;
+            dn[i + ns * (15 + k)] = dn[i + ns * k] + un;
+        }
+}
+
+/* ImplicitModel3.cs:247-256 */
+void orc_nodes_add_dx(double *st, int ss, const int *alt, int n, const double *dx)
+{
+    for (int i = 0; i < n; i++)
+        if (alt[i] >= 0)
+            for (int k = 0; k < 3; k++) st[i + ss * (9 + k)] += dx[alt[i] * 3 + k];
+}
+
+/* Node.cs:62-77 */
+void orc_nodes_accept(double *st, int ss, const double *dn, int ns, int n, double h)
+{
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            const double un = dn[i + ns * (3 + k)];
+            const double vn = (un - st[i + ss * k]) / h;
+            st[i + ss * (6 + k)] = (vn - st[i + ss * (3 + k)]) / h;
+            st[i + ss * (3 + k)] = vn;
+            st[i + ss * k] = un;
+            st[i + ss * (9 + k)] = 0;
+        }
+}

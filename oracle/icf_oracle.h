@@ -62,6 +62,20 @@ double orc_pt(const double x[12], double fd[12], double sd[144], double *s, doub
 void   orc_collision_response(const int *icr, double *dn, double h, double *A, double *b,
                               int nImpacts, int cr_stride, int nd_stride, double k, double distEps);
 
+/* --- node state between iterations (v3Library/Geometry/Node.cs, ImplicitModel3.cs) ---
+ * st: double SoA 12 x st_stride per node: u 0-2, v 3-5, a 6-8, du 9-11 (Node.ux.., vx.., ax..,
+ * dux..).  dn: the kernels' node buffer (18 x nd_stride). */
+/* Node.InferTentativeValues / InferTentativeUVA (Node.cs:79-97) for every node, stored the way
+ * GPU_Functionality.TransferNodesToGPU packs them (GPU_Functionality.cs:307-340): un, vn, an,
+ * x_current = x0 + un, F = 0 (x0 is read from dn). */
+void orc_nodes_infer(const double *st, int st_stride, double *dn, int nd_stride, int n, double h,
+                     double beta, double gamma);
+/* ImplicitModel3._XtoDU (ImplicitModel3.cs:247-256): du += dx[3 altId + k] for altId >= 0. */
+void orc_nodes_add_dx(double *st, int st_stride, const int *alt, int n, const double *dx);
+/* Node.AcceptTentativeValues (Node.cs:62-77), with un as last inferred (dn): v' = (un - u)/h,
+ * a = (v' - v)/h, v = v', u = un, du = 0. */
+void orc_nodes_accept(double *st, int st_stride, const double *dn, int nd_stride, int n, double h);
+
 #ifdef __cplusplus
 }
 #endif

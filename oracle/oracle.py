@@ -50,6 +50,9 @@ _SIGS = {
     "orc_pt": (c_double, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "orc_collision_response": (None, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int, c_int, c_int,
                                       c_double, c_double]),
+    "orc_nodes_infer": (None, [c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double]),
+    "orc_nodes_add_dx": (None, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "orc_nodes_accept": (None, [c_void_p, c_int, c_void_p, c_int, c_int, c_double]),
 }
 
 _lib = None
@@ -161,3 +164,15 @@ def pt(x12):
     br = np.zeros(1, dtype=np.int32)
     d = load().orc_pt(_p(x12), _p(fd), _p(sd), _p(s), _p(t), _p(br))
     return d, fd, sd, s[0], t[0], int(br[0])
+
+
+def nodes_infer(st, ss, dn, ns, n, h, beta, gamma):
+    load().orc_nodes_infer(_p(st), ss, _p(dn), ns, n, h, beta, gamma)
+
+
+def nodes_add_dx(st, ss, alt, n, dx):
+    load().orc_nodes_add_dx(_p(st), ss, _p(alt), n, _p(dx))
+
+
+def nodes_accept(st, ss, dn, ns, n, h):
+    load().orc_nodes_accept(_p(st), ss, _p(dn), ns, n, h)

@@ -1,0 +1,61 @@
+"""Device broad phase and NL2set (icf_broad_phase, icf_nl2set) against the host versions
+(icfh_kdop_pairs, icfh_nl2set): identical pair sets in the same (a, b) order, identical
+(node, tet) tuples in first-occurrence order, and the closest faces that follow."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from icflow_amd import _lib
+from icflow_amd.gpu_functionality import broad_phase, stride_of
+from workloads import Pipeline, make_state, small_specimen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("exclude_shared", [1, 0])
+def test_device_broad_phase_and_nl2set_match_host(specimen_consts, exclude_shared):
+    col = small_specimen(cells=(6, 5, 4), size=(0.06, 0.05, 0.04), grains=6, seed=14)
+    st = make_state(col, specimen_consts, seed=15, platen_dz_frac=0.3)
+    p = Pipeline(col, specimen_consts, st)
+    gf, lib, ctx = p.gf, p.gf.lib, p.gf.ctx
+    nS = len(col.surf)
+    host = broad_phase(gf, gf.c_dn, gf.nd_stride, gf.c_ie, gf.el_all_stride, nS, exclude_shared=bool(exclude_shared))
+    Q = len(host)
+    assert Q > 0
+    d_pairs = gf.buf("bp_pairs", 2 * Q, np.int32)
+    n = ctypes.c_longlong()
+    r = lib.icf_broad_phase(ctx, gf.g_dn.p, gf.nd_stride, gf.g_ie.p, gf.el_all_stride, nS, exclude_shared,
+                            d_pairs.p, Q - 1, ctypes.byref(n))
+    assert r == _lib.ICF_E_CAPACITY and n.value == Q
+    _lib.check(lib.icf_broad_phase(ctx, gf.g_dn.p, gf.nd_stride, gf.g_ie.p, gf.el_all_stride, nS, exclude_shared,
+                                   d_pairs.p, Q, ctypes.byref(n)), ctx, "broad_phase")
+    assert n.value == Q
+    got = d_pairs.download(2 * Q).reshape(-1, 2)
+    assert np.array_equal(got, host)
+    # narrow phase on a copy, then NL2set on the device and on the host
+    d_nl = gf.buf("bp_nl", 2 * Q, np.int32)
+    d_nl.upload(host.ravel())
+    _lib.check(lib.icf_narrow_phase(ctx, Q, gf.g_dn.p, gf.g_ie.p, d_nl.p, gf.el_all_stride, gf.nd_stride), ctx,
+               "narrow")
+    masks = d_nl.download(2 * Q)
+    cap = int(np.count_nonzero(masks[0::2])) * 8 + 8
+    ts = stride_of(cap)
+    ref = np.zeros(2 * ts, np.int32)
+    nh = ctypes.c_int()
+    _lib.check(lib.icfh_nl2set(_lib.ptr(masks), _lib.ptr(host.ravel().copy()), Q, _lib.ptr(gf.c_ie),
+                               gf.el_all_stride, _lib.ptr(ref), ts, ts, ctypes.byref(nh)))
+    assert nh.value > 0
+    d_nl2 = gf.buf("bp_nl2", 4 * ts, np.int32)
+    nd = ctypes.c_int()
+    _lib.check(lib.icf_nl2set(ctx, d_nl.p, d_pairs.p, Q, gf.g_ie.p, gf.el_all_stride, d_nl2.p, ts, ctypes.byref(nd)),
+               ctx, "nl2set")
+    assert nd.value == nh.value
+    out = d_nl2.download(4 * ts)
+    k = nh.value
+    assert np.array_equal(out[:k], ref[:k])
+    assert np.array_equal(out[ts:ts + k], ref[ts:ts + k])
+    small = gf.buf("bp_small", 4 * 64, np.int32)
+    r = lib.icf_nl2set(ctx, d_nl.p, d_pairs.p, Q, gf.g_ie.p, gf.el_all_stride, small.p, 1, ctypes.byref(nd))
+    assert r == _lib.ICF_E_CAPACITY and nd.value == nh.value
+    gf.close()
