@@ -34,6 +34,19 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def build_batched(n_specimens, seed):
+    """Batched specimens: n cubes with lognormal tet counts (median 12k, truncated to 2k-40k,
+    log-sd 0.6), 8-64 Voronoi grains each, laid side by side; no contact."""
+    from icflow_amd import _lib, specimens
+    consts = _lib.consts_from_prms(**_lib.SPECIMEN_PRMS)
+    t = time.time()
+    col = specimens.lognormal_cubes(n_specimens, seed)
+    st = specimens.synthetic_state(col, consts, seed, equal_gp=True)
+    log(f"batched specimens: {n_specimens} cubes, {len(col.x0)} nodes, {len(col.elastic)} tets, {len(col.czs)} CZs "
+        f"({time.time() - t:.1f} s)")
+    return consts, col, st
+
+
 def build_workload(cells, grains, seed, platen_frac):
     from icflow_amd import _lib, specimens
     consts = _lib.consts_from_prms(**_lib.SPECIMEN_PRMS)
@@ -51,16 +64,29 @@ def build_workload(cells, grains, seed, platen_frac):
 class Runner:
     """Device state for the timed step, built once through the GPU_Functionality mirror."""
 
-    def __init__(self, consts, col, st, mode):
+    def __init__(self, consts, col, st, mode, contact=True):
         from icflow_amd import _lib
         from icflow_amd.gpu_functionality import GPUFunctionality, broad_phase, stride_of
         self.lib = _lib.load()
         self.check = _lib.check
+        self.contact = contact
+        self.col = col
+        self.h = 5e-5
         gf = self.gf = GPUFunctionality(0)
         gf.SetConstants(consts)
         gf.set_precision(mode)
         gf.TransferStaticDataToDevice(col)
         gf.TransferNodesToGPU(st)
+        if not contact:
+            self.pairs, self.nT, self.Q = np.zeros((0, 2), np.int32), 0, 0
+            self.contact_touched = {"vals": 0, "rhs": 0, "force": 0}
+            t = time.time()
+            gf.CreateStructure(None)
+            gf.TransferPCSR(st)
+            log(f"pattern: N={gf.csr.N} nnzb={gf.csr.nnz} ({time.time() - t:.1f} s)")
+            self.vals = gf.buf("vals", 2 * gf.csr.nnz * 9, np.float64)
+            self.rhs = gf.buf("rhs", gf.csr.N * 3, np.float64)
+            return
         t = time.time()
         self.pairs = broad_phase(gf, gf.c_dn, gf.nd_stride, gf.c_ie, gf.el_all_stride, len(col.surf))
         log(f"broad phase: {len(self.pairs)} candidate pairs ({time.time() - t:.1f} s)")
@@ -110,6 +136,12 @@ class Runner:
         """One iteration's device calls.  Blocking by default (the drop-in sequence); with
         icf_set_blocking(0) they only enqueue, and between icf_graph_begin/end they are captured."""
         gf, lib, ctx, chk = self.gf, self.lib, self.gf.ctx, self.check
+        if not self.contact:
+            chk(lib.icf_assemble_elems_and_czs(
+                ctx, gf.g_ie_pcsr.p, len(self.col.elastic), gf.el_elastic_stride, gf.g_dcz.p, gf.g_icz.p,
+                len(self.col.czs), gf.cz_stride, gf.g_dn.p, gf.nd_stride, self.h, self.vals.p, gf.csr.nnz,
+                self.rhs.p, gf.csr.N, None, None), ctx, "icf_assemble_elems_and_czs")
+            return
         chk(lib.icf_d2d(ctx, self.itet.p, 0, self.itet_src.p, 0, 8 * self.Q), ctx, "icf_d2d")
         chk(lib.icf_d2d(ctx, self.nl2.p, 0, self.nl2_src.p, 0, 8 * gf.tet_stride), ctx, "icf_d2d")
         chk(lib.icf_narrow_phase(ctx, self.Q, gf.g_dn.p, gf.g_ie.p, self.itet.p, gf.el_all_stride, gf.nd_stride),
@@ -200,7 +232,7 @@ def algorithmic_bytes(r):
     n_def = int(np.unique(col.elastic).size)
     n_cz = int(np.unique(col.czs).size) if C else 0
     n_surf = int(np.unique(col.surf).size)
-    adj = gf.maxAdjacentFaces
+    adj = getattr(gf, "maxAdjacentFaces", 0)
     touched = r.contact_touched
     return {
         # ids 16 B/tet + x0,un,vn,an 96 B per node
@@ -223,7 +255,7 @@ def algorithmic_bytes(r):
         "k_reduce_vec3<rhs+acc>": 48 * touched["rhs"] + 16 * P,
         "k_reduce_vec3<force+acc>": 48 * touched["force"] + 16 * P,
         # the two restores of a step (8Q and 8 tet_stride bytes), read + write, averaged
-        "k_copy": 8 * Q + 8 * gf.tet_stride,
+        "k_copy": 8 * Q + 8 * getattr(gf, "tet_stride", 0),
         "k_count_flags": 8 * C,
         "k_fill_int": 8,
     }
@@ -247,6 +279,43 @@ def kernel_bytes(alg, name):
             return alg.get(name.replace(",acc>", "+acc>"))
         return alg.get(base + "+acc")
     return alg.get(name, alg.get(base))
+
+
+def _rocprof_symbol(name):
+    """The kernel symbol rocprofv3 reports for a per-kernel profile name of the library
+    (template arguments spelled out as the compiler prints them)."""
+    import re
+    m = re.match(r"(\w+)<(.*)>$", name)
+    if not m:
+        return name
+    base, args = m.group(1), m.group(2).split(",")
+    b = lambda x: "true" if x else "false"   # noqa: E731
+    if base == "k_reduce_vals":
+        if args[0] == "contact":
+            mixed = args[1] == "mixed"
+            return f"k_reduce_vals<{b(mixed)}, false, true, {'float' if mixed else 'double'}>"
+        mixed, iso, acc = args[0] == "mixed", args[1] == "iso", "acc" in args
+        return f"k_reduce_vals<{b(mixed)}, {b(iso)}, {b(acc)}, {'float' if mixed else 'double'}>"
+    if base in ("k_tet_phase1",):
+        return f"{base}<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}>"
+    if base in ("k_cz_phase1",):
+        return f"{base}<{b(args[0] == 'mixed')}>"
+    if base == "k_contact_phase1":
+        return f"{base}<{'float' if args[0] == 'mixed' else 'double'}>"
+    return name
+
+
+def profiled_traffic(name, path=os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-probe factors), or None."""
+    sym = _rocprof_symbol(name)
+    try:
+        for line in open(path):
+            if line.startswith(f"| `{sym}` |") and line.count("|") == 5:
+                return float(line.split("|")[4]) * 1e6
+    except OSError:
+        return None
+    return None
 
 
 def broad_phase_timing(r, reps=5):
@@ -305,6 +374,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="precision modes to time (fp64, mixed)")
+    ap.add_argument("--workload", default="compression", choices=["compression", "batched"],
+                    help="compression: the 40^3 cube with platens and contact; batched: --specimens cubes, "
+                         "lognormal sizes (seed 2), no contact")
+    ap.add_argument("--specimens", type=int, default=256)
     ap.add_argument("--newton-steps", type=int, default=10,
                     help="steps of the FP64-vs-mixed three-point-bend Newton run (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -317,8 +390,12 @@ def main():
         sys.exit(2)
 
     from icflow_amd import _lib
-    consts, col, st = build_workload(args.cells, args.grains, args.seed, args.platen_frac)
-    r = Runner(consts, col, st, _lib.ICF_MIXED)
+    contact = args.workload == "compression"
+    if contact:
+        consts, col, st = build_workload(args.cells, args.grains, args.seed, args.platen_frac)
+    else:
+        consts, col, st = build_batched(args.specimens, 2)
+    r = Runner(consts, col, st, _lib.ICF_MIXED, contact=contact)
 
     # correctness guard of the two modes on this workload (assembled arrays, normwise)
     res = {}
@@ -363,18 +440,22 @@ def main():
     alg = algorithmic_bytes(r)
     # dominant kernel of the step (largest total time) and its roofline
     dname, (dms, dn) = max(prof.items(), key=lambda kv: kv[1][0])
-    dbytes = kernel_bytes(alg, dname)
+    dbytes = kernel_bytes(alg, dname) or 0
     avg_s = dms / dn / 1e3
     achieved = dbytes / avg_s / 1e9
+    traffic = profiled_traffic(dname) if contact else None
     roof = dict(kernel=dname, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes=int(dbytes),
-                avg_us=round(avg_s * 1e6, 2))
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=int(traffic) if traffic else None,
+                alg_bytes=int(dbytes), avg_us=round(avg_s * 1e6, 2),
+                traffic_source="profiles/assembly_compression40_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                               "copy-calibrated)" if traffic else None)
     it_bytes = iteration_bytes(r)
 
     # device broad phase over the same surface tets (reported beside the step; the reference's
     # broad phase is a host BVH outside the assembly path)
-    bp = broad_phase_timing(r)
-    log(f"device broad phase: {bp}")
+    bp = broad_phase_timing(r) if contact else None
+    if bp:
+        log(f"device broad phase: {bp}")
     study = None
     if args.newton_steps > 0:
         from icflow_amd import newton
@@ -393,8 +474,11 @@ def main():
         "ms_per_step": round(sec * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
-        "config": {"workload": f"compression with contact: cube {args.cells}^3 hex x 6 tets, {args.grains} "
-                               f"grains (seed {args.seed}), PPR CZs on inter-grain faces, 2 rigid 32x32 platens",
+        "config": {"workload": (f"compression with contact: cube {args.cells}^3 hex x 6 tets, {args.grains} "
+                                f"grains (seed {args.seed}), PPR CZs on inter-grain faces, 2 rigid 32x32 platens"
+                                if contact else
+                                f"batched: {args.specimens} cubes, lognormal tet counts (median 12k, 2k-40k, "
+                                f"log-sd 0.6, seed 2), 8-64 grains each, PPR CZs, no contact, one launch per phase"),
                    "tets": T, "czs": C, "active_czs": active_C, "impacts": P, "candidate_pairs": r.Q,
                    "nodes": len(col.x0), "rows": r.gf.csr.N, "nnzb": r.gf.csr.nnz, "mode": head},
         "timing": "one captured HIP graph per Newton iteration, replayed; per-kernel times from a separate "

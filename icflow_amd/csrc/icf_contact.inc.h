@@ -173,16 +173,18 @@ __global__ __launch_bounds__(256) void k_closest_face(int nPairs, int ts, int *_
     nl2[u + ts * (1 + lane)] = bestf >= 0 ? faces[bestf + fs * lane] : -1;
 }
 
-// Phase 2: add block (a,b) of an impact record (symmetric from its upper triangle).
-__device__ inline void contact_block(double acc[9], const double *__restrict__ rec, int a, int b)
+// Phase 2: block (a,b) of an impact's block record (the transpose of block (b,a) when a > b).
+template <typename T>
+__device__ inline void contact_block(double acc[9], const T *__restrict__ tan, int a, int b)
 {
+    const T *blk = tan + ct_block(a < b ? a : b, a < b ? b : a) * 9;
+    T v[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) v[i] = blk[i];
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
-        for (int l = 0; l < 3; l++) {
-            const int r = 3 * a + k, c = 3 * b + l;
-            acc[3 * k + l] += r <= c ? rec[upper12(r, c)] : rec[upper12(c, r)];
-        }
+        for (int l = 0; l < 3; l++) acc[3 * k + l] += (double)(a <= b ? v[3 * k + l] : v[3 * l + k]);
 }
 
 }  // namespace icf

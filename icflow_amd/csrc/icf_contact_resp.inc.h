@@ -325,6 +325,17 @@ __device__ inline double grad_at(const Impact &I, int r)
     return ve_grad(I, r);
 }
 
+// Entry (r, c), r <= c, of the tangent into an impact's block record; inside a diagonal
+// block the mirrored entry is written too, so that every block is whole.
+template <typename T>
+__device__ inline void ct_store(T *__restrict__ tan, int r, int c, double v)
+{
+    const int a = r / 3, b = c / 3, k = r % 3, l = c % 3;
+    T *blk = tan + ct_block(a, b) * 9;
+    blk[k * 3 + l] = (T)v;
+    if (a == b && k != l) blk[l * 3 + k] = (T)v;
+}
+
 // ---- interior, closed form ----
 // In the interior region the squared distance is that to the plane of the triangle,
 // f = h^2 / m with N = E1 x E2, W = p - v0, h = N.W, m = N.N (E1 = v1 - v0, E2 = v2 - v0);
@@ -406,8 +417,9 @@ __device__ inline double tperm(int a, int b, const double V[3])
     return a == b ? 0.0 : (d == 1 ? vc : -vc);
 }
 
-// Row i of (k/2) Hess f, entries c >= i, into rec.
-__device__ __attribute__((always_inline)) inline void pp_closed_row(const PpClosed &P, int i, double hk, double *__restrict__ rec)
+// Row i of (k/2) Hess f, entries c >= i, into the impact's block record.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void pp_closed_row(const PpClosed &P, int i, double hk, T *__restrict__ tan)
 {
     const int na = i / 3, ka = i % 3;
     const double a1i = na == 1 ? -1.0 : (na == 2 ? 1.0 : 0.0);
@@ -462,23 +474,25 @@ __device__ __attribute__((always_inline)) inline void pp_closed_row(const PpClos
         }
         const double hh = A * tw[kb] + B * t2[kb] + C * t1[kb];
         const double hm = 2 * (ci[0] * cj[0] + ci[1] * cj[1] + ci[2] * cj[2]) + 2 * A * tn[kb];
-        rec[upper12(i, j)] = hk * (P.cu * ui * P.u[j] + P.chh * hh - P.chm * hm);
+        ct_store(tan, i, j, hk * (P.cu * ui * P.u[j] + P.chh * hh - P.chm * hm));
     }
 }
 
-// Phase 1 for impacts: record = 78 upper entries of (k/2) Hess d^2, then 12 forces
+// Phase 1 for impacts: the tangent blocks (k/2) Hess d^2 and the 12 forces
 // f = [-fp, w0 fp, w1 fp, w2 fp] with fp = (k/2) d(d^2)/dp (collision_response.h:89-114).
 // Rigid-rigid impacts and d^2 <= eps^2 stage zeros.  CT_GROUP lanes per impact; lane l
 // produces rows l, 7 - l and 8 + l (20, 20, 19 and 19 entries for l = 0..3).
+template <typename T>
 __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ icr, const double *__restrict__ dn,
                                                         int n, int cs, int ns, double kpen, double eps,
-                                                        double *__restrict__ st)
+                                                        T *__restrict__ st_tan, double *__restrict__ st_f)
 {
     const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / CT_GROUP;
     const int lane = threadIdx.x % CT_GROUP;
     if (gid >= n) return;
     const int u = gid;
-    double *rec = st + (size_t)u * CT_REC;
+    T *tan = st_tan + (size_t)u * CT_TAN_REC;
+    double *frc = st_f + (size_t)u * CT_F_REC;
     bool active = false;
 #pragma unroll
     for (int i = 0; i < 4; i++) active |= icr[u + cs * (CR_ROWS + i)] >= 0;
@@ -502,8 +516,8 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
 #pragma unroll
         for (int rr = 0; rr < 3; rr++) {
             const int r = rows[rr];
-            for (int c = r; c < 12; c++) rec[upper12(r, c)] = 0.;
-            rec[CT_F_OFF + r] = 0.;
+            for (int c = r; c < 12; c++) ct_store(tan, r, c, 0.0);
+            frc[r] = 0.;
         }
         return;
     }
@@ -512,7 +526,7 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
         PpClosed P;
         pp_closed_setup(I, P);
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++) pp_closed_row(P, rows[rr], hk, rec);
+        for (int rr = 0; rr < 3; rr++) pp_closed_row(P, rows[rr], hk, tan);
 #pragma unroll
         for (int k = 0; k < 3; k++) gpk[k] = hk * P.gp * P.N[k];
     } else {
@@ -520,9 +534,9 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
         for (int rr = 0; rr < 3; rr++) {
             const int r = rows[rr];
             if (I.branch <= 3) {
-                for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * vv_hess(I, r, c);
+                for (int c = r; c < 12; c++) ct_store(tan, r, c, hk * vv_hess(I, r, c));
             } else {
-                for (int c = r; c < 12; c++) rec[upper12(r, c)] = hk * ve_hess(I, r, c);
+                for (int c = r; c < 12; c++) ct_store(tan, r, c, hk * ve_hess(I, r, c));
             }
         }
 #pragma unroll
@@ -535,7 +549,7 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
         const double fpc = pick3(gpk, r % 3);
         const int nd = r / 3;
         const double w = nd == 0 ? -1.0 : (nd == 1 ? 1 - (I.s + I.t) : (nd == 2 ? I.s : I.t));
-        rec[CT_F_OFF + r] = nd == 0 ? -fpc : w * fpc;
+        frc[r] = nd == 0 ? -fpc : w * fpc;
     }
 }
 

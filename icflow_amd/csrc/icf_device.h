@@ -44,7 +44,11 @@ constexpr int CZ_TAN_REC = 18;
 // CZ residual: 18 doubles.
 constexpr int CZ_RHS_REC = 18;
 // contact: 78 packed upper-triangle entries of (k/2) Hess d^2, then 12 forces, 6 pad.
-constexpr int CT_REC = 96, CT_F_OFF = 78;
+// impact records: the 10 upper 3x3 blocks of the 12x12 tangent, block (a, b) a <= b at
+// ct_block(a, b), each whole and row-major (90 values, FP32 in mixed mode), and the 12 node
+// forces (FP64) in a separate array
+constexpr int CT_TAN_REC = 90, CT_F_REC = 12;
+__host__ __device__ inline int ct_block(int a, int b) { return a * 4 - (a * (a - 1)) / 2 + (b - a); }
 
 // ---- model constants -----------------------------------------------------------------------
 struct DevConsts {

@@ -28,25 +28,24 @@ __device__ inline int xcd_remap(int b, int nb)
 }
 
 // Orthonormal frame of three points; columns = (p1-p0), (p1-p0)x(p2-p0), col1 x col0, each
-// normalised.  R[row*3+col].  Evaluated without FMA contraction so that it rounds exactly
-// like the reference's expression order (matrix_math1.h:57-96).
+// normalised.  R[row*3+col].  Same construction as fastRotationMatrix (matrix_math1.h:57-96);
+// each column is scaled by one reciprocal of its length.
 __device__ inline void frame3(const double *p0, const double *p1, const double *p2, double R[9])
 {
-#pragma clang fp contract(off)
     double d0 = p1[0] - p0[0], d1 = p1[1] - p0[1], d2 = p1[2] - p0[2];
-    double len = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    R[0] = d0 / len; R[3] = d1 / len; R[6] = d2 / len;
+    double il = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    R[0] = d0 * il; R[3] = d1 * il; R[6] = d2 * il;
     double w0 = p2[0] - p0[0], w1 = p2[1] - p0[1], w2 = p2[2] - p0[2];
     double c0 = -d2 * w1 + d1 * w2;
     double c1 = d2 * w0 - d0 * w2;
     double c2 = -d1 * w0 + d0 * w1;
-    len = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-    R[1] = c0 / len; R[4] = c1 / len; R[7] = c2 / len;
+    il = 1.0 / sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    R[1] = c0 * il; R[4] = c1 * il; R[7] = c2 * il;
     double e0 = R[4] * R[6] - R[3] * R[7];
     double e1 = -R[1] * R[6] + R[0] * R[7];
     double e2 = R[1] * R[3] - R[0] * R[4];
-    len = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-    R[2] = e0 / len; R[5] = e1 / len; R[8] = e2 / len;
+    il = 1.0 / sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    R[2] = e0 * il; R[5] = e1 * il; R[8] = e2 * il;
 }
 
 // Rest gradients of the four linear shape functions, g[n*3+k], and the rest volume.
@@ -68,11 +67,15 @@ __device__ inline void rest_gradients(const double x0[12], double g[12], double 
     g[3] = y31 * z43 - y34 * z13;  g[4] = x43 * z31 - x13 * z34;  g[5] = x31 * y43 - x34 * y13;
     g[6] = y24 * z14 - y14 * z24;  g[7] = x14 * z24 - x24 * z14;  g[8] = x24 * y14 - x14 * y24;
     g[9] = y13 * z21 - y12 * z31;  g[10] = x21 * z13 - x31 * z12; g[11] = x13 * y21 - x12 * y31;
+    const double iJ = 1.0 / J;
 #pragma unroll
-    for (int i = 0; i < 12; i++) g[i] /= J;
+    for (int i = 0; i < 12; i++) g[i] *= iJ;
 }
 
-// out = V B^T E B y for the 12-vector y (the rest stiffness applied to y).
+// out = V B^T E B y for the 12-vector y (the rest stiffness applied to y).  ISO: E is the
+// isotropic lam/mu form (checked by icf_set_constants), so E e = lam tr(e) + 2 mu e (normal
+// strains), mu e (engineering shear strains).
+template <bool ISO>
 __device__ inline void apply_K0(const double g[12], double V, const double y[12], double out[12])
 {
     double e[6] = {0, 0, 0, 0, 0, 0};
@@ -88,12 +91,22 @@ __device__ inline void apply_K0(const double g[12], double V, const double y[12]
         e[5] += c * yx + a * yz;
     }
     double s[6];
+    if (ISO) {
+        const double lv = g_c.lam * (e[0] + e[1] + e[2]), mu = g_c.mu;
+        s[0] = (lv + 2 * mu * e[0]) * V;
+        s[1] = (lv + 2 * mu * e[1]) * V;
+        s[2] = (lv + 2 * mu * e[2]) * V;
+        s[3] = mu * e[3] * V;
+        s[4] = mu * e[4] * V;
+        s[5] = mu * e[5] * V;
+    } else {
 #pragma unroll
-    for (int r = 0; r < 6; r++) {
-        double v = 0;
+        for (int r = 0; r < 6; r++) {
+            double v = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) v += g_c.E[r * 6 + k] * e[k];
-        s[r] = v * V;
+            for (int k = 0; k < 6; k++) v += g_c.E[r * 6 + k] * e[k];
+            s[r] = v * V;
+        }
     }
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -161,33 +174,25 @@ __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, 
         }
     }
 
+    // elastic force and Rayleigh stiffness damping together: f + dampK Df vn
+    // = R K0 (R^T x_c - x0) + dampK R K0 R^T vn = R K0 (y + dampK R^T vn)
     double rhs[12];
     {
+        const double dK = g_c.dampK;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double v0 = dn[nid[i] + ns * (ND_VN + 0)], v1 = dn[nid[i] + ns * (ND_VN + 1)], v2 = dn[nid[i] + ns * (ND_VN + 2)];
+            y[3 * i + 0] += dK * (R[0] * v0 + R[3] * v1 + R[6] * v2);
+            y[3 * i + 1] += dK * (R[1] * v0 + R[4] * v1 + R[7] * v2);
+            y[3 * i + 2] += dK * (R[2] * v0 + R[5] * v1 + R[8] * v2);
+        }
         double Ky[12];
-        apply_K0(g, V, y, Ky);
+        apply_K0<ISO>(g, V, y, Ky);
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int k = 0; k < 3; k++)
                 rhs[3 * i + k] = -(R[k * 3 + 0] * Ky[3 * i] + R[k * 3 + 1] * Ky[3 * i + 1] + R[k * 3 + 2] * Ky[3 * i + 2]);
-    }
-    {
-        // Rayleigh stiffness damping: Df vn = R K0 R^T vn
-        double w[12], Kw[12];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            double v0 = dn[nid[i] + ns * (ND_VN + 0)], v1 = dn[nid[i] + ns * (ND_VN + 1)], v2 = dn[nid[i] + ns * (ND_VN + 2)];
-            w[3 * i + 0] = R[0] * v0 + R[3] * v1 + R[6] * v2;
-            w[3 * i + 1] = R[1] * v0 + R[4] * v1 + R[7] * v2;
-            w[3 * i + 2] = R[2] * v0 + R[5] * v1 + R[8] * v2;
-        }
-        apply_K0(g, V, w, Kw);
-        const double dK = g_c.dampK;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-                rhs[3 * i + k] -= dK * (R[k * 3 + 0] * Kw[3 * i] + R[k * 3 + 1] * Kw[3 * i + 1] + R[k * 3 + 2] * Kw[3 * i + 2]);
     }
     {
         // inertia and mass damping: V M (dampM vn + an); gravity rho g V / 4 on z
@@ -202,8 +207,13 @@ __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, 
 #pragma unroll
         for (int r = 0; r < 12; r++) {
             double m = 0;
+            if (ISO) {            // M = m_ij I3 per node pair
 #pragma unroll
-            for (int c = 0; c < 12; c++) m += g_c.M[r * 12 + c] * q[c];
+                for (int j = 0; j < 4; j++) m += g_c.m4[(r / 3) * 4 + j] * q[3 * j + r % 3];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 12; c++) m += g_c.M[r * 12 + c] * q[c];
+            }
             rhs[r] -= V * m;
             if (r % 3 == 2) rhs[r] += gz;
         }

@@ -62,14 +62,14 @@ struct SlotMap {
     uint32_t *codes = nullptr;     // ncodes
     int *keys = nullptr;           // accumulate mode: the keys with contributions (null: all keys)
     int nlaunch = 0;               // keys a reduction visits
-    int *chunks = nullptr;         // BCSR-value maps: chunk starts (list positions), nchunks + 1
-    int nchunks = 0;
+    int *chunks[2] = {nullptr, nullptr};   // BCSR-value maps: chunk starts (list positions), per
+    int nchunks[2] = {0, 0};               // staging type (0: FP32 values, 1: FP64 values)
     int id = 0;                    // unique per map built
-    // BCSR-value maps: the rhs map (by id) whose rows this map's diagonal slots were checked
-    // against, the outcome, and slot -> rhs row (-1 off the diagonal) when it holds
-    int fused_with = 0;
-    bool fuse_ok = false;
-    int *slot_row = nullptr;
+    // rhs maps of the fused call: the force map (by id) whose per-node tet sums seed the rows,
+    // the outcome of that check, and per row the node and the first non-tet code
+    int tsum_with = 0;
+    bool tsum_ok = false;
+    int *row_node = nullptr, *cz_begin = nullptr;
     // copy of the key fields the map was built from; after a write that may have touched them
     // (suspect) the map is kept if they still compare equal
     int *snap = nullptr;
@@ -92,6 +92,8 @@ struct icf_ctx {
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
     DevBuf pat_a, pat_b;           // pattern build / broad phase / NL2set: 64-bit keys
     DevBuf bp_f, bp_k, bp_i, bp_l;   // broad phase and NL2set scratch (doubles, keys, ints, int64)
+    DevBuf tsum;                     // per-node tet sums of the fused call
+    DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
     int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     int next_map_id = 0;
@@ -148,15 +150,17 @@ void free_map(SlotMap &m)
     if (m.off) (void)hipFree(m.off);
     if (m.codes) (void)hipFree(m.codes);
     if (m.keys) (void)hipFree(m.keys);
-    if (m.slot_row) (void)hipFree(m.slot_row);
-    if (m.chunks) (void)hipFree(m.chunks);
+    if (m.row_node) (void)hipFree(m.row_node);
+    if (m.cz_begin) (void)hipFree(m.cz_begin);
+    for (int k = 0; k < 2; k++)
+        if (m.chunks[k]) (void)hipFree(m.chunks[k]);
     if (m.snap) (void)hipFree(m.snap);
     m.snap = nullptr;
-    m.chunks = nullptr;
+    m.chunks[0] = m.chunks[1] = nullptr;
     m.off = nullptr;
     m.codes = nullptr;
     m.keys = nullptr;
-    m.slot_row = nullptr;
+    m.row_node = m.cz_begin = nullptr;
 }
 
 // Maps whose key fields overlap [lo, hi): dropped when the memory goes away (erase), else
@@ -302,7 +306,8 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
             if (s.n == 0) continue;
             MapSrc ms{s.base, s.n, s.stride, s.field0, s.nq, s.kind};
             long long tot = (long long)s.n * s.nq;
-            hipLaunchKernelGGL(k_map_maxkey, dim3(blocks_for(tot, 256)), dim3(256), 0, ctx->stream, ms, d_max);
+            const unsigned nb = blocks_for(tot, 256);
+            hipLaunchKernelGGL(k_map_maxkey, dim3(nb < 1024 ? nb : 1024), dim3(256), 0, ctx->stream, ms, d_max);
         }
         int mx = -1;
         HIPCHK(ctx, hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -361,9 +366,10 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
     }
     m.keys = list;                 // null when overwriting: every key, in order
     if (target == TGT_VALS && nlist > 0) {
-        // chunks of consecutive list positions for the staged reduction of BCSR values: at
-        // most FLAT_KEYS keys and FLAT_CAP contributions each (a single key with more
-        // contributions gets a chunk of its own, reduced without staging)
+        // chunks of consecutive list positions for the staged reduction of BCSR values, one
+        // table per staging type: at most FLAT_KEYS keys and FLAT_CAP_F (FP32) or FLAT_CAP_D
+        // (FP64) contributions each (a single key with more contributions gets a chunk of its
+        // own, reduced without staging)
         std::vector<int> h_off((size_t)nkeys + 1), h_list(list ? (size_t)nlist : 0);
         HIPCHK(ctx, hipMemcpyAsync(h_off.data(), m.off, sizeof(int) * ((size_t)nkeys + 1), hipMemcpyDeviceToHost,
                                    ctx->stream));
@@ -371,23 +377,28 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
             HIPCHK(ctx, hipMemcpyAsync(h_list.data(), list, sizeof(int) * (size_t)nlist, hipMemcpyDeviceToHost,
                                        ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        std::vector<int> ch(1, 0);
-        int cnt = 0, nk = 0;
-        for (int p = 0; p < nlist; p++) {
-            const int k = list ? h_list[p] : p;
-            const int c = h_off[k + 1] - h_off[k];
-            if (nk > 0 && (nk == FLAT_KEYS || cnt + c > FLAT_CAP)) {
-                ch.push_back(p);
-                cnt = 0;
-                nk = 0;
+        for (int kind = 0; kind < 2; kind++) {
+            const int cap = kind == 0 ? FLAT_CAP_F : FLAT_CAP_D;
+            std::vector<int> ch(1, 0);
+            int cnt = 0, nk = 0;
+            for (int p = 0; p < nlist; p++) {
+                const int k = list ? h_list[p] : p;
+                const int c = h_off[k + 1] - h_off[k];
+                if (nk > 0 && (nk == FLAT_KEYS || cnt + c > cap)) {
+                    ch.push_back(p);
+                    cnt = 0;
+                    nk = 0;
+                }
+                cnt += c;
+                nk++;
             }
-            cnt += c;
-            nk++;
+            ch.push_back(nlist);
+            m.nchunks[kind] = (int)ch.size() - 1;
+            HIPCHK(ctx, hipMalloc(&m.chunks[kind], sizeof(int) * ch.size()));
+            HIPCHK(ctx, hipMemcpyAsync(m.chunks[kind], ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice,
+                                       ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         }
-        ch.push_back(nlist);
-        m.nchunks = (int)ch.size() - 1;
-        HIPCHK(ctx, hipMalloc(&m.chunks, sizeof(int) * ch.size()));
-        HIPCHK(ctx, hipMemcpyAsync(m.chunks, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
     m.nlaunch = nlist;
@@ -425,40 +436,35 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
     return 0;
 }
 
-// Whether the rhs rows of map mr can be reduced by the diagonal-slot threads of map mv (see
-// k_fuse_check); decided once per pair of maps.  *slot_row = the slot -> row table, or null.
-int get_fusion(icf_ctx *ctx, SlotMap *mv, const SlotMap *mr, const MapSrc &tet, const MapSrc &cz,
-               const int **slot_row)
+// Whether the rows of rhs map mr can start from the per-node tet sums of force map mf (see
+// k_rhs_check); decided once per pair of maps.
+int get_tsum_link(icf_ctx *ctx, SlotMap *mr, const SlotMap *mf, const MapSrc &tet, bool *ok)
 {
-    *slot_row = nullptr;
-    if (mv->fused_with != mr->id) {
+    *ok = false;
+    if (mr->tsum_with != mf->id) {
         if (ctx->capturing)
             return fail(ctx, ICF_E_CAPTURE, "a slot map would be built during capture: run the call once uncaptured");
-        if (mv->slot_row) (void)hipFree(mv->slot_row);
-        mv->slot_row = nullptr;
-        mv->fuse_ok = false;
-        mv->fused_with = mr->id;
-        if (mr->nkeys > 0 && mv->nkeys > 0) {
-            HIPCHK(ctx, hipMalloc(&mv->slot_row, sizeof(int) * (size_t)mv->nkeys));
+        if (mr->row_node) (void)hipFree(mr->row_node);
+        if (mr->cz_begin) (void)hipFree(mr->cz_begin);
+        mr->row_node = mr->cz_begin = nullptr;
+        mr->tsum_ok = false;
+        mr->tsum_with = mf->id;
+        if (mr->nkeys > 0) {
+            HIPCHK(ctx, hipMalloc(&mr->row_node, sizeof(int) * (size_t)mr->nkeys));
+            HIPCHK(ctx, hipMalloc(&mr->cz_begin, sizeof(int) * (size_t)mr->nkeys));
             int *d_err = ctx->small + 1;
-            hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(mv->nkeys, 256)), dim3(256), 0, ctx->stream,
-                               mv->slot_row, mv->nkeys, -1);
             hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
-            hipLaunchKernelGGL(k_fuse_check, dim3(blocks_for(mr->nkeys, 256)), dim3(256), 0, ctx->stream, mr->off,
-                               mr->codes, mr->nkeys, mv->off, mv->codes, mv->nkeys, tet, cz, mv->slot_row, d_err);
+            hipLaunchKernelGGL(k_rhs_check, dim3(blocks_for(mr->nkeys, 256)), dim3(256), 0, ctx->stream, mr->off,
+                               mr->codes, mr->nkeys, mf->off, mf->codes, mf->nkeys, tet, mr->row_node, mr->cz_begin,
+                               d_err);
             int err = 1;
             HIPCHK(ctx, hipGetLastError());
             HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-            if (err) {
-                HIPCHK(ctx, hipFree(mv->slot_row));
-                mv->slot_row = nullptr;
-            } else {
-                mv->fuse_ok = true;
-            }
+            mr->tsum_ok = err == 0;
         }
     }
-    if (mv->fuse_ok) *slot_row = mv->slot_row;
+    *ok = mr->tsum_ok;
     return 0;
 }
 
@@ -537,42 +543,43 @@ Stage make_stage(icf_ctx *ctx, double h)
     sv.tet_tan = ctx->tet_tan.p;
     sv.cz_rhs = (const double *)ctx->cz_rhs.p;
     sv.cz_tan = ctx->cz_tan.p;
-    sv.ct = (const double *)ctx->ct.p;
+    sv.ct_tan = ctx->ct.p;
+    sv.ct_f = (const double *)ctx->ct_f.p;
     const DevConsts &c = ctx->hc;
     sv.stiff = 1.0 + c.dampK * c.ngamma / (h * c.nbeta);
     sv.mcoef = (1.0 / (h * h) + c.dampM * c.ngamma / h) / c.nbeta;
     return sv;
 }
 
-template <bool MX, bool IS, bool ACC, bool RHS, typename L>
-void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, const Stage &sv, const int *slot_row,
-                 double *b)
+template <bool MX, bool IS, bool ACC, typename L>
+void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, const Stage &sv)
 {
-    LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, RHS, L>), dim3(m->nchunks), dim3(FLAT_KEYS), m->off, m->codes,
-           m->keys, m->chunks, m->nchunks, A, sv, slot_row, b);
+    const int k = sizeof(L) == 4 ? 0 : 1;
+    if (m->nchunks[k] == 0) return;
+    LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L>), dim3(m->nchunks[k]), dim3(FLAT_KEYS), m->off, m->codes,
+           m->keys, m->chunks[k], m->nchunks[k], A, sv);
 }
 
 // Phase 2 of BCSR values for map m.  accumulate: add into A over the keys with contributions
-// (single-kernel entry points); otherwise overwrite every block, with the rhs rows reduced
-// alongside when slot_row is given.  contact: the map's units are impacts, whose FP64
-// records are staged as FP64 in both modes.
-void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, const Stage &sv,
-                 const int *slot_row = nullptr, double *b = nullptr, bool contact = false)
+// (single-kernel entry points); otherwise overwrite every block.  contact: the map's units
+// are impacts, whose FP64 records are staged as FP64 in both modes.
+void launch_vals(icf_ctx *ctx, const SlotMap *m, double *A, int accumulate, const Stage &sv, bool contact = false)
 {
-    if (m->nchunks == 0) return;
+    if (m->nlaunch == 0) return;
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     if (contact) {
-        vals_kernel<false, false, true, false, double>(ctx, "k_reduce_vals<contact,acc>", m, A, sv, nullptr, nullptr);
+        // impact records are FP64; in mixed mode their blocks are staged (and added) as FP32
+        // values, like the other tangent blocks of that mode
+        if (mixed) vals_kernel<true, false, true, float>(ctx, "k_reduce_vals<contact,mixed,acc>", m, A, sv);
+        else vals_kernel<false, false, true, double>(ctx, "k_reduce_vals<contact,fp64,acc>", m, A, sv);
         return;
     }
 #define ICF_VALS(MX, IS, L, NAME)                                                                              \
     do {                                                                                                       \
         if (accumulate)                                                                                        \
-            vals_kernel<MX, IS, true, false, L>(ctx, "k_reduce_vals<" NAME ",acc>", m, A, sv, nullptr, nullptr); \
-        else if (slot_row)                                                                                     \
-            vals_kernel<MX, IS, false, true, L>(ctx, "k_reduce_vals<" NAME ",rhs>", m, A, sv, slot_row, b);    \
+            vals_kernel<MX, IS, true, L>(ctx, "k_reduce_vals<" NAME ",acc>", m, A, sv);                        \
         else                                                                                                   \
-            vals_kernel<MX, IS, false, false, L>(ctx, "k_reduce_vals<" NAME ">", m, A, sv, nullptr, nullptr);  \
+            vals_kernel<MX, IS, false, L>(ctx, "k_reduce_vals<" NAME ">", m, A, sv);                           \
     } while (0)
     if (mixed && iso) ICF_VALS(true, true, float, "mixed,iso");
     else if (mixed) ICF_VALS(true, false, float, "mixed,gen");
@@ -638,7 +645,8 @@ int icf_destroy(icf_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
-                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l};
+                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l,
+                      &ctx->tsum,    &ctx->ct_f};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->small) (void)hipFree(ctx->small);
@@ -1037,13 +1045,20 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
     if (int r = get_map(ctx, TGT_VALS, -1, sv, &mv)) return r;
     if (int r = get_map(ctx, TGT_RHS, -1, sr, &mr)) return r;
     if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
-    if (int r = grow(ctx, ctx->ct, sizeof(double) * CT_REC * (size_t)nImpacts)) return r;
+    const bool mixed = ctx->precision == ICF_MIXED;
+    if (int r = grow(ctx, ctx->ct, (mixed ? sizeof(float) : sizeof(double)) * CT_TAN_REC * (size_t)nImpacts)) return r;
+    if (int r = grow(ctx, ctx->ct_f, sizeof(double) * CT_F_REC * (size_t)nImpacts)) return r;
     Timer tm(ctx);
-    LAUNCH(ctx, "k_contact_phase1", k_contact_phase1, dim3(blocks_for((long long)nImpacts * CT_GROUP, 256)), dim3(256), icr, dn,
-           nImpacts, cr_stride, nd_stride, k, distanceEpsilon, (double *)ctx->ct.p);
+    const dim3 cg(blocks_for((long long)nImpacts * CT_GROUP, 256)), cb(256);
+    if (mixed)
+        LAUNCH(ctx, "k_contact_phase1<mixed>", k_contact_phase1<float>, cg, cb, icr, dn, nImpacts, cr_stride, nd_stride,
+               k, distanceEpsilon, (float *)ctx->ct.p, (double *)ctx->ct_f.p);
+    else
+        LAUNCH(ctx, "k_contact_phase1<fp64>", k_contact_phase1<double>, cg, cb, icr, dn, nImpacts, cr_stride,
+               nd_stride, k, distanceEpsilon, (double *)ctx->ct.p, (double *)ctx->ct_f.p);
     Stage st = make_stage(ctx, 1.0);
     launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
-    launch_vals(ctx, mv, A, 1, st, nullptr, nullptr, true);
+    launch_vals(ctx, mv, A, 1, st, true);
     launch_vec3(ctx, mr, b, 1, 3, 1, st);
     return tm.finish();
 }
@@ -1087,19 +1102,30 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
     if (int r = get_map(ctx, TGT_RHS, N, sr, &mr)) return r;
     if (nElems)
         if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
-    const int *slot_row = nullptr;
-    {
-        MapSrc tet{nElems ? ie : nullptr, nElems, el_stride, EL_SLOTS, 16, KIND_TET};
-        MapSrc cz{nCZs ? icz : nullptr, nCZs, cz_stride, CZI_SLOTS, 36, KIND_CZ};
-        if (int r = get_fusion(ctx, mv, mr, tet, cz, &slot_row)) return r;
+    bool tsum = false;
+    if (mf) {
+        MapSrc tet{ie, nElems, el_stride, EL_NODES, 4, KIND_TET};
+        if (int r = get_tsum_link(ctx, mr, mf, tet, &tsum)) return r;
+        if (tsum)
+            if (int r = grow(ctx, ctx->tsum, sizeof(double) * 3 * (size_t)mf->nkeys)) return r;
     }
     Timer tm(ctx);
     if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
     if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
     Stage st = make_stage(ctx, h);
-    if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
-    launch_vals(ctx, mv, A, 0, st, slot_row, b);
-    if (!slot_row) launch_vec3(ctx, mr, b, 1, 3, 0, st);
+    if (tsum) {
+        // node forces, keeping each node's tet sum; rhs rows start from it and add their CZs
+        LAUNCH(ctx, "k_reduce_vec3<force,acc>", (k_reduce_vec3<true, true>), dim3(blocks_for(mf->nlaunch, 256)),
+               dim3(256), mf->off, mf->codes, mf->keys, mf->nlaunch, dn + (size_t)nd_stride * ND_F, nd_stride, 1, st,
+               (double *)ctx->tsum.p);
+        if (N)
+            LAUNCH(ctx, "k_reduce_rhs_tsum", k_reduce_rhs_tsum, dim3(blocks_for(N, 256)), dim3(256), mr->off,
+                   mr->codes, mr->cz_begin, mr->row_node, (const double *)ctx->tsum.p, N, b, st);
+    } else {
+        if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
+        launch_vec3(ctx, mr, b, 1, 3, 0, st);
+    }
+    launch_vals(ctx, mv, A, 0, st);
     // the two counts of GPU_Functionality.cs:474-475 (kernel-cleared, so the call can be captured)
     LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(64), ctx->small + 2, 2, 0);
     if (nCZs)

@@ -23,19 +23,24 @@ struct MapSrc {
     uint32_t kind;
 };
 
-// q-major enumeration: adjacent threads read adjacent units of one field (coalesced).
+// Largest key of a source: grid-stride over the q-major enumeration (adjacent threads read
+// adjacent units of one field), block maximum, one atomic per block.
 __global__ __launch_bounds__(256) void k_map_maxkey(MapSrc s, int *__restrict__ maxkey)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int part[4];
     const long long tot = (long long)s.n * s.nq;
     int key = -1;
-    if (i < tot) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
         const int q = (int)(i / s.n), u = (int)(i % s.n);
-        key = s.base[u + (size_t)s.stride * (s.field0 + q)];
+        key = max(key, s.base[u + (size_t)s.stride * (s.field0 + q)]);
     }
-    // wavefront max, then one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off));
-    if ((threadIdx.x & 63) == 0 && key >= 0) atomicMax(maxkey, key);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int m = max(max(part[0], part[1]), max(part[2], part[3]));
+        if (m >= 0) atomicMax(maxkey, m);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_map_count(MapSrc s, int nkeys, int *__restrict__ count, int *__restrict__ err)
@@ -115,9 +120,12 @@ __global__ __launch_bounds__(256) void k_map_compact(const int *__restrict__ fla
 }
 
 // Chunk limits of the staged BCSR-value reduction: keys per workgroup (one thread each) and
-// contributions staged in LDS (FLAT_CAP * 9 values: 27 KB FP32, 54 KB FP64).
+// contributions staged in LDS: FLAT_CAP_F FP32 or FLAT_CAP_D FP64 values x 9 (27 KB either
+// way, so that five workgroups share a CU).  Each map has a chunk table per staging type.
 constexpr int FLAT_KEYS = 256;
-constexpr int FLAT_CAP = 768;
+constexpr int FLAT_CAP_F = 768, FLAT_CAP_D = 384;
+template <typename L>
+__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? FLAT_CAP_F : FLAT_CAP_D; }
 
 // Staged per-unit records of the current call.
 struct Stage {
@@ -125,7 +133,8 @@ struct Stage {
     const void *tet_tan;
     const double *cz_rhs;
     const void *cz_tan;
-    const double *ct;
+    const void *ct_tan;     // impact tangent blocks (float in mixed mode)
+    const double *ct_f;     // impact forces
     double stiff, mcoef;   // tet LHS coefficients (element_kernels2.h:239-240)
 };
 
@@ -134,54 +143,7 @@ __device__ inline const double *rhs_of(const Stage &sv, uint32_t code)
     const uint32_t kind = code_kind(code), u = code_unit(code), q = code_q(code);
     if (kind == KIND_TET) return sv.tet_rhs + (size_t)u * TET_RHS_REC + 3 * q;
     if (kind == KIND_CZ) return sv.cz_rhs + (size_t)u * CZ_RHS_REC + 3 * q;
-    return sv.ct + (size_t)u * CT_REC + CT_F_OFF + 3 * q;
-}
-
-// Residual part of a diagonal-block code (local block (a, a) of a tet or CZ).
-__device__ inline const double *rhs_of_diag(const Stage &sv, uint32_t code)
-{
-    const uint32_t u = code_unit(code), q = code_q(code);
-    if (code_kind(code) == KIND_TET) return sv.tet_rhs + (size_t)u * TET_RHS_REC + 3 * (q >> 2);
-    return sv.cz_rhs + (size_t)u * CZ_RHS_REC + 3 * (q / 6);
-}
-
-// For each rhs row r: its contribution list must equal, code for code, the list of the
-// diagonal slot its first contributor names (local block (a, a)); then slot_row[slot] = r.
-// Any row without contributions, any mismatch, or two rows on one slot sets *err.
-__global__ __launch_bounds__(256) void k_fuse_check(const int *__restrict__ roff, const uint32_t *__restrict__ rcodes,
-                                                    int N, const int *__restrict__ soff,
-                                                    const uint32_t *__restrict__ scodes, int nnzb, MapSrc tet,
-                                                    MapSrc cz, int *__restrict__ slot_row, int *__restrict__ err)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
-    const int b = roff[r], e = roff[r + 1];
-    if (b == e) {
-        atomicOr(err, 1);
-        return;
-    }
-    const uint32_t c0 = rcodes[b];
-    const int u = (int)code_unit(c0), a = (int)code_q(c0);
-    int s = -1;
-    if (code_kind(c0) == KIND_TET && tet.base && u < tet.n && a < 4)
-        s = tet.base[u + (size_t)tet.stride * (tet.field0 + 5 * a)];
-    else if (code_kind(c0) == KIND_CZ && cz.base && u < cz.n && a < 6)
-        s = cz.base[u + (size_t)cz.stride * (cz.field0 + 7 * a)];
-    if (s < 0 || s >= nnzb || soff[s + 1] - soff[s] != e - b) {
-        atomicOr(err, 1);
-        return;
-    }
-    const int sb = soff[s];
-    for (int k = 0; k < e - b; k++) {
-        const uint32_t rc = rcodes[b + k], sc = scodes[sb + k];
-        const uint32_t kind = code_kind(rc), side = kind == KIND_TET ? 5u : 7u;
-        if (kind == KIND_CONTACT || code_kind(sc) != kind || code_unit(sc) != code_unit(rc) ||
-            code_q(sc) != code_q(rc) * side) {
-            atomicOr(err, 1);
-            return;
-        }
-    }
-    if (atomicCAS(&slot_row[s], -1, r) != -1) atomicOr(err, 1);
+    return sv.ct_f + (size_t)u * CT_F_REC + 3 * q;
 }
 
 // One contribution's 3x3 block, as the values that are added into the slot's FP64
@@ -207,13 +169,13 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
     else if (kind == KIND_CZ)
         cz_block<T>(acc, (const T *)sv.cz_tan + (size_t)u * CZ_TAN_REC, q / 6, q % 6);
     else
-        contact_block(acc, sv.ct + (size_t)u * CT_REC, q >> 2, q & 3);
+        contact_block<T>(acc, (const T *)sv.ct_tan + (size_t)u * CT_TAN_REC, q >> 2, q & 3);
 #pragma unroll
     for (int k = 0; k < 9; k++) v[k] = (L)acc[k];
 }
 
 // Phase 2 for BCSR values, one workgroup per chunk of consecutive keys (at most FLAT_KEYS
-// keys and FLAT_CAP contributions; the chunk table is built with the map).  The chunk's
+// keys and flat_cap<L>() contributions; the chunk tables are built with the map).  The chunk's
 // contributions are contiguous in `codes`: first every thread evaluates contributions
 // (uniform work across lanes) into LDS, then one thread per key adds its contributions from
 // LDS in list order into an FP64 accumulator and writes the block.  The values and their
@@ -221,97 +183,108 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
 // on the chunking.
 //   keys   null: chunk positions are keys; else keys[position] (accumulate mode)
 //   ACC    add to A (single-kernel calls); otherwise overwrite, zero where nothing contributes
-//   RHS    (overwrite only) rows reduced by their diagonal slots: the contributions to rhs
-//          row r are the same units, in the same order, as those to block (r, r) -- checked
-//          when the maps are built (k_fuse_check) -- so the thread of slot s with
-//          slot_row[s] = r >= 0 also sums the units' residual parts into b[3r .. 3r+2].
 // L is the staging type: float for tet and CZ blocks in mixed mode, double otherwise.
-template <bool MIXED, bool ISO, bool ACC, bool RHS, typename L>
+template <bool MIXED, bool ISO, bool ACC, typename L>
 __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
                                                            const int *__restrict__ keys, const int *__restrict__ chunks,
-                                                           int nchunks, double *__restrict__ A, Stage sv,
-                                                           const int *__restrict__ slot_row, double *__restrict__ bvec)
+                                                           int nchunks, double *__restrict__ A, Stage sv)
 {
-    __shared__ L lds[FLAT_CAP * 9];
+    constexpr int CAP = flat_cap<L>();
+    static_assert(CAP * 9 * sizeof(L) >= FLAT_KEYS * 9 * sizeof(double), "output staging fits the LDS");
+    constexpr int PER = (CAP + FLAT_KEYS - 1) / FLAT_KEYS;
+    __shared__ L lds[CAP * 9];
     const int ch = xcd_remap(blockIdx.x, gridDim.x);
     if (ch >= nchunks) return;
     const int p0 = chunks[ch], p1 = chunks[ch + 1];
     const int kfirst = keys ? keys[p0] : p0, klast = keys ? keys[p1 - 1] : p1 - 1;
     const int cb = off[kfirst], ce = off[klast + 1];
-    const bool staged = ce - cb <= FLAT_CAP;
+    const int nc = ce - cb;
+    const bool staged = nc <= CAP;
     if (staged) {
-        for (int i = threadIdx.x; i < ce - cb; i += FLAT_KEYS) {
-            L v[9];
-            contrib9<MIXED, ISO, L>(sv, codes[cb + i], v);
+        // codes first (coalesced), then the contributions, so their loads overlap
+        uint32_t cd[PER];
 #pragma unroll
-            for (int k = 0; k < 9; k++) lds[i * 9 + k] = v[k];
+        for (int r = 0; r < PER; r++) {
+            const int i = threadIdx.x + r * FLAT_KEYS;
+            cd[r] = i < nc ? codes[cb + i] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int i = threadIdx.x + r * FLAT_KEYS;
+            if (i < nc) {
+                L v[9];
+                contrib9<MIXED, ISO, L>(sv, cd[r], v);
+#pragma unroll
+                for (int k = 0; k < 9; k++) lds[i * 9 + k] = v[k];
+            }
         }
         __syncthreads();
     }
     const int p = p0 + threadIdx.x;
-    if (p >= p1) return;
-    const int s = keys ? keys[p] : p;
-    const int b = off[s], e = off[s + 1];
+    const bool live = p < p1;
     double acc[9];
-    double *dst = A + (size_t)s * 9;
+#pragma unroll
+    for (int i = 0; i < 9; i++) acc[i] = 0.;
+    int s = 0;
+    if (live) {
+        s = keys ? keys[p] : p;
+        const int b = off[s], e = off[s + 1];
+        if (ACC) {
+            const double *src = A + (size_t)s * 9;
+#pragma unroll
+            for (int i = 0; i < 9; i++) acc[i] = src[i];
+        }
+        if (staged) {
+            for (int c = b - cb; c < e - cb; c++) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) acc[k] += (double)lds[c * 9 + k];
+            }
+        } else {
+            for (int c = b; c < e; c++) {
+                L v[9];
+                contrib9<MIXED, ISO, L>(sv, codes[c], v);
+#pragma unroll
+                for (int k = 0; k < 9; k++) acc[k] += (double)v[k];
+            }
+        }
+    }
     if (ACC) {
+        if (live) {
+            double *dst = A + (size_t)s * 9;
 #pragma unroll
-        for (int i = 0; i < 9; i++) acc[i] = dst[i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < 9; i++) acc[i] = 0.;
+            for (int i = 0; i < 9; i++) dst[i] = acc[i];
+        }
+        return;
     }
-    if (staged) {
-        for (int c = b - cb; c < e - cb; c++) {
+    // overwrite mode: the chunk's blocks A[p0 .. p1) are contiguous; write them through LDS
+    // as consecutive doubles (coalesced) instead of one 72-byte block per thread
+    double *out = reinterpret_cast<double *>(lds);
+    __syncthreads();
+    if (live) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] += (double)lds[c * 9 + k];
-        }
-    } else {
-        for (int c = b; c < e; c++) {
-            L v[9];
-            contrib9<MIXED, ISO, L>(sv, codes[c], v);
-#pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] += (double)v[k];
-        }
+        for (int k = 0; k < 9; k++) out[threadIdx.x * 9 + k] = acc[k];
     }
-#pragma unroll
-    for (int i = 0; i < 9; i++) dst[i] = acc[i];
-    if (RHS) {
-        const int row = slot_row[s];
-        if (row < 0) return;
-        double a0 = 0., a1 = 0., a2 = 0.;
-        int c = b;
-        for (; c + 4 <= e; c += 4) {      // loads of four contributions in flight, added in order
-            const double *r0 = rhs_of_diag(sv, codes[c]), *r1 = rhs_of_diag(sv, codes[c + 1]);
-            const double *r2 = rhs_of_diag(sv, codes[c + 2]), *r3 = rhs_of_diag(sv, codes[c + 3]);
-            const double x0 = r0[0], y0 = r0[1], z0 = r0[2], x1 = r1[0], y1 = r1[1], z1 = r1[2];
-            const double x2 = r2[0], y2 = r2[1], z2 = r2[2], x3 = r3[0], y3 = r3[1], z3 = r3[2];
-            a0 += x0; a1 += y0; a2 += z0;
-            a0 += x1; a1 += y1; a2 += z1;
-            a0 += x2; a1 += y2; a2 += z2;
-            a0 += x3; a1 += y3; a2 += z3;
-        }
-        for (; c < e; c++) {
-            const double *r = rhs_of_diag(sv, codes[c]);
-            a0 += r[0]; a1 += r[1]; a2 += r[2];
-        }
-        double *o = bvec + (size_t)row * 3;
-        o[0] = a0; o[1] = a1; o[2] = a2;
-    }
+    __syncthreads();
+    const int nout = (p1 - p0) * 9;
+    double *dst = A + (size_t)p0 * 9;
+    for (int i = threadIdx.x; i < nout; i += FLAT_KEYS) dst[i] = out[i];
 }
 
 // Phase 2 for rhs rows (b[row*3+k]) or node forces (dn[node + ns*(F+k)]): one thread per key.
-template <bool ACC>
+// TSUM (node forces of the fused call): the sum of the key's contributions from zero is also
+// written to tsum[3 key .. 3 key + 2], from which k_reduce_rhs_tsum starts the rhs rows.
+template <bool ACC, bool TSUM = false>
 __global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off, const uint32_t *__restrict__ codes,
                                                      const int *__restrict__ keys, int nkeys, double *__restrict__ out,
-                                                     int comp_stride, int elem_stride, Stage sv)
+                                                     int comp_stride, int elem_stride, Stage sv,
+                                                     double *__restrict__ tsum = nullptr)
 {
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= nkeys) return;
     const int s = keys ? keys[t] : t;
     const int b = off[s], e = off[s + 1];
     double *dst = out + (size_t)s * elem_stride;
-    double a0, a1, a2;
+    double a0, a1, a2, t0 = 0., t1 = 0., t2 = 0.;
     if (ACC) {
         if (b == e) return;
         a0 = dst[0]; a1 = dst[comp_stride]; a2 = dst[2 * comp_stride];
@@ -328,12 +301,83 @@ __global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off
         a0 += x1; a1 += y1; a2 += z1;
         a0 += x2; a1 += y2; a2 += z2;
         a0 += x3; a1 += y3; a2 += z3;
+        if (TSUM) {
+            t0 += x0; t1 += y0; t2 += z0;
+            t0 += x1; t1 += y1; t2 += z1;
+            t0 += x2; t1 += y2; t2 += z2;
+            t0 += x3; t1 += y3; t2 += z3;
+        }
     }
     for (; c < e; c++) {
         const double *r = rhs_of(sv, codes[c]);
         a0 += r[0]; a1 += r[1]; a2 += r[2];
+        if (TSUM) { t0 += r[0]; t1 += r[1]; t2 += r[2]; }
     }
     dst[0] = a0; dst[comp_stride] = a1; dst[2 * comp_stride] = a2;
+    if (TSUM) {
+        double *o = tsum + (size_t)s * 3;
+        o[0] = t0; o[1] = t1; o[2] = t2;
+    }
+}
+
+// For each rhs row r of the fused call: the length of its leading run of tet codes and the
+// node those tets share.  The row's tet contributions must be exactly the force key's list
+// of that node (same codes, same order); any other case sets *err.
+__global__ __launch_bounds__(256) void k_rhs_check(const int *__restrict__ roff, const uint32_t *__restrict__ rcodes,
+                                                   int N, const int *__restrict__ foff,
+                                                   const uint32_t *__restrict__ fcodes, int nfkeys, MapSrc tet,
+                                                   int *__restrict__ row_node, int *__restrict__ cz_begin,
+                                                   int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int b = roff[r], e = roff[r + 1];
+    int L = 0;
+    while (b + L < e && code_kind(rcodes[b + L]) == KIND_TET) L++;
+    cz_begin[r] = b + L;
+    if (L == 0) {
+        row_node[r] = -1;
+        return;
+    }
+    const uint32_t c0 = rcodes[b];
+    const int u = (int)code_unit(c0), q = (int)code_q(c0);
+    const int n = (u < tet.n && q < 4) ? tet.base[u + (size_t)tet.stride * (tet.field0 + q)] : -1;
+    if (n < 0 || n >= nfkeys || foff[n + 1] - foff[n] != L) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int fb = foff[n];
+    for (int k = 0; k < L; k++)
+        if (fcodes[fb + k] != rcodes[b + k]) {
+            atomicOr(err, 1);
+            return;
+        }
+    row_node[r] = n;
+}
+
+// rhs rows of the fused call: the node's tet sum (k_reduce_vec3 TSUM), then the row's CZ
+// contributions in order -- the sequential sum over the row's list.
+__global__ __launch_bounds__(256) void k_reduce_rhs_tsum(const int *__restrict__ off, const uint32_t *__restrict__ codes,
+                                                         const int *__restrict__ cz_begin, const int *__restrict__ row_node,
+                                                         const double *__restrict__ tsum, int N, double *__restrict__ out,
+                                                         Stage sv)
+{
+    const int r = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int n = row_node[r];
+    double a0 = 0., a1 = 0., a2 = 0.;
+    if (n >= 0) {
+        a0 = tsum[(size_t)n * 3];
+        a1 = tsum[(size_t)n * 3 + 1];
+        a2 = tsum[(size_t)n * 3 + 2];
+    }
+    const int e = off[r + 1];
+    for (int c = cz_begin[r]; c < e; c++) {
+        const double *p = rhs_of(sv, codes[c]);
+        a0 += p[0]; a1 += p[1]; a2 += p[2];
+    }
+    double *o = out + (size_t)r * 3;
+    o[0] = a0; o[1] = a1; o[2] = a2;
 }
 
 // Byte copy, 16 bytes per thread where both ends allow it (icf_d2d; a kernel so that it can

@@ -65,9 +65,57 @@ def upper_bcsr_to_csc(rows, cols, vals):
 
 def solve_upper_bcsr(rows, cols, vals, rhs):
     """x with A x = rhs for the symmetric A of upper_bcsr_to_csc.  Host substitute for
-    PARDISO (LinearSystem.cs:45-64): SuperLU with a minimum-degree ordering."""
+    PARDISO (LinearSystem.cs:45-64): SuperLU with a column minimum-degree ordering."""
     from scipy.sparse.linalg import splu
-    return splu(upper_bcsr_to_csc(rows, cols, vals), permc_spec="MMD_AT_PLUS_A").solve(np.asarray(rhs, np.float64))
+    return splu(upper_bcsr_to_csc(rows, cols, vals), permc_spec="COLAMD").solve(np.asarray(rhs, np.float64))
+
+
+class UpperBcsrSolver:
+    """Host substitute for PARDISO on one BCSR pattern: reverse Cuthill-McKee ordering of the
+    3x3 block graph (computed once per pattern), then a banded Cholesky factorisation
+    (LAPACK pbsv through scipy.linalg.solveh_banded); if the matrix is not positive definite,
+    or its band is too wide, SuperLU (solve_upper_bcsr).  Not PARDISO."""
+
+    def __init__(self, rows, cols, max_band_entries=2e8):
+        import scipy.sparse as sp
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        self.rows, self.cols = np.asarray(rows), np.asarray(cols)
+        N = len(rows) - 1
+        self.n = 3 * N
+        br = np.repeat(np.arange(N), np.diff(rows))
+        G = sp.coo_matrix((np.ones(len(cols)), (br, cols)), shape=(N, N)).tocsr()
+        perm = reverse_cuthill_mckee((G + G.T).tocsr(), symmetric_mode=True)
+        inv = np.empty(N, np.int64)
+        inv[perm] = np.arange(N)
+        # scalar positions of every stored entry after the permutation, folded to the upper
+        # triangle; of a diagonal block only its upper half is taken (the block is symmetric)
+        k = np.arange(3)
+        r = (3 * inv[br])[:, None, None] + k[None, :, None] + np.zeros((1, 1, 3), np.int64)
+        c = (3 * inv[self.cols])[:, None, None] + k[None, None, :] + np.zeros((1, 3, 1), np.int64)
+        keep = np.ones(r.shape, bool)
+        diag = br == self.cols
+        keep[diag] = np.triu(np.ones((3, 3), bool))[None]
+        lo, hi = np.minimum(r, c)[keep], np.maximum(r, c)[keep]
+        self.take = np.flatnonzero(keep.ravel())
+        self.bw = int((hi - lo).max()) if len(lo) else 0
+        self.banded = (self.bw + 1) * self.n <= max_band_entries
+        self.ab_row, self.ab_col = self.bw + lo - hi, hi
+        self.perm = (3 * perm[:, None] + k[None, :]).ravel()
+
+    def solve(self, vals, rhs):
+        import scipy.linalg as la
+        rhs = np.asarray(rhs, np.float64)
+        if self.banded:
+            ab = np.zeros((self.bw + 1, self.n))
+            ab[self.ab_row, self.ab_col] = np.asarray(vals, np.float64).ravel()[self.take]
+            try:
+                y = la.solveh_banded(ab, rhs[self.perm], check_finite=False)
+                x = np.empty_like(y)
+                x[self.perm] = y
+                return x
+            except la.LinAlgError:
+                pass
+        return solve_upper_bcsr(self.rows, self.cols, vals, rhs)
 
 
 class ImplicitModel:
@@ -142,6 +190,7 @@ class ImplicitModel:
             self.g_icz.p if C else None, C, self.cs, None, 0, 0), self.ctx, "icf_pack_pcsr")
         self.rows = self.g_rows.download(self.N + 1)
         self.cols = self.g_cols.download(self.nnz)
+        self.solver = UpperBcsrSolver(self.rows, self.cols)
         self.g_vals = gf.buf("vals", 9 * self.nnz, np.float64)
         self.g_rhs = gf.buf("rhs", 3 * self.N, np.float64)
         self.g_dx = gf.buf("dx", 3 * self.N, np.float64)
@@ -196,7 +245,7 @@ class ImplicitModel:
                 self.g_rhs.p, self.N, ctypes.byref(dmg), ctypes.byref(fld)), ctx, "assemble")
             vals = self.g_vals.download(9 * self.nnz)
             rhs = self.g_rhs.download(3 * self.N)
-            dx = solve_upper_bcsr(self.rows, self.cols, vals, rhs)
+            dx = self.solver.solve(vals, rhs)
             norm = float(dx @ dx)                       # LinearSystem.NormOfDx (:68-74)
             norms.append(norm)
             # _checkDivergence (ImplicitModel3.cs:214-245)

@@ -338,3 +338,38 @@ def synthetic_state(col, consts, seed, strain=1e-4, noise_decades=(-2.5, -0.5), 
     pm[z] = 0.0
     tm[z] = 0.0
     return State(un, vn, an, col.x0 + un, pm, tm)
+
+
+def batch(collections):
+    """Many independent specimens as one collection: their meshes side by side, node, tet,
+    CZ and face ids offset per specimen, so that one pass over the flattened unit lists
+    assembles every specimen into its own diagonal block of one BCSR matrix (rows are
+    numbered specimen by specimen).  Specimen k's nodes are col.node_mesh == k when every
+    specimen is one deformable mesh."""
+    meshes = [m for c in collections for m in c.meshes]
+    anchored = None
+    if any(c.anchored is not None for c in collections):
+        anchored = np.concatenate([c.anchored if c.anchored is not None else
+                                   np.zeros(sum(len(m.x0) for m in c.meshes), bool) for c in collections])
+    out = Collection(meshes)
+    out.anchored = anchored
+    return out.finalize()
+
+
+def lognormal_cubes(n_specimens, seed, median_tets=12000, min_tets=2000, max_tets=40000, sigma=0.6,
+                    grains=(8, 64), size=0.02):
+    """The batched workload: cubes whose tet counts are lognormal (median `median_tets`,
+    log-standard deviation `sigma`, truncated to [min_tets, max_tets]), each a hex grid of
+    n^3 cells x 6 tets with n = round((tets/6)^(1/3)) and its own Voronoi grain count drawn
+    uniformly from `grains` and its own grain seed.  Specimens are laid out along x, apart."""
+    rng = np.random.default_rng(seed)
+    cols = []
+    x = 0.0
+    for k in range(n_specimens):
+        t = float(np.clip(rng.lognormal(np.log(median_tets), sigma), min_tets, max_tets))
+        n = max(2, int(round((t / 6.0) ** (1.0 / 3.0))))
+        g = int(rng.integers(grains[0], grains[1] + 1))
+        c = make_specimen((n, n, n), (size, size, size), g, int(rng.integers(1 << 30)), origin=(x, 0.0, 0.0))
+        cols.append(c)
+        x += 2 * size
+    return batch(cols)

@@ -59,7 +59,7 @@ def _oracle_run(orc, m, steps):
             b = np.zeros(3 * csr.N)
             orc.elem_force(iep, dn, h, A, b, T, es, ns)
             orc.cz_force(dcz, iczi, dn, A, b, h, C, cs, ns)
-            dx = newton.solve_upper_bcsr(csr.rows, csr.cols, A, b)
+            dx = newton.UpperBcsrSolver(csr.rows, csr.cols).solve(A, b)
             norm = float(dx @ dx)
             if it == 0:
                 err0 = norm if norm != 0 else p.conv_eps

@@ -252,3 +252,17 @@ def test_slot_with_more_contributions_than_one_chunk(orc, specimen_consts):
             assert rel_err(b, ref_b) <= TOL64
     finally:
         lib.icf_destroy(ctx)
+
+
+def test_mixed_precision_with_contact(orc, specimen_consts):
+    """Mixed mode on the contact pipeline: tet, CZ and impact blocks are FP32 values added in
+    FP64; residuals and node forces stay at the FP64 bar."""
+    col = small_specimen(cells=(5, 4, 4), size=(0.05, 0.04, 0.04), grains=5, seed=16)
+    st = make_state(col, specimen_consts, seed=17, platen_dz_frac=0.3)
+    p = Pipeline(col, specimen_consts, st, precision=_lib.ICF_MIXED, contact=True)
+    assert p.gf.nImpacts > 0
+    got = p.assemble()
+    ref = oracle_assemble(orc, p)
+    errs = _compare(got, ref, tol_vals=1e-5, tol_rhs=TOL64)
+    assert errs["vals"] > 0
+    p.gf.close()

@@ -45,3 +45,26 @@ def test_three_point_bend_supports_and_load_line(lib):
     u_half = prescribed(100 * 1e-4)
     assert np.allclose(u_half[on_top, 2], -1e-3) and not u_half[on_support].any()
     assert np.allclose(prescribed(1.0)[on_top, 2], -2e-3)
+
+
+def test_banded_solver_matches_dense_and_falls_back():
+    rng = np.random.default_rng(1)
+    N = 30
+    keep = rng.random((N, N)) < 0.15
+    keep = keep | keep.T | np.eye(N, dtype=bool)
+    M = rng.normal(size=(3 * N, 3 * N))
+    spd = np.where(np.kron(keep, np.ones((3, 3), bool)), M @ M.T, 0.0) + 40 * N * np.eye(3 * N)
+    indef = spd - 80 * N * np.eye(3 * N)
+    for A in (spd, indef):
+        rows, cols, vals = [0], [], []
+        for i in range(N):
+            for j in range(i, N):
+                if keep[i, j]:
+                    cols.append(j)
+                    vals.append(A[3 * i:3 * i + 3, 3 * j:3 * j + 3])
+            rows.append(len(cols))
+        rows, cols, vals = np.array(rows), np.array(cols), np.array(vals).ravel()
+        s = newton.UpperBcsrSolver(rows, cols)
+        b = rng.normal(size=3 * N)
+        x = s.solve(vals, b)
+        assert np.allclose(A @ x, b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
