@@ -190,7 +190,8 @@ def _ptr(a):
 def timed(runner, steps, warmup, how):
     """Seconds per step.  how = "dropin": blocking calls (each returns after its work, as the
     reference's Run()); "graph": replays of one captured iteration graph; "profile": the calls
-    enqueued without waiting, each kernel bracketed by HIP events on the library's stream."""
+    enqueued without waiting, each kernel bracketed by HIP events on the library's stream (the
+    library runs the fused call on one stream while profiling, so each kernel is timed alone)."""
     lib, ctx = runner.lib, runner.gf.ctx
     g = None
     if how == "graph":
@@ -380,6 +381,8 @@ def main():
     ap.add_argument("--specimens", type=int, default=256)
     ap.add_argument("--newton-steps", type=int, default=10,
                     help="steps of the FP64-vs-mixed three-point-bend Newton run (0: skip)")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="keep every kernel on the library's stream (no second-stream overlap)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run icf_d2d over this many MiB (a plain copy of known size, for "
                          "calibrating hardware byte counters)")
@@ -396,6 +399,8 @@ def main():
     else:
         consts, col, st = build_batched(args.specimens, 2)
     r = Runner(consts, col, st, _lib.ICF_MIXED, contact=contact)
+    if args.one_stream:
+        r.check(r.lib.icf_set_overlap(r.gf.ctx, 0), r.gf.ctx, "icf_set_overlap")
 
     # correctness guard of the two modes on this workload (assembled arrays, normwise)
     res = {}
@@ -481,8 +486,10 @@ def main():
                                 f"log-sd 0.6, seed 2), 8-64 grains each, PPR CZs, no contact, one launch per phase"),
                    "tets": T, "czs": C, "active_czs": active_C, "impacts": P, "candidate_pairs": r.Q,
                    "nodes": len(col.x0), "rows": r.gf.csr.N, "nnzb": r.gf.csr.nnz, "mode": head},
-        "timing": "one captured HIP graph per Newton iteration, replayed; per-kernel times from a separate "
-                  "pass with HIP events around each kernel on the library's stream",
+        "timing": ("one captured HIP graph per Newton iteration, replayed"
+                   + ("" if args.one_stream else " (independent kernels of the fused call on a second stream)")
+                   + "; per-kernel times from a separate pass with HIP events around each kernel, all on "
+                     "the library's stream"),
         "dropin_ms_per_step": {k: round(out[k]["dropin"][0] * 1e3, 4) for k in out},
         "fp64": ({"value": round(units / out["fp64"]["graph"][0] / 1e6, 3),
                   "ms_per_step": round(out["fp64"]["graph"][0] * 1e3, 4)} if "fp64" in out else None),

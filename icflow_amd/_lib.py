@@ -75,6 +75,7 @@ _SIGS = {
     "icf_graph_launch": (c_int, [c_void_p, c_void_p]),
     "icf_graph_destroy": (c_int, [c_void_p]),
     "icf_set_profiling": (c_int, [c_void_p, c_int]),
+    "icf_set_overlap": (c_int, [c_void_p, c_int]),
     "icf_profile_reset": (c_int, [c_void_p]),
     "icf_profile_count": (c_int, [c_void_p]),
     "icf_profile_entry": (c_int, [c_void_p, c_int, ctypes.c_char_p, c_int, ctypes.POINTER(c_double),

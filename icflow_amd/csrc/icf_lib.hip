@@ -86,6 +86,10 @@ struct icf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // second stream for independent work inside one call (forked from and joined back into
+    // `stream`, so callers see one stream); `cur` is where LAUNCH issues
+    hipStream_t stream2 = nullptr, cur = nullptr;
+    hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
     bool has_consts = false;
     int precision = ICF_FP64;
     DevConsts hc;
@@ -101,6 +105,7 @@ struct icf_ctx {
     double last_ms = -1;
     bool blocking = true;          // icf_set_blocking
     bool capturing = false;        // between icf_graph_begin and icf_graph_end
+    bool overlap = true;           // icf_set_overlap: independent kernels of one call on two streams
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
     struct Pending {
@@ -208,7 +213,7 @@ hipEvent_t prof_begin(icf_ctx *ctx)
     } else if (hipEventCreate(&e) != hipSuccess) {
         return nullptr;
     }
-    (void)hipEventRecord(e, ctx->stream);
+    (void)hipEventRecord(e, ctx->cur);
     return e;
 }
 
@@ -249,7 +254,7 @@ void prof_flush(icf_ctx *ctx)
 #define LAUNCH(ctx, name, kern, g, b, ...)                                   \
     do {                                                                     \
         hipEvent_t ea_ = prof_begin(ctx);                                    \
-        hipLaunchKernelGGL(kern, g, b, 0, (ctx)->stream, __VA_ARGS__);       \
+        hipLaunchKernelGGL(kern, g, b, 0, (ctx)->cur, __VA_ARGS__);          \
         prof_end(ctx, name, ea_);                                            \
     } while (0)
 
@@ -502,12 +507,48 @@ struct Timer {
 
 // ---- launch helpers ---------------------------------------------------------------------------
 
-int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int ns)
+// Work after fork(ctx, k) and before join(ctx, k) may be issued on stream2 (set ctx->cur);
+// join makes `stream` wait for it.
+// With overlap off, or while profiling (so each kernel's events time it alone), everything
+// stays on `stream`.
+bool two_streams(const icf_ctx *ctx) { return ctx->overlap && !ctx->profiling; }
+
+int fork(icf_ctx *ctx, int k)
+{
+    if (!two_streams(ctx)) return 0;
+    HIPCHK(ctx, hipEventRecord(ctx->ev_fork[k], ctx->stream));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork[k], 0));
+    return 0;
+}
+
+int join(icf_ctx *ctx, int k)
+{
+    ctx->cur = ctx->stream;
+    if (!two_streams(ctx)) return 0;
+    HIPCHK(ctx, hipEventRecord(ctx->ev_join[k], ctx->stream2));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[k], 0));
+    return 0;
+}
+
+void use_stream2(icf_ctx *ctx)
+{
+    if (two_streams(ctx)) ctx->cur = ctx->stream2;
+}
+
+// puts launches back on `stream` on every return path of a forking call
+struct LaunchOnStream {
+    icf_ctx *ctx;
+    ~LaunchOnStream() { ctx->cur = ctx->stream; }
+};
+
+// reserve > n: only grow the staging buffers for `reserve` units
+int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int ns, int reserve = 0)
 {
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const size_t rec = iso ? TET_ISO_REC : TET_GEN_REC;
-    if (int r = grow(ctx, ctx->tet_rhs, sizeof(double) * TET_RHS_REC * (size_t)(n > 0 ? n : 1))) return r;
-    if (int r = grow(ctx, ctx->tet_tan, (mixed ? 4 : 8) * rec * (size_t)(n > 0 ? n : 1))) return r;
+    const size_t cap = (size_t)std::max(std::max(n, reserve), 1);
+    if (int r = grow(ctx, ctx->tet_rhs, sizeof(double) * TET_RHS_REC * cap)) return r;
+    if (int r = grow(ctx, ctx->tet_tan, (mixed ? 4 : 8) * rec * cap)) return r;
     if (n == 0) return 0;
     dim3 g(blocks_for(n, 256)), b(256);
     double *sr = (double *)ctx->tet_rhs.p;
@@ -519,11 +560,12 @@ int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int
     return 0;
 }
 
-int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int cs, int ns)
+int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int cs, int ns, int reserve = 0)
 {
     const bool mixed = ctx->precision == ICF_MIXED;
-    if (int r = grow(ctx, ctx->cz_rhs, sizeof(double) * CZ_RHS_REC * (size_t)(n > 0 ? n : 1))) return r;
-    if (int r = grow(ctx, ctx->cz_tan, (mixed ? 4 : 8) * CZ_TAN_REC * (size_t)(n > 0 ? n : 1))) return r;
+    const size_t cap = (size_t)std::max(std::max(n, reserve), 1);
+    if (int r = grow(ctx, ctx->cz_rhs, sizeof(double) * CZ_RHS_REC * cap)) return r;
+    if (int r = grow(ctx, ctx->cz_tan, (mixed ? 4 : 8) * CZ_TAN_REC * cap)) return r;
     if (n == 0) return 0;
     const long long waves = (n + CZ_PER_WAVE - 1) / CZ_PER_WAVE;   // three lanes per CZ
     dim3 g(blocks_for(waves * 64, 256)), b(256);
@@ -627,6 +669,12 @@ int icf_create(int device, icf_ctx **out)
     memset(&ctx->hc, 0, sizeof(ctx->hc));
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        e = hipEventCreateWithFlags(&ctx->ev_fork[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming);
+    }
+    ctx->cur = ctx->stream;
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) e = hipMalloc(&ctx->small, 8 * sizeof(int));
@@ -657,6 +705,11 @@ int icf_destroy(icf_ctx *ctx)
         ctx->ev_pool.push_back(p.b);
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; k++) {
+        if (ctx->ev_fork[k]) (void)hipEventDestroy(ctx->ev_fork[k]);
+        if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
+    }
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
     delete ctx;
@@ -931,7 +984,16 @@ int icf_graph_destroy(icf_graph *g)
 int icf_set_profiling(icf_ctx *ctx, int on)
 {
     if (!ctx) return ICF_E_INVALID;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_set_profiling during capture");
     ctx->profiling = on != 0;
+    return 0;
+}
+
+int icf_set_overlap(icf_ctx *ctx, int on)
+{
+    if (!ctx) return ICF_E_INVALID;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_set_overlap during capture");
+    ctx->overlap = on != 0;
     return 0;
 }
 
@@ -1109,10 +1171,23 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
         if (tsum)
             if (int r = grow(ctx, ctx->tsum, sizeof(double) * 3 * (size_t)mf->nkeys)) return r;
     }
+    // buffers grow before any work is issued (growing synchronizes the context's stream)
+    if (int r = stage_tets(ctx, ie, dn, 0, el_stride, nd_stride, nElems)) return r;
+    if (int r = stage_czs(ctx, dcz, icz, dn, 0, cz_stride, nd_stride, nCZs)) return r;
     Timer tm(ctx);
+    LaunchOnStream on_stream{ctx};
+    // tets on the context's stream, CZs on the second one; then the node forces and rhs rows
+    // on the first, the BCSR values on the second
+    if (int r = fork(ctx, 0)) return r;
     if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
+    use_stream2(ctx);
     if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
+    if (int r = join(ctx, 0)) return r;
     Stage st = make_stage(ctx, h);
+    if (int r = fork(ctx, 1)) return r;
+    use_stream2(ctx);
+    launch_vals(ctx, mv, A, 0, st);
+    ctx->cur = ctx->stream;
     if (tsum) {
         // node forces, keeping each node's tet sum; rhs rows start from it and add their CZs
         LAUNCH(ctx, "k_reduce_vec3<force,acc>", (k_reduce_vec3<true, true>), dim3(blocks_for(mf->nlaunch, 256)),
@@ -1125,12 +1200,12 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
         if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
         launch_vec3(ctx, mr, b, 1, 3, 0, st);
     }
-    launch_vals(ctx, mv, A, 0, st);
     // the two counts of GPU_Functionality.cs:474-475 (kernel-cleared, so the call can be captured)
     LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(64), ctx->small + 2, 2, 0);
     if (nCZs)
         LAUNCH(ctx, "k_count_flags", k_count_flags, dim3(nCZs < 64 * 256 ? blocks_for(nCZs, 256) : 64), dim3(256),
                icz + (size_t)cz_stride * CZI_TDAMAGED, icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 2);
+    if (int r = join(ctx, 1)) return r;
     if (int r = tm.finish()) return r;
     if ((nCZDamaged || nCZFailed) && ctx->blocking && !ctx->capturing) {
         int sums[2] = {0, 0};

@@ -160,6 +160,13 @@ int icf_graph_launch(icf_ctx *ctx, icf_graph *g);
 int icf_graph_destroy(icf_graph *g);
 
 int icf_set_profiling(icf_ctx *ctx, int on);
+/* icf_set_overlap(ctx, 1) (the default): the fused call runs independent kernels on a second
+ * stream of the context (tets beside CZs, BCSR values beside node forces and rhs rows) and
+ * joins it back before returning, so callers still see one stream.  0 keeps every kernel on the
+ * context's stream.  While profiling is on the call runs on one stream either way, so each
+ * kernel's events time it alone.  No reference counterpart (the reference's Run() issues one
+ * stream, GPU_Functionality.cs:451-476). */
+int icf_set_overlap(icf_ctx *ctx, int on);
 int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);
 int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches);

@@ -90,6 +90,16 @@ def test_assembly_is_bitwise_deterministic(col, specimen_consts):
     b = p.assemble()
     for k in ("vals", "rhs", "F"):
         assert np.array_equal(a[k], b[k]), k
+    # one stream instead of two (and profiling, which also runs on one stream): same bits
+    gf = p.gf
+    for setter in (gf.lib.icf_set_overlap, gf.lib.icf_set_profiling):
+        setter(gf.ctx, 1 if setter is gf.lib.icf_set_profiling else 0)
+        gf.TransferNodesToGPU(st)
+        c = p.assemble()
+        for k in ("vals", "rhs", "F"):
+            assert np.array_equal(a[k], c[k]), k
+    gf.lib.icf_set_profiling(gf.ctx, 0)
+    gf.lib.icf_set_overlap(gf.ctx, 1)
     p.gf.close()
 
 
