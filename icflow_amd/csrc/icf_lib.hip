@@ -35,6 +35,63 @@ using namespace icf;
 
 namespace {
 
+// Device bytes a call reads or writes: up to MAXS spans, or `all` (conflicts with everything).
+struct Span {
+    uintptr_t lo, hi;
+    bool w;
+};
+struct Access {
+    static constexpr int MAXS = 12;
+    Span s[MAXS];
+    int n = 0;
+    bool all = false;
+    Access &add(const void *p, size_t bytes, bool w)
+    {
+        if (!p || !bytes) return *this;
+        if (n == MAXS) {
+            all = true;
+            return *this;
+        }
+        s[n++] = {(uintptr_t)p, (uintptr_t)p + bytes, w};
+        return *this;
+    }
+    template <typename T> Access &rd(const T *p, size_t count) { return add(p, count * sizeof(T), false); }
+    template <typename T> Access &wr(const T *p, size_t count) { return add(p, count * sizeof(T), true); }
+    // the whole allocation holding p, for buffers whose used extent the call does not know
+    Access &alloc(const void *p, bool w)
+    {
+        if (!p) return *this;
+        void *base = nullptr;
+        size_t sz = 0;
+        if (hipMemGetAddressRange(&base, &sz, const_cast<void *>(p)) == hipSuccess && sz) return add(base, sz, w);
+        (void)hipGetLastError();
+        all = true;
+        return *this;
+    }
+    static Access everything()
+    {
+        Access a;
+        a.all = true;
+        return a;
+    }
+};
+
+bool conflict(const Access &a, const Access &b)
+{
+    if (a.all || b.all) return true;
+    for (int i = 0; i < a.n; i++)
+        for (int j = 0; j < b.n; j++)
+            if ((a.s[i].w || b.s[j].w) && a.s[i].lo < b.s[j].hi && b.s[j].lo < a.s[i].hi) return true;
+    return false;
+}
+
+// A call issued on one lane, with the event recorded after its last launch.
+struct LaneRec {
+    Access a;
+    hipEvent_t ev;
+};
+constexpr int LANE_EVENTS = 32, LANE_LOG = 24;   // a logged event is one of the lane's last LANE_LOG
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -90,6 +147,13 @@ struct icf_ctx {
     // `stream`, so callers see one stream); `cur` is where LAUNCH issues
     hipStream_t stream2 = nullptr, cur = nullptr;
     hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
+    // lanes across calls: lane 0 is `stream`, lane 1 is `stream_c` (contact detection); each
+    // lane's log holds its calls that the other lane has not yet waited for
+    hipStream_t stream_c = nullptr;
+    hipEvent_t lane_ev[2][LANE_EVENTS] = {};
+    unsigned lane_ev_next[2] = {0, 0};
+    std::vector<LaneRec> lane_log[2];
+    bool lane0_unlogged = false;   // lane-0 work issued since the last lane-0 log entry
     bool has_consts = false;
     int precision = ICF_FP64;
     DevConsts hc;
@@ -105,7 +169,7 @@ struct icf_ctx {
     double last_ms = -1;
     bool blocking = true;          // icf_set_blocking
     bool capturing = false;        // between icf_graph_begin and icf_graph_end
-    bool overlap = true;           // icf_set_overlap: independent kernels of one call on two streams
+    bool overlap = true;           // icf_set_overlap: independent kernels on more than one stream
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
     struct Pending {
@@ -140,6 +204,7 @@ int grow(icf_ctx *ctx, DevBuf &b, size_t bytes)
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "an internal buffer must grow: run the call once uncaptured");
     if (b.p) {
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream_c));
         HIPCHK(ctx, hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
@@ -509,9 +574,11 @@ struct Timer {
 
 // Work after fork(ctx, k) and before join(ctx, k) may be issued on stream2 (set ctx->cur);
 // join makes `stream` wait for it.
-// With overlap off, or while profiling (so each kernel's events time it alone), everything
-// stays on `stream`.
-bool two_streams(const icf_ctx *ctx) { return ctx->overlap && !ctx->profiling; }
+// Used inside captured graphs, whose replays issue every kernel without host gaps: eager
+// blocking calls measured no faster with it, and eager non-blocking calls overlap across calls
+// instead (lanes, below).  With overlap off, or while profiling (so each kernel's events time it
+// alone), everything stays on `stream`.
+bool two_streams(const icf_ctx *ctx) { return ctx->overlap && !ctx->profiling && ctx->capturing; }
 
 int fork(icf_ctx *ctx, int k)
 {
@@ -539,6 +606,111 @@ void use_stream2(icf_ctx *ctx)
 struct LaunchOnStream {
     icf_ctx *ctx;
     ~LaunchOnStream() { ctx->cur = ctx->stream; }
+};
+
+// ---- lanes across calls -----------------------------------------------------------------------
+// When calls are enqueued without waiting (icf_set_blocking(0), outside capture, overlap on), the
+// contact-detection work -- narrow phase, closest face, the per-impact phase of the collision
+// response -- runs on lane 1 (`stream_c`) beside the assembly on lane 0 (`stream`).  Every call
+// declares the device bytes it reads and writes; before issuing, it waits for the latest call of
+// the other lane whose bytes conflict with its own (one of the two writes bytes the other
+// touches).  Blocking calls finish before they return, and captured calls replay as one graph
+// whose branches the runtime does not run side by side, so both use lane 0 only; lane-0 work
+// that was not logged is waited for in full by the next lane-1 call.
+
+bool lanes_on(const icf_ctx *ctx) { return ctx->overlap && !ctx->profiling && !ctx->blocking && !ctx->capturing; }
+
+hipStream_t lane_stream(const icf_ctx *ctx, int l) { return l ? ctx->stream_c : ctx->stream; }
+
+// lane for a call that may run on lane 1
+int contact_lane(const icf_ctx *ctx) { return lanes_on(ctx) ? 1 : 0; }
+
+hipEvent_t next_lane_event(icf_ctx *ctx, int l) { return ctx->lane_ev[l][ctx->lane_ev_next[l]++ % LANE_EVENTS]; }
+
+// `access` returns the call's Access; it is only evaluated when the other lane has logged calls
+template <typename G> int lane_enter(icf_ctx *ctx, int l, G &&access)
+{
+    if (l == 1 && ctx->lane0_unlogged) {
+        hipEvent_t ev = next_lane_event(ctx, 0);
+        HIPCHK(ctx, hipEventRecord(ev, ctx->stream));
+        HIPCHK(ctx, hipStreamWaitEvent(ctx->stream_c, ev, 0));
+        ctx->lane_log[0].clear();
+        ctx->lane0_unlogged = false;
+    }
+    std::vector<LaneRec> &other = ctx->lane_log[1 - l];
+    if (!other.empty()) {
+        auto &&a = access();
+        for (int i = (int)other.size() - 1; i >= 0; i--) {
+            if (!conflict(a, other[i].a)) continue;
+            HIPCHK(ctx, hipStreamWaitEvent(lane_stream(ctx, l), other[i].ev, 0));
+            // the other lane's calls up to i are now ordered before everything later on this lane
+            other.erase(other.begin(), other.begin() + i + 1);
+            break;
+        }
+    }
+    ctx->cur = lane_stream(ctx, l);
+    return 0;
+}
+
+template <typename G> int lane_exit(icf_ctx *ctx, int l, G &&access)
+{
+    ctx->cur = ctx->stream;
+    if (l == 0 && !lanes_on(ctx)) {
+        ctx->lane0_unlogged = true;
+        return 0;
+    }
+    hipEvent_t ev = next_lane_event(ctx, l);
+    HIPCHK(ctx, hipEventRecord(ev, lane_stream(ctx, l)));
+    std::vector<LaneRec> &log = ctx->lane_log[l];
+    if ((int)log.size() + 1 >= LANE_LOG) {
+        // the latest event stands for every earlier call of the lane
+        log.clear();
+        log.push_back({Access::everything(), ev});
+    } else {
+        log.push_back({access(), ev});
+    }
+    return 0;
+}
+
+Access everything() { return Access::everything(); }
+
+// A call that touches unknown memory, or returns host data: after all lane-1 work.
+int lane_barrier(icf_ctx *ctx) { return lane_enter(ctx, 0, everything); }
+
+// Both lanes idle, logs empty.
+int sync_lanes(icf_ctx *ctx)
+{
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream_c));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->lane_log[0].clear();
+    ctx->lane_log[1].clear();
+    ctx->lane0_unlogged = false;
+    return 0;
+}
+
+// One call's lane: enters on construction (check `err`), done() logs it.  `make` builds the
+// call's Access when a lane needs it.
+template <typename F> struct OnLane {
+    icf_ctx *ctx;
+    int lane;
+    F make;
+    Access acc;
+    bool have = false;
+    int err = 0;
+    OnLane(icf_ctx *c, int l, F f) : ctx(c), lane(l), make(f)
+    {
+        err = lane_enter(ctx, lane, [this]() -> const Access & { return get(); });
+    }
+    const Access &get()
+    {
+        if (!have) {
+            acc = make();
+            have = true;
+        }
+        return acc;
+    }
+    int done() { return lane_exit(ctx, lane, [this]() -> const Access & { return get(); }); }
+    ~OnLane() { ctx->cur = ctx->stream; }
 };
 
 // reserve > n: only grow the staging buffers for `reserve` units
@@ -670,10 +842,15 @@ int icf_create(int device, icf_ctx **out)
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream_c, hipStreamNonBlocking);
     for (int k = 0; k < 2 && e == hipSuccess; k++) {
         e = hipEventCreateWithFlags(&ctx->ev_fork[k], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming);
     }
+    for (int l = 0; l < 2; l++)
+        for (int k = 0; k < LANE_EVENTS && e == hipSuccess; k++)
+            e = hipEventCreateWithFlags(&ctx->lane_ev[l][k], hipEventDisableTiming);
+
     ctx->cur = ctx->stream;
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
@@ -690,6 +867,8 @@ int icf_destroy(icf_ctx *ctx)
 {
     if (!ctx) return ICF_E_INVALID;
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream_c) (void)hipStreamSynchronize(ctx->stream_c);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     (void)hipStreamSynchronize(ctx->stream);
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
@@ -709,6 +888,10 @@ int icf_destroy(icf_ctx *ctx)
         if (ctx->ev_fork[k]) (void)hipEventDestroy(ctx->ev_fork[k]);
         if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
     }
+    for (int l = 0; l < 2; l++)
+        for (int k = 0; k < LANE_EVENTS; k++)
+            if (ctx->lane_ev[l][k]) (void)hipEventDestroy(ctx->lane_ev[l][k]);
+    if (ctx->stream_c) (void)hipStreamDestroy(ctx->stream_c);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
@@ -823,7 +1006,7 @@ int icf_free(icf_ctx *ctx, void *dptr)
     if (!dptr) return 0;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_free during capture");
     if (int r = check_ctx(ctx)) return r;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int r = sync_lanes(ctx)) return r;
     size_t sz = 0;
     void *basep = nullptr;
     if (hipMemGetAddressRange(&basep, &sz, dptr) == hipSuccess && sz)
@@ -842,6 +1025,7 @@ int icf_h2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t byt
     if (!bytes) return 0;
     char *d = (char *)dst + dst_off;
     invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
+    if (int r = lane_barrier(ctx)) return r;
     HIPCHK(ctx, hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
@@ -853,6 +1037,7 @@ int icf_d2h(icf_ctx *ctx, void *dst, const void *src, size_t src_off, size_t byt
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "host copies cannot be captured");
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
+    if (int r = lane_barrier(ctx)) return r;
     HIPCHK(ctx, hipMemcpyAsync(dst, (const char *)src + src_off, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
@@ -865,6 +1050,7 @@ int icf_memset(icf_ctx *ctx, void *dst, int value, size_t bytes)
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
     invalidate_range(ctx, (uintptr_t)dst, (uintptr_t)dst + bytes);
+    if (int r = lane_barrier(ctx)) return r;
     HIPCHK(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
@@ -874,7 +1060,7 @@ int icf_synchronize(icf_ctx *ctx)
 {
     if (int r = check_ctx(ctx)) return r;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_synchronize during capture");
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int r = sync_lanes(ctx)) return r;
     prof_flush(ctx);
     return 0;
 }
@@ -896,9 +1082,12 @@ int icf_d2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t src
     if (!bytes) return 0;
     char *d = (char *)dst + dst_off;
     if (!ctx->capturing) invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
+    OnLane ln(ctx, 0, [&] { return Access().rd((const char *)src + src_off, bytes).wr(d, bytes); });
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     LAUNCH(ctx, "k_copy", k_copy, dim3(blocks_for((long long)((bytes + 15) / 16), 256)), dim3(256), d,
            (const char *)src + src_off, bytes);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -913,6 +1102,7 @@ int icf_cz_counts(icf_ctx *ctx, int *nCZDamaged, int *nCZFailed)
 {
     if (int r = check_ctx(ctx)) return r;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_cz_counts during capture");
+    if (int r = lane_barrier(ctx)) return r;
     int sums[2] = {0, 0};
     HIPCHK(ctx, hipMemcpyAsync(sums, ctx->small + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -931,7 +1121,7 @@ int icf_graph_begin(icf_ctx *ctx)
 {
     if (int r = check_ctx(ctx)) return r;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "capture already in progress");
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int r = sync_lanes(ctx)) return r;
     prof_flush(ctx);
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->capturing = true;
@@ -966,8 +1156,11 @@ int icf_graph_launch(icf_ctx *ctx, icf_graph *g)
     if (!g || g->device != ctx->device) return fail(ctx, ICF_E_INVALID, "icf_graph_launch: bad graph");
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_graph_launch during capture");
     if (int r = upload_consts(ctx)) return r;
+    OnLane ln(ctx, 0, everything);
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     HIPCHK(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1028,6 +1221,8 @@ int icf_elem_force(icf_ctx *ctx, const int *ie, double *dn, double h, double *A,
         return fail(ctx, ICF_E_INVALID, "icf_elem_force: invalid argument");
     if (int r = upload_consts(ctx)) return r;
     if (nElems == 0) return 0;
+    OnLane ln(ctx, 0, everything);
+    if (ln.err) return ln.err;
     SlotMap *mv, *mr, *mf;
     std::vector<SrcKey> sv{{KIND_TET, ie, nElems, el_stride, EL_SLOTS, 16}};
     std::vector<SrcKey> sr{{KIND_TET, ie, nElems, el_stride, EL_ROWS, 4}};
@@ -1041,6 +1236,7 @@ int icf_elem_force(icf_ctx *ctx, const int *ie, double *dn, double h, double *A,
     launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
     launch_vals(ctx, mv, A, 1, st);
     launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1052,6 +1248,8 @@ int icf_cz_force(icf_ctx *ctx, double *dcz, int *icz, double *dn, double *A, dou
         return fail(ctx, ICF_E_INVALID, "icf_cz_force: invalid argument");
     if (int r = upload_consts(ctx)) return r;
     if (nCZs == 0) return 0;
+    OnLane ln(ctx, 0, everything);
+    if (ln.err) return ln.err;
     SlotMap *mv, *mr;
     std::vector<SrcKey> sv{{KIND_CZ, icz, nCZs, cz_stride, CZI_SLOTS, 36}};
     std::vector<SrcKey> sr{{KIND_CZ, icz, nCZs, cz_stride, CZI_ROWS, 6}};
@@ -1062,6 +1260,7 @@ int icf_cz_force(icf_ctx *ctx, double *dcz, int *icz, double *dn, double *A, dou
     Stage st = make_stage(ctx, h);
     launch_vals(ctx, mv, A, 1, st);
     launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1072,9 +1271,18 @@ int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, 
     if (nPairs < 0 || el_all_stride < 0 || nd_stride < 0 || (nPairs && (!dn || !ie || !narrowList)))
         return fail(ctx, ICF_E_INVALID, "icf_narrow_phase: invalid argument");
     if (nPairs == 0) return 0;
+    // reads node positions and the tets' node ids, rewrites the list
+    OnLane ln(ctx, contact_lane(ctx), [&] {
+        return Access()
+            .rd(dn + (size_t)nd_stride * ND_XC, 3 * (size_t)nd_stride)
+            .rd(ie, 4 * (size_t)el_all_stride)
+            .wr(narrowList, 2 * (size_t)nPairs);
+    });
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     LAUNCH(ctx, "k_narrow_phase", k_narrow_phase, dim3(blocks_for(nPairs, 256)), dim3(256), nPairs, dn, ie,
            narrowList, el_all_stride, nd_stride);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1086,9 +1294,19 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
         (nPairs && (!nl2 || !dn || !ie || !faces)))
         return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: invalid argument");
     if (nPairs == 0) return 0;
+    // the adjacency rows of ie are as many as the mesh needs: the whole allocation is read
+    OnLane ln(ctx, contact_lane(ctx), [&] {
+        return Access()
+            .wr(nl2, 4 * (size_t)tet_stride)
+            .rd(dn + (size_t)nd_stride * ND_XC, 3 * (size_t)nd_stride)
+            .alloc(ie, false)
+            .rd(faces, 3 * (size_t)fc_stride);
+    });
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256), nPairs, tet_stride,
            nl2, dn, ie, faces, nd_stride, el_all_stride, fc_stride);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1111,6 +1329,17 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
     if (int r = grow(ctx, ctx->ct, (mixed ? sizeof(float) : sizeof(double)) * CT_TAN_REC * (size_t)nImpacts)) return r;
     if (int r = grow(ctx, ctx->ct_f, sizeof(double) * CT_F_REC * (size_t)nImpacts)) return r;
     Timer tm(ctx);
+    const size_t tan_bytes = (mixed ? sizeof(float) : sizeof(double)) * CT_TAN_REC * (size_t)nImpacts;
+    const size_t f_bytes = sizeof(double) * CT_F_REC * (size_t)nImpacts;
+    // per-impact phase: node positions and the impact records in, impact blocks out
+    OnLane l1(ctx, contact_lane(ctx), [&] {
+        return Access()
+            .rd(icr, (CR_ROWS + 4) * (size_t)cr_stride)
+            .rd(dn + (size_t)nd_stride * ND_XC, 3 * (size_t)nd_stride)
+            .wr((const char *)ctx->ct.p, tan_bytes)
+            .wr((const char *)ctx->ct_f.p, f_bytes);
+    });
+    if (l1.err) return l1.err;
     const dim3 cg(blocks_for((long long)nImpacts * CT_GROUP, 256)), cb(256);
     if (mixed)
         LAUNCH(ctx, "k_contact_phase1<mixed>", k_contact_phase1<float>, cg, cb, icr, dn, nImpacts, cr_stride, nd_stride,
@@ -1118,10 +1347,23 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
     else
         LAUNCH(ctx, "k_contact_phase1<fp64>", k_contact_phase1<double>, cg, cb, icr, dn, nImpacts, cr_stride,
                nd_stride, k, distanceEpsilon, (double *)ctx->ct.p, (double *)ctx->ct_f.p);
+    if (int r = l1.done()) return r;
+    // sums into node forces, BCSR blocks (at slots the call does not bound) and rhs rows
+    OnLane l0(ctx, 0, [&] {
+        return Access()
+            .rd((const char *)ctx->ct.p, tan_bytes)
+            .rd((const char *)ctx->ct_f.p, f_bytes)
+            .rd(icr, (CR_ROWS + 4) * (size_t)cr_stride)
+            .wr(dn + (size_t)nd_stride * ND_F, 3 * (size_t)nd_stride)
+            .alloc(A, true)
+            .alloc(b, true);
+    });
+    if (l0.err) return l0.err;
     Stage st = make_stage(ctx, 1.0);
     launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
     launch_vals(ctx, mv, A, 1, st, true);
     launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    if (int r = l0.done()) return r;
     return tm.finish();
 }
 
@@ -1132,6 +1374,7 @@ int icf_sum_int(icf_ctx *ctx, const int *d, int n, int *out)
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_sum_int returns a host value");
     *out = 0;
     if (n == 0) return 0;
+    if (int r = lane_barrier(ctx)) return r;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, ctx->small + 4, 1, 0);
     hipLaunchKernelGGL(k_sum_int, dim3(n < 64 * 256 ? blocks_for(n, 256) : 64), dim3(256), 0, ctx->stream, d, n,
                        ctx->small + 4);
@@ -1149,6 +1392,20 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
         (nElems && !ie) || (nCZs && (!dcz || !icz)) || !dn || (nnzb && !A) || (N && !b) || !(h > 0))
         return fail(ctx, ICF_E_INVALID, "icf_assemble_elems_and_czs: invalid argument");
     if (int r = upload_consts(ctx)) return r;
+    // node fields read (x0, u, v, a, x) and written (F); CZ records rewritten in place
+    const size_t ns = (size_t)nd_stride;
+    OnLane ln(ctx, 0, [&] {
+        return Access()
+            .rd(ie, (EL_ROWS + 4) * (size_t)el_stride)
+            .alloc(nCZs ? dcz : nullptr, true)
+            .alloc(nCZs ? icz : nullptr, true)
+            .rd(dn, ND_F * ns)
+            .wr(dn + ND_F * ns, 3 * ns)
+            .rd(dn + ND_XC * ns, 3 * ns)
+            .wr(A, 9 * (size_t)nnzb)
+            .wr(b, 3 * (size_t)N);
+    });
+    if (ln.err) return ln.err;
     std::vector<SrcKey> sv, sr, sfv;
     if (nElems) {
         sv.push_back({KIND_TET, ie, nElems, el_stride, EL_SLOTS, 16});
@@ -1206,6 +1463,7 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
         LAUNCH(ctx, "k_count_flags", k_count_flags, dim3(nCZs < 64 * 256 ? blocks_for(nCZs, 256) : 64), dim3(256),
                icz + (size_t)cz_stride * CZI_TDAMAGED, icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 2);
     if (int r = join(ctx, 1)) return r;
+    if (int r = ln.done()) return r;
     if (int r = tm.finish()) return r;
     if ((nCZDamaged || nCZFailed) && ctx->blocking && !ctx->capturing) {
         int sums[2] = {0, 0};
@@ -1225,9 +1483,12 @@ int icf_nodes_infer(icf_ctx *ctx, const double *st, int st_stride, double *dn, i
         !(beta > 0))
         return fail(ctx, ICF_E_INVALID, "icf_nodes_infer: invalid argument");
     if (nNodes == 0) return 0;
+    OnLane ln(ctx, 0, everything);
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     LAUNCH(ctx, "k_nodes_infer", k_nodes_infer, dim3(blocks_for(3LL * nNodes, 256)), dim3(256), st, st_stride, dn,
            nd_stride, nNodes, h, beta, gamma);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1237,9 +1498,12 @@ int icf_nodes_add_dx(icf_ctx *ctx, double *st, int st_stride, const int *altId, 
     if (bad_units(nNodes, st_stride) || (nNodes && (!st || !altId || !dx)))
         return fail(ctx, ICF_E_INVALID, "icf_nodes_add_dx: invalid argument");
     if (nNodes == 0) return 0;
+    OnLane ln(ctx, 0, everything);
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     LAUNCH(ctx, "k_nodes_add_dx", k_nodes_add_dx, dim3(blocks_for(3LL * nNodes, 256)), dim3(256), st, st_stride,
            altId, nNodes, dx);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1249,9 +1513,12 @@ int icf_nodes_accept(icf_ctx *ctx, double *st, int st_stride, const double *dn, 
     if (bad_units(nNodes, st_stride) || bad_units(nNodes, nd_stride) || (nNodes && (!st || !dn)) || !(h > 0))
         return fail(ctx, ICF_E_INVALID, "icf_nodes_accept: invalid argument");
     if (nNodes == 0) return 0;
+    OnLane ln(ctx, 0, everything);
+    if (ln.err) return ln.err;
     Timer tm(ctx);
     LAUNCH(ctx, "k_nodes_accept", k_nodes_accept, dim3(blocks_for(3LL * nNodes, 256)), dim3(256), st, st_stride, dn,
            nd_stride, nNodes, h);
+    if (int r = ln.done()) return r;
     return tm.finish();
 }
 
@@ -1260,6 +1527,8 @@ int icf_build_pattern(icf_ctx *ctx, const int *altId, int nNodes, const int *ie,
                       int *csr_rows, int *csr_cols, int max_nnz, int *nnz)
 {
     if (int r = check_ctx(ctx)) return r;
+    if (!ctx->capturing)
+        if (int r = lane_barrier(ctx)) return r;
     if (!nnz || nNodes < 0 || N < 0 || max_nnz < 0 || bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) ||
         bad_units(nImpacts, cr_stride) || (nNodes && !altId) || (nElems && !ie) || (nCZs && !icz) ||
         (nImpacts && !icr) || !csr_rows || (max_nnz && !csr_cols))
@@ -1320,6 +1589,8 @@ int icf_pack_pcsr(icf_ctx *ctx, const int *altId, int nNodes, const int *csr_row
                   int cr_stride)
 {
     if (int r = check_ctx(ctx)) return r;
+    if (!ctx->capturing)
+        if (int r = lane_barrier(ctx)) return r;
     if (nNodes < 0 || N < 0 || bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) ||
         bad_units(nImpacts, cr_stride) || (nNodes && !altId) || !csr_rows || (nElems && !ie) || (nCZs && !icz) ||
         (nImpacts && !icr))
@@ -1353,6 +1624,8 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
                     int exclude_shared, int *pairs, long long max_pairs, long long *nOut)
 {
     if (int r = check_ctx(ctx)) return r;
+    if (!ctx->capturing)
+        if (int r = lane_barrier(ctx)) return r;
     if (!nOut || bad_units(nTets, el_all_stride) || nd_stride < 0 || max_pairs < 0 || (nTets && (!dn || !ie)) ||
         (max_pairs && !pairs))
         return fail(ctx, ICF_E_INVALID, "icf_broad_phase: invalid argument");
@@ -1422,6 +1695,8 @@ int icf_nl2set(icf_ctx *ctx, const int *masks, const int *pairs, int nPairs, con
                int *nl2, int tet_stride, int *nOut)
 {
     if (int r = check_ctx(ctx)) return r;
+    if (!ctx->capturing)
+        if (int r = lane_barrier(ctx)) return r;
     if (!nOut || nPairs < 0 || el_all_stride < 0 || tet_stride < 0 || (nPairs && (!masks || !pairs || !ie)) ||
         (tet_stride && !nl2))
         return fail(ctx, ICF_E_INVALID, "icf_nl2set: invalid argument");

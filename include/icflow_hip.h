@@ -160,12 +160,17 @@ int icf_graph_launch(icf_ctx *ctx, icf_graph *g);
 int icf_graph_destroy(icf_graph *g);
 
 int icf_set_profiling(icf_ctx *ctx, int on);
-/* icf_set_overlap(ctx, 1) (the default): the fused call runs independent kernels on a second
- * stream of the context (tets beside CZs, BCSR values beside node forces and rhs rows) and
- * joins it back before returning, so callers still see one stream.  0 keeps every kernel on the
- * context's stream.  While profiling is on the call runs on one stream either way, so each
- * kernel's events time it alone.  No reference counterpart (the reference's Run() issues one
- * stream, GPU_Functionality.cs:451-476). */
+/* icf_set_overlap(ctx, 1) (the default) lets independent kernels share the GPU:
+ *  - non-blocking (icf_set_blocking(ctx, 0)), outside capture: the contact-detection work
+ *    (icf_narrow_phase, icf_find_closest_face, the per-impact phase of icf_collision_response)
+ *    runs on a second stream of the context beside the assembly.  Each call waits for the
+ *    earlier calls that write bytes it reads, or read or write bytes it writes, so results are
+ *    those of one stream; device memory touched outside the library needs icf_synchronize().
+ *  - captured: the fused call runs tets beside CZs and the BCSR values beside the node forces
+ *    and rhs rows, joined back before it returns.
+ * 0 keeps every kernel on the context's stream.  While profiling is on everything runs on one
+ * stream, so each kernel's events time it alone.  No reference counterpart (the reference's
+ * Run() issues one stream, GPU_Functionality.cs:451-476). */
 int icf_set_overlap(icf_ctx *ctx, int on);
 int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);
