@@ -251,7 +251,7 @@ def algorithmic_bytes(r):
         # tuple in/out 24 B + adjacency count and ids + face ids
         "k_closest_face": 24 * P + 4 * P * (1 + adj),
         # impact int record (96 B) + 4 nodes' x_current (96 B)
-        "k_contact_phase1": 96 * P + 96 * P,
+        "k_contact_impact": 96 * P + 96 * P,
         # read-modify-write of the touched blocks + the 16 slot fields per impact
         "k_reduce_vals+acc": 144 * touched["vals"] + 64 * P,
         "k_reduce_vec3<rhs+acc>": 48 * touched["rhs"] + 16 * P,
@@ -302,7 +302,7 @@ def _rocprof_symbol(name):
         return f"{base}<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}>"
     if base in ("k_cz_phase1",):
         return f"{base}<{b(args[0] == 'mixed')}>"
-    if base == "k_contact_phase1":
+    if base == "k_contact_impact":
         return f"{base}<{'float' if args[0] == 'mixed' else 'double'}>"
     return name
 

@@ -2,16 +2,12 @@
 //
 // Restated from collision_kernels9/collision_response.h:38-126 (kCollisionResponseForce) and
 // point_triange_v2.h:13-842 (pt and its helpers).  The reference evaluates one impact per
-// thread with 12x12 temporaries; here one impact is shared by a group of 8 lanes: lane l
-// (l < 6) produces Hessian rows l and 11-l (13 upper-triangle entries each) and the forces of
-// those rows.  Every first-derivative term an entry needs is recomputed from the impact's few
-// vectors by index selection, so no lane holds per-index arrays, and each entry is the same
-// formula as the reference's.
+// thread with 12x12 temporaries; here each thread also takes one impact, but computes every
+// upper-triangle entry of the Hessian from the impact's few vectors and stores it at once,
+// so no 12x12 array is held.
 #pragma once
 
 namespace icf {
-
-constexpr int CT_GROUP = 4;
 
 __device__ inline double sel3(const double v[3], int c) { return c == 0 ? v[0] : (c == 1 ? v[1] : v[2]); }
 
@@ -26,9 +22,6 @@ struct Impact {
     int ea, eb;
     double y0[3], y1[3], y2[3];
     double q, g, f2, f2sq, f2cu, if2, if2sq, if2cu;   // if* = reciprocals
-    // interior branch
-    double E1[3], E2[3], W[3], a, b, c, d, e, det, det2, det3, Ns, Nt, ps, pt, pu, r[3];
-    double idet, idet2, idet3;
 };
 
 // coordinate k of impact node nd (0 = point, 1..3 = triangle vertices), by selection so that
@@ -88,36 +81,25 @@ __device__ inline void setup_ve(Impact &I, int na, int nb)
     I.if2cu = 1.0 / I.f2cu;
 }
 
-// point_plane_distance (point_triange_v2.h:449-582), the values shared by all entries.
+// point_plane_distance (point_triange_v2.h:449-582): the squared distance through the
+// projection's barycentric coordinates, as the reference evaluates it.
 __device__ inline void setup_pp(Impact &I)
 {
+    double E1[3], E2[3], W[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        I.E1[k] = I.v1[k] - I.v0[k];
-        I.E2[k] = I.v2[k] - I.v0[k];
-        I.W[k] = I.v0[k] - I.p[k];
+        E1[k] = I.v1[k] - I.v0[k];
+        E2[k] = I.v2[k] - I.v0[k];
+        W[k] = I.v0[k] - I.p[k];
     }
-    I.a = dot3(I.E1, I.E1);
-    I.b = dot3(I.E2, I.E1);
-    I.c = dot3(I.E2, I.E2);
-    I.d = dot3(I.W, I.E1);
-    I.e = dot3(I.W, I.E2);
-    I.det = I.a * I.c - I.b * I.b;
-    I.det2 = I.det * I.det;
-    I.det3 = I.det2 * I.det;
-    I.idet = 1.0 / I.det;
-    I.idet2 = 1.0 / I.det2;
-    I.idet3 = 1.0 / I.det3;
-    I.Ns = I.b * I.e - I.c * I.d;
-    I.Nt = I.b * I.d - I.a * I.e;
-    I.ps = I.Ns * (1. / I.det);
-    I.pt = I.Nt * (1. / I.det);
-    I.pu = 1 - (I.ps + I.pt);
+    const double a = dot3(E1, E1), b = dot3(E2, E1), c = dot3(E2, E2), d = dot3(W, E1), e = dot3(W, E2);
+    const double det = a * c - b * b;
+    const double ps = (b * e - c * d) * (1. / det), pt = (b * d - a * e) * (1. / det), pu = 1 - (ps + pt);
     double v = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        I.r[k] = I.v1[k] * I.ps + I.v2[k] * I.pt + I.v0[k] * I.pu - I.p[k];
-        v += I.r[k] * I.r[k];
+        const double r = I.v1[k] * ps + I.v2[k] * pt + I.v0[k] * pu - I.p[k];
+        v += r * r;
     }
     I.dsq = v;
 }
@@ -248,81 +230,6 @@ __device__ inline double ve_hess(const Impact &I, int r, int c)
     const double gdd = 2 * (A.qd * B.qd + I.q * qdd);
     return -2 * I.g * A.f2d * B.f2d * I.if2cu + (gdI * B.f2d + gdJ * A.f2d) * I.if2sq + f1dd + I.g * f2dd * I.if2sq -
            gdd * I.if2;
-}
-
-// ---- interior: chain rule through s(x), t(x) ----
-struct PpD { double a1, b1, c1, d1, e1, dt1, s1, t1, u1; };
-__device__ inline PpD pp_d1(const Impact &I, int i)
-{
-    const int nd = i / 3, k = i % 3;
-    const double e1k = sel3(I.E1, k), e2k = sel3(I.E2, k), wk = sel3(I.W, k);
-    PpD D;
-    if (nd == 0) { D.a1 = 0; D.b1 = 0; D.c1 = 0; D.d1 = -e1k; D.e1 = -e2k; }
-    else if (nd == 1) { D.a1 = -2 * e1k; D.b1 = -e1k - e2k; D.c1 = -2 * e2k; D.d1 = e1k - wk; D.e1 = e2k - wk; }
-    else if (nd == 2) { D.a1 = 2 * e1k; D.b1 = e2k; D.c1 = 0; D.d1 = wk; D.e1 = 0; }
-    else { D.a1 = 0; D.b1 = e1k; D.c1 = 2 * e2k; D.d1 = 0; D.e1 = wk; }
-    D.dt1 = I.c * D.a1 + I.a * D.c1 - 2 * I.b * D.b1;
-    D.s1 = ((I.c * I.d - I.b * I.e) * D.dt1) * I.idet2 + ((I.e * D.b1 + I.b * D.e1) - (I.d * D.c1 + I.c * D.d1)) * I.idet;
-    D.t1 = ((I.a * I.e - I.b * I.d) * D.dt1) * I.idet2 + ((I.d * D.b1 + I.b * D.d1) - (I.a * D.e1 + I.e * D.a1)) * I.idet;
-    D.u1 = -(D.s1 + D.t1);
-    return D;
-}
-// d r_k / d x_i
-__device__ inline double pp_dr(const Impact &I, const PpD &D, int i, int k)
-{
-    double v = sel3(I.v1, k) * D.s1 + sel3(I.v2, k) * D.t1 + sel3(I.v0, k) * D.u1;
-    const int nd = i / 3;
-    if (i % 3 == k) v += (nd == 0 ? -1.0 : nd == 1 ? I.pu : nd == 2 ? I.ps : I.pt);
-    return v;
-}
-__device__ inline double pp_grad(const Impact &I, int i)
-{
-    const PpD D = pp_d1(I, i);
-    return 2 * (I.r[0] * pp_dr(I, D, i, 0) + I.r[1] * pp_dr(I, D, i, 1) + I.r[2] * pp_dr(I, D, i, 2));
-}
-// constant node patterns of the second derivatives of a..e (point_triange_v2.h:298-371)
-__device__ inline void pp_patterns(int ni, int nj, double &a2, double &b2, double &c2, double &d2, double &e2)
-{
-    const int k = ni * 4 + nj;
-    a2 = (k == 5 || k == 10) ? 2.0 : ((k == 6 || k == 9) ? -2.0 : 0.0);
-    b2 = k == 5 ? 2.0 : ((k == 6 || k == 9 || k == 7 || k == 13) ? -1.0 : ((k == 11 || k == 14) ? 1.0 : 0.0));
-    c2 = (k == 5 || k == 15) ? 2.0 : ((k == 7 || k == 13) ? -2.0 : 0.0);
-    d2 = (k == 1 || k == 4 || k == 6 || k == 9) ? 1.0 : ((k == 2 || k == 8) ? -1.0 : (k == 5 ? -2.0 : 0.0));
-    e2 = (k == 1 || k == 4 || k == 7 || k == 13) ? 1.0 : ((k == 3 || k == 12) ? -1.0 : (k == 5 ? -2.0 : 0.0));
-}
-__device__ inline double pp_hess(const Impact &I, const PpD &Di, int i, int j)
-{
-    const PpD Dj = pp_d1(I, j);
-    const int ni = i / 3, nj = j / 3;
-    double a2 = 0, b2 = 0, c2 = 0, d2 = 0, e2 = 0;
-    if ((i % 3) == (j % 3)) pp_patterns(ni, nj, a2, b2, c2, d2, e2);
-    const double a = I.a, b = I.b, c = I.c, d = I.d, e = I.e;
-    const double ddet = -2 * Di.b1 * Dj.b1 + Dj.a1 * Di.c1 + Di.a1 * Dj.c1 + c * a2 - 2 * b * b2 + a * c2;
-    const double s2 = (-(Dj.c1 * Di.d1) - Di.c1 * Dj.d1 + Dj.b1 * Di.e1 + Di.b1 * Dj.e1 + e * b2 - d * c2 - c * d2 + b * e2) * I.idet -
-                      ((Dj.dt1 * (e * Di.b1 - d * Di.c1 - c * Di.d1 + b * Di.e1)) +
-                       (Di.dt1 * (e * Dj.b1 - d * Dj.c1 - c * Dj.d1 + b * Dj.e1)) + (I.Ns * ddet)) * I.idet2 +
-                      (2 * I.Ns * Di.dt1 * Dj.dt1) * I.idet3;
-    const double t2 = (Dj.b1 * Di.d1 + Di.b1 * Dj.d1 - Dj.a1 * Di.e1 - Di.a1 * Dj.e1 - e * a2 + d * b2 + b * d2 - a * e2) * I.idet -
-                      ((Dj.dt1 * (-(e * Di.a1) + d * Di.b1 + b * Di.d1 - a * Di.e1)) +
-                       (Di.dt1 * (-(e * Dj.a1) + d * Dj.b1 + b * Dj.d1 - a * Dj.e1)) + (I.Nt * ddet)) * I.idet2 +
-                      (2 * I.Nt * Di.dt1 * Dj.dt1) * I.idet3;
-    const double u2 = -(s2 + t2);
-    double acc = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        double v = sel3(I.v1, k) * s2 + sel3(I.v2, k) * t2 + sel3(I.v0, k) * u2;
-        if (i % 3 == k) v += (ni == 1 ? Dj.u1 : ni == 2 ? Dj.s1 : ni == 3 ? Dj.t1 : 0);
-        if (j % 3 == k) v += (nj == 1 ? Di.u1 : nj == 2 ? Di.s1 : nj == 3 ? Di.t1 : 0);
-        acc += pp_dr(I, Di, i, k) * pp_dr(I, Dj, j, k) + I.r[k] * v;
-    }
-    return 2 * acc;
-}
-
-__device__ inline double grad_at(const Impact &I, int r)
-{
-    if (I.branch == 0) return pp_grad(I, r);
-    if (I.branch <= 3) return vv_grad(I, r);
-    return ve_grad(I, r);
 }
 
 // Entry (r, c), r <= c, of the tangent into an impact's block record; inside a diagonal
@@ -478,19 +385,17 @@ __device__ __attribute__((always_inline)) inline void pp_closed_row(const PpClos
     }
 }
 
-// Phase 1 for impacts: the tangent blocks (k/2) Hess d^2 and the 12 forces
-// f = [-fp, w0 fp, w1 fp, w2 fp] with fp = (k/2) d(d^2)/dp (collision_response.h:89-114).
-// Rigid-rigid impacts and d^2 <= eps^2 stage zeros.  CT_GROUP lanes per impact; lane l
-// produces rows l, 7 - l and 8 + l (20, 20, 19 and 19 entries for l = 0..3).
+// Phase 1 for impacts, one thread per impact: the tangent blocks (k/2) Hess d^2 and the 12
+// forces f = [-fp, w0 fp, w1 fp, w2 fp] with fp = (k/2) d(d^2)/dp (collision_response.h:89-114).
+// Rigid-rigid impacts and d^2 <= eps^2 stage zeros.  Every (row, column) index is a
+// compile-time constant after unrolling, so the entry formulas' index selections fold away.
 template <typename T>
-__global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ icr, const double *__restrict__ dn,
+__global__ __launch_bounds__(256) void k_contact_impact(const int *__restrict__ icr, const double *__restrict__ dn,
                                                         int n, int cs, int ns, double kpen, double eps,
                                                         T *__restrict__ st_tan, double *__restrict__ st_f)
 {
-    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / CT_GROUP;
-    const int lane = threadIdx.x % CT_GROUP;
-    if (gid >= n) return;
-    const int u = gid;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
     T *tan = st_tan + (size_t)u * CT_TAN_REC;
     double *frc = st_f + (size_t)u * CT_F_REC;
     bool active = false;
@@ -511,14 +416,11 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
         active = I.dsq > eps * eps;
     }
     const double hk = kpen / 2;
-    const int rows[3] = {lane, 7 - lane, 8 + lane};
     if (!active) {
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++) {
-            const int r = rows[rr];
-            for (int c = r; c < 12; c++) ct_store(tan, r, c, 0.0);
-            frc[r] = 0.;
-        }
+        for (int k = 0; k < CT_TAN_REC; k++) tan[k] = (T)0;
+#pragma unroll
+        for (int r = 0; r < 12; r++) frc[r] = 0.;
         return;
     }
     double gpk[3];    // (k/2) d(d^2)/dp
@@ -526,31 +428,27 @@ __global__ __launch_bounds__(256) void k_contact_phase1(const int *__restrict__ 
         PpClosed P;
         pp_closed_setup(I, P);
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++) pp_closed_row(P, rows[rr], hk, tan);
+        for (int r = 0; r < 12; r++) pp_closed_row(P, r, hk, tan);
 #pragma unroll
         for (int k = 0; k < 3; k++) gpk[k] = hk * P.gp * P.N[k];
+    } else if (I.branch <= 3) {
+#pragma unroll
+        for (int r = 0; r < 12; r++)
+#pragma unroll
+            for (int c = r; c < 12; c++) ct_store(tan, r, c, hk * vv_hess(I, r, c));
+#pragma unroll
+        for (int k = 0; k < 3; k++) gpk[k] = hk * vv_grad(I, k);
     } else {
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++) {
-            const int r = rows[rr];
-            if (I.branch <= 3) {
-                for (int c = r; c < 12; c++) ct_store(tan, r, c, hk * vv_hess(I, r, c));
-            } else {
-                for (int c = r; c < 12; c++) ct_store(tan, r, c, hk * ve_hess(I, r, c));
-            }
-        }
+        for (int r = 0; r < 12; r++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) gpk[k] = hk * (I.branch <= 3 ? vv_grad(I, k) : ve_grad(I, k));
-    }
-    // force of row r: the point gets -fp, triangle vertex m gets w_m fp
+            for (int c = r; c < 12; c++) ct_store(tan, r, c, hk * ve_hess(I, r, c));
 #pragma unroll
-    for (int rr = 0; rr < 3; rr++) {
-        const int r = rows[rr];
-        const double fpc = pick3(gpk, r % 3);
-        const int nd = r / 3;
-        const double w = nd == 0 ? -1.0 : (nd == 1 ? 1 - (I.s + I.t) : (nd == 2 ? I.s : I.t));
-        frc[r] = nd == 0 ? -fpc : w * fpc;
+        for (int k = 0; k < 3; k++) gpk[k] = hk * ve_grad(I, k);
     }
+    const double w[4] = {-1.0, 1 - (I.s + I.t), I.s, I.t};
+#pragma unroll
+    for (int r = 0; r < 12; r++) frc[r] = w[r / 3] * gpk[r % 3];
 }
 
 }  // namespace icf

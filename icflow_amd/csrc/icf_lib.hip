@@ -119,8 +119,8 @@ struct SlotMap {
     uint32_t *codes = nullptr;     // ncodes
     int *keys = nullptr;           // accumulate mode: the keys with contributions (null: all keys)
     int nlaunch = 0;               // keys a reduction visits
-    int *chunks[2] = {nullptr, nullptr};   // BCSR-value maps: chunk starts (list positions), per
-    int nchunks[2] = {0, 0};               // staging type (0: FP32 values, 1: FP64 values)
+    int2 *chunks[2] = {nullptr, nullptr};  // BCSR-value maps, per staging type (0: FP32 values, 1:
+    int nchunks[2] = {0, 0};               // FP64 values): chunk starts (list position, code position)
     int id = 0;                    // unique per map built
     // rhs maps of the fused call: the force map (by id) whose per-node tet sums seed the rows,
     // the outcome of that check, and per row the node and the first non-tet code
@@ -447,25 +447,28 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
             HIPCHK(ctx, hipMemcpyAsync(h_list.data(), list, sizeof(int) * (size_t)nlist, hipMemcpyDeviceToHost,
                                        ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        // a chunk's codes are contiguous (keys in between, if any, have none): each entry is the
+        // chunk's first list position and first code position; the last entry closes the table
+        auto key_of = [&](int p) { return list ? h_list[p] : p; };
         for (int kind = 0; kind < 2; kind++) {
             const int cap = kind == 0 ? FLAT_CAP_F : FLAT_CAP_D;
-            std::vector<int> ch(1, 0);
+            std::vector<int2> ch(1, make_int2(0, h_off[key_of(0)]));
             int cnt = 0, nk = 0;
             for (int p = 0; p < nlist; p++) {
-                const int k = list ? h_list[p] : p;
+                const int k = key_of(p);
                 const int c = h_off[k + 1] - h_off[k];
                 if (nk > 0 && (nk == FLAT_KEYS || cnt + c > cap)) {
-                    ch.push_back(p);
+                    ch.push_back(make_int2(p, h_off[k]));
                     cnt = 0;
                     nk = 0;
                 }
                 cnt += c;
                 nk++;
             }
-            ch.push_back(nlist);
+            ch.push_back(make_int2(nlist, h_off[key_of(nlist - 1) + 1]));
             m.nchunks[kind] = (int)ch.size() - 1;
-            HIPCHK(ctx, hipMalloc(&m.chunks[kind], sizeof(int) * ch.size()));
-            HIPCHK(ctx, hipMemcpyAsync(m.chunks[kind], ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice,
+            HIPCHK(ctx, hipMalloc(&m.chunks[kind], sizeof(int2) * ch.size()));
+            HIPCHK(ctx, hipMemcpyAsync(m.chunks[kind], ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice,
                                        ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         }
@@ -1340,12 +1343,12 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
             .wr((const char *)ctx->ct_f.p, f_bytes);
     });
     if (l1.err) return l1.err;
-    const dim3 cg(blocks_for((long long)nImpacts * CT_GROUP, 256)), cb(256);
+    const dim3 cg(blocks_for(nImpacts, 256)), cb(256);
     if (mixed)
-        LAUNCH(ctx, "k_contact_phase1<mixed>", k_contact_phase1<float>, cg, cb, icr, dn, nImpacts, cr_stride, nd_stride,
+        LAUNCH(ctx, "k_contact_impact<mixed>", k_contact_impact<float>, cg, cb, icr, dn, nImpacts, cr_stride, nd_stride,
                k, distanceEpsilon, (float *)ctx->ct.p, (double *)ctx->ct_f.p);
     else
-        LAUNCH(ctx, "k_contact_phase1<fp64>", k_contact_phase1<double>, cg, cb, icr, dn, nImpacts, cr_stride,
+        LAUNCH(ctx, "k_contact_impact<fp64>", k_contact_impact<double>, cg, cb, icr, dn, nImpacts, cr_stride,
                nd_stride, k, distanceEpsilon, (double *)ctx->ct.p, (double *)ctx->ct_f.p);
     if (int r = l1.done()) return r;
     // sums into node forces, BCSR blocks (at slots the call does not bound) and rhs rows

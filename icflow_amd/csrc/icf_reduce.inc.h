@@ -175,8 +175,9 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
 }
 
 // Phase 2 for BCSR values, one workgroup per chunk of consecutive keys (at most FLAT_KEYS
-// keys and flat_cap<L>() contributions; the chunk tables are built with the map).  The chunk's
-// contributions are contiguous in `codes`: first every thread evaluates contributions
+// keys and flat_cap<L>() contributions; the chunk table, built with the map, holds each
+// chunk's first list and code positions).  The chunk's contributions are contiguous in
+// `codes`: first every thread evaluates contributions
 // (uniform work across lanes) into LDS, then one thread per key adds its contributions from
 // LDS in list order into an FP64 accumulator and writes the block.  The values and their
 // order are those of a sequential loop over the key's list, so the result does not depend
@@ -186,7 +187,7 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
 // L is the staging type: float for tet and CZ blocks in mixed mode, double otherwise.
 template <bool MIXED, bool ISO, bool ACC, typename L>
 __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
-                                                           const int *__restrict__ keys, const int *__restrict__ chunks,
+                                                           const int *__restrict__ keys, const int2 *__restrict__ chunks,
                                                            int nchunks, double *__restrict__ A, Stage sv)
 {
     constexpr int CAP = flat_cap<L>();
@@ -195,11 +196,19 @@ __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict
     __shared__ L lds[CAP * 9];
     const int ch = xcd_remap(blockIdx.x, gridDim.x);
     if (ch >= nchunks) return;
-    const int p0 = chunks[ch], p1 = chunks[ch + 1];
-    const int kfirst = keys ? keys[p0] : p0, klast = keys ? keys[p1 - 1] : p1 - 1;
-    const int cb = off[kfirst], ce = off[klast + 1];
+    const int2 c0 = chunks[ch], c1 = chunks[ch + 1];
+    const int p0 = c0.x, p1 = c1.x, cb = c0.y, ce = c1.y;
     const int nc = ce - cb;
     const bool staged = nc <= CAP;
+    // this thread's key and its code range, loaded beside the chunk's codes
+    const int p = p0 + threadIdx.x;
+    const bool live = p < p1;
+    int s = 0, b = 0, e = 0;
+    if (live) {
+        s = keys ? keys[p] : p;
+        b = off[s];
+        e = off[s + 1];
+    }
     if (staged) {
         // codes first (coalesced), then the contributions, so their loads overlap
         uint32_t cd[PER];
@@ -220,15 +229,10 @@ __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict
         }
         __syncthreads();
     }
-    const int p = p0 + threadIdx.x;
-    const bool live = p < p1;
     double acc[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) acc[i] = 0.;
-    int s = 0;
     if (live) {
-        s = keys ? keys[p] : p;
-        const int b = off[s], e = off[s + 1];
         if (ACC) {
             const double *src = A + (size_t)s * 9;
 #pragma unroll
