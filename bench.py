@@ -236,7 +236,7 @@ def algorithmic_bytes(r):
     n_surf = int(np.unique(col.surf).size)
     adj = getattr(gf, "maxAdjacentFaces", 0)
     touched = r.contact_touched
-    return {
+    alg = {
         # ids 16 B/tet + x0,un,vn,an 96 B per node
         "k_tet_phase1": 16 * T + 96 * n_def,
         # failed flag + 6 ids + pmax/tmax in (76 B), tentative pmax/tmax/averages + 3 flags out (92 B);
@@ -261,6 +261,12 @@ def algorithmic_bytes(r):
         "k_count_flags": 8 * C,
         "k_fill_int": 8,
     }
+    # launches that run several of the phases above side by side
+    alg["k_phase1"] = alg["k_tet_phase1"] + alg["k_cz_phase1"]
+    alg["k_vals_force"] = alg["k_reduce_vals"] + alg["k_reduce_vec3<force>"]
+    alg["k_rhs_counts"] = alg["k_reduce_vec3<rhs>"] + alg["k_count_flags"]
+    alg["k_contact_sums"] = alg["k_reduce_vals+acc"] + alg["k_reduce_vec3<rhs+acc>"] + alg["k_reduce_vec3<force+acc>"]
+    return alg
 
 
 def iteration_bytes(r):
@@ -285,8 +291,11 @@ def kernel_bytes(alg, name):
 
 def _rocprof_symbol(name):
     """The kernel symbol rocprofv3 reports for a per-kernel profile name of the library
-    (template arguments spelled out as the compiler prints them)."""
+    (template arguments spelled out as the compiler prints them; for kernels templated on their
+    workgroup size, without that trailing argument and the closing bracket)."""
     import re
+    if name == "k_rhs_counts":
+        return "k_rhs_counts_fused"
     m = re.match(r"(\w+)<(.*)>$", name)
     if not m:
         return name
@@ -295,15 +304,23 @@ def _rocprof_symbol(name):
     if base == "k_reduce_vals":
         if args[0] == "contact":
             mixed = args[1] == "mixed"
-            return f"k_reduce_vals<{b(mixed)}, false, true, {'float' if mixed else 'double'}>"
+            return f"k_reduce_vals<{b(mixed)}, false, true, {'float' if mixed else 'double'}"
         mixed, iso, acc = args[0] == "mixed", args[1] == "iso", "acc" in args
-        return f"k_reduce_vals<{b(mixed)}, {b(iso)}, {b(acc)}, {'float' if mixed else 'double'}>"
+        return f"k_reduce_vals<{b(mixed)}, {b(iso)}, {b(acc)}, {'float' if mixed else 'double'}"
     if base in ("k_tet_phase1",):
         return f"{base}<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}>"
     if base in ("k_cz_phase1",):
         return f"{base}<{b(args[0] == 'mixed')}>"
     if base == "k_contact_impact":
         return f"{base}<{'float' if args[0] == 'mixed' else 'double'}>"
+    if base == "k_phase1":
+        return f"k_phase1_fused<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}>"
+    if base == "k_vals_force":
+        mixed = args[0] == "mixed"
+        return f"k_vals_force_fused<{b(mixed)}, {b(args[1] == 'iso')}, {'float' if mixed else 'double'}"
+    if base == "k_contact_sums":
+        mixed = args[0] == "mixed"
+        return f"k_contact_sums<{b(mixed)}, {'float' if mixed else 'double'}"
     return name
 
 
@@ -313,7 +330,8 @@ def profiled_traffic(name, path=os.path.join(ROOT, "profiles", "assembly_compres
     sym = _rocprof_symbol(name)
     try:
         for line in open(path):
-            if line.startswith(f"| `{sym}` |") and line.count("|") == 5:
+            head = f"| `{sym}"
+            if line.startswith(head) and line[len(head)] in ",>`" and line.count("|") == 5:
                 return float(line.split("|")[4]) * 1e6
     except OSError:
         return None

@@ -16,12 +16,11 @@ constexpr int CZ_PER_WAVE = 21;
 __device__ inline double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_cz_phase1(double *__restrict__ dcz, int *__restrict__ icz,
-                                                   const double *__restrict__ dn, int n, int cs, int ns,
-                                                   double *__restrict__ st_rhs, void *__restrict__ st_tan)
+__device__ __forceinline__ void cz_phase1(int bid_, int nb_, double *__restrict__ dcz, int *__restrict__ icz,
+        const double *__restrict__ dn, int n, int cs, int ns, double *__restrict__ st_rhs, void *__restrict__ st_tan)
 {
     const int wl = threadIdx.x & 63;                              // lane in the wave
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int wave = (bid_ * blockDim.x + threadIdx.x) >> 6;
     const int slot = wl / 3, gp = wl % 3;                         // CZ within the wave, Gauss point
     const int base = slot * 3;                                    // lane of this CZ's gp 0
     int u = wave * CZ_PER_WAVE + slot;
@@ -203,6 +202,14 @@ __global__ __launch_bounds__(256) void k_cz_phase1(double *__restrict__ dcz, int
 #pragma unroll
         for (int i = 0; i < 6; i++) t[i] = q[i];
     }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_cz_phase1(double *__restrict__ dcz, int *__restrict__ icz,
+                                                   const double *__restrict__ dn, int n, int cs, int ns,
+                                                   double *__restrict__ st_rhs, void *__restrict__ st_tan)
+{
+    cz_phase1<MIXED>(blockIdx.x, gridDim.x, dcz, icz, dn, n, cs, ns, st_rhs, st_tan);
 }
 
 }  // namespace icf

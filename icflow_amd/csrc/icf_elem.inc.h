@@ -121,12 +121,11 @@ __device__ inline void apply_K0(const double g[12], double V, const double y[12]
 // (element_kernels2.h:228-251), written FP64 to st_rhs; tangent record to st_tan
 // (double, or float when MIXED).
 template <bool MIXED, bool ISO>
-__global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, const double *__restrict__ dn,
-                                                    int n, int es, int ns, double *__restrict__ st_rhs,
-                                                    void *__restrict__ st_tan)
+__device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restrict__ ie,
+        const double *__restrict__ dn, int n, int es, int ns, double *__restrict__ st_rhs, void *__restrict__ st_tan)
 {
-    const int nb = gridDim.x;
-    const int u = xcd_remap(blockIdx.x, nb) * blockDim.x + threadIdx.x;
+    const int nb = nb_;
+    const int u = xcd_remap(bid_, nb) * blockDim.x + threadIdx.x;
     if (u >= n) return;
 
     int nid[4];
@@ -263,6 +262,14 @@ __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, 
             t[22] = t[23] = 0.;
         }
     }
+}
+
+template <bool MIXED, bool ISO>
+__global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, const double *__restrict__ dn,
+                                                    int n, int es, int ns, double *__restrict__ st_rhs,
+                                                    void *__restrict__ st_tan)
+{
+    tet_phase1<MIXED, ISO>(blockIdx.x, gridDim.x, ie, dn, n, es, ns, st_rhs, st_tan);
 }
 
 // Phase 2: LHS block (i,j) of an isotropic tet record.  LHS = stiff * R K R^T + mcoef V M

@@ -27,6 +27,7 @@ __constant__ DevConsts g_c;
 #include "icf_contact.inc.h"
 #include "icf_contact_resp.inc.h"
 #include "icf_reduce.inc.h"
+#include "icf_fused.inc.h"
 #include "icf_nodes.inc.h"
 #include "icf_pattern.inc.h"
 #include "icf_broad.inc.h"
@@ -121,6 +122,7 @@ struct SlotMap {
     int nlaunch = 0;               // keys a reduction visits
     int2 *chunks[2] = {nullptr, nullptr};  // BCSR-value maps, per staging type (0: FP32 values, 1:
     int nchunks[2] = {0, 0};               // FP64 values): chunk starts (list position, code position)
+    int chunk_keys = 0;                    // key limit of the chunks (one thread per key)
     int id = 0;                    // unique per map built
     // rhs maps of the fused call: the force map (by id) whose per-node tet sums seed the rows,
     // the outcome of that check, and per row the node and the first non-tet code
@@ -141,12 +143,10 @@ int g_const_owner[64];             // per device: id of the context whose consta
 struct icf_ctx {
     int id = 0;
     int device = 0;
+    int ncu = 256;                 // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // second stream for independent work inside one call (forked from and joined back into
-    // `stream`, so callers see one stream); `cur` is where LAUNCH issues
-    hipStream_t stream2 = nullptr, cur = nullptr;
-    hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
+    hipStream_t cur = nullptr;       // where LAUNCH issues: `stream`, or `stream_c` (lanes)
     // lanes across calls: lane 0 is `stream`, lane 1 is `stream_c` (contact detection); each
     // lane's log holds its calls that the other lane has not yet waited for
     hipStream_t stream_c = nullptr;
@@ -437,9 +437,10 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
     m.keys = list;                 // null when overwriting: every key, in order
     if (target == TGT_VALS && nlist > 0) {
         // chunks of consecutive list positions for the staged reduction of BCSR values, one
-        // table per staging type: at most FLAT_KEYS keys and FLAT_CAP_F (FP32) or FLAT_CAP_D
-        // (FP64) contributions each (a single key with more contributions gets a chunk of its
-        // own, reduced without staging)
+        // table per staging type: at most chunk_keys keys and flat_cap_of(FP32?, chunk_keys)
+        // contributions each (a single key with more contributions gets a chunk of its own,
+        // reduced without staging)
+        m.chunk_keys = list ? FLAT_KEYS_ACC : FLAT_KEYS_OW;
         std::vector<int> h_off((size_t)nkeys + 1), h_list(list ? (size_t)nlist : 0);
         HIPCHK(ctx, hipMemcpyAsync(h_off.data(), m.off, sizeof(int) * ((size_t)nkeys + 1), hipMemcpyDeviceToHost,
                                    ctx->stream));
@@ -451,13 +452,13 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         // chunk's first list position and first code position; the last entry closes the table
         auto key_of = [&](int p) { return list ? h_list[p] : p; };
         for (int kind = 0; kind < 2; kind++) {
-            const int cap = kind == 0 ? FLAT_CAP_F : FLAT_CAP_D;
+            const int cap = flat_cap_of(kind == 0, m.chunk_keys);
             std::vector<int2> ch(1, make_int2(0, h_off[key_of(0)]));
             int cnt = 0, nk = 0;
             for (int p = 0; p < nlist; p++) {
                 const int k = key_of(p);
                 const int c = h_off[k + 1] - h_off[k];
-                if (nk > 0 && (nk == FLAT_KEYS || cnt + c > cap)) {
+                if (nk > 0 && (nk == m.chunk_keys || cnt + c > cap)) {
                     ch.push_back(make_int2(p, h_off[k]));
                     cnt = 0;
                     nk = 0;
@@ -575,42 +576,6 @@ struct Timer {
 
 // ---- launch helpers ---------------------------------------------------------------------------
 
-// Work after fork(ctx, k) and before join(ctx, k) may be issued on stream2 (set ctx->cur);
-// join makes `stream` wait for it.
-// Used inside captured graphs, whose replays issue every kernel without host gaps: eager
-// blocking calls measured no faster with it, and eager non-blocking calls overlap across calls
-// instead (lanes, below).  With overlap off, or while profiling (so each kernel's events time it
-// alone), everything stays on `stream`.
-bool two_streams(const icf_ctx *ctx) { return ctx->overlap && !ctx->profiling && ctx->capturing; }
-
-int fork(icf_ctx *ctx, int k)
-{
-    if (!two_streams(ctx)) return 0;
-    HIPCHK(ctx, hipEventRecord(ctx->ev_fork[k], ctx->stream));
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork[k], 0));
-    return 0;
-}
-
-int join(icf_ctx *ctx, int k)
-{
-    ctx->cur = ctx->stream;
-    if (!two_streams(ctx)) return 0;
-    HIPCHK(ctx, hipEventRecord(ctx->ev_join[k], ctx->stream2));
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[k], 0));
-    return 0;
-}
-
-void use_stream2(icf_ctx *ctx)
-{
-    if (two_streams(ctx)) ctx->cur = ctx->stream2;
-}
-
-// puts launches back on `stream` on every return path of a forking call
-struct LaunchOnStream {
-    icf_ctx *ctx;
-    ~LaunchOnStream() { ctx->cur = ctx->stream; }
-};
-
 // ---- lanes across calls -----------------------------------------------------------------------
 // When calls are enqueued without waiting (icf_set_blocking(0), outside capture, overlap on), the
 // contact-detection work -- narrow phase, closest face, the per-impact phase of the collision
@@ -618,8 +583,9 @@ struct LaunchOnStream {
 // declares the device bytes it reads and writes; before issuing, it waits for the latest call of
 // the other lane whose bytes conflict with its own (one of the two writes bytes the other
 // touches).  Blocking calls finish before they return, and captured calls replay as one graph
-// whose branches the runtime does not run side by side, so both use lane 0 only; lane-0 work
-// that was not logged is waited for in full by the next lane-1 call.
+// (whose independent branches the HIP runtime was measured to run one after the other), so
+// both use lane 0 only; lane-0 work that was not logged is waited for in full by the next
+// lane-1 call.
 
 bool lanes_on(const icf_ctx *ctx) { return ctx->overlap && !ctx->profiling && !ctx->blocking && !ctx->capturing; }
 
@@ -753,6 +719,76 @@ int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int 
     return 0;
 }
 
+// ---- fused launches (icf_fused.inc.h) ----------------------------------------------------------
+
+int round8(long long n) { return (int)((n + 7) / 8 * 8); }
+
+// Phase 1 of the fused call as one launch; the staging buffers have already grown.
+void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es, int ns, double *dcz, int *icz, int nc,
+                  int cs, int *counts)
+{
+    const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
+    const int nbt = nt ? round8(blocks_for(nt, 256)) : 0;
+    const int nbc = nc ? (int)blocks_for((long long)(nc + CZ_PER_WAVE - 1) / CZ_PER_WAVE * 64, 256) : 0;
+    const dim3 g(std::max(nbt + nbc, 1)), b(256);
+    double *tr = (double *)ctx->tet_rhs.p, *cr = (double *)ctx->cz_rhs.p;
+    void *tt = ctx->tet_tan.p, *ct = ctx->cz_tan.p;
+#define ICF_P1(MX, IS, NAME)                                                                                      \
+    LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS>), g, b, nbt, ie, dn, nt, es, ns, tr, tt, dcz, icz, nc, cs, cr, ct, counts)
+    if (mixed && iso) ICF_P1(true, true, "k_phase1<mixed,iso>");
+    else if (mixed) ICF_P1(true, false, "k_phase1<mixed,gen>");
+    else if (iso) ICF_P1(false, true, "k_phase1<fp64,iso>");
+    else ICF_P1(false, false, "k_phase1<fp64,gen>");
+#undef ICF_P1
+}
+
+// BCSR values (overwrite) and node forces with their tet sums as one launch.  False when the
+// value map is not an overwrite map of FLAT_KEYS_OW-key chunks: the caller launches them apart.
+bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap *mf, double *F, int ns,
+                      const Stage &st)
+{
+    static_assert(FLAT_KEYS_OW == 256, "fused with 256-lane workgroups");
+    if (mv->keys || mv->chunk_keys != FLAT_KEYS_OW) return false;
+    const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
+    const int k = mixed ? 0 : 1, nch = mv->nlaunch ? mv->nchunks[k] : 0;
+    const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, 256)) : 0, nbv = nch;
+    if (nbv + nbf == 0) return true;
+    const dim3 g(nbv + nbf), b(256);
+    double *ts = (double *)ctx->tsum.p;
+#define ICF_VF(MX, IS, L, NAME)                                                                                   \
+    LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, FLAT_KEYS_OW>), g, b, nbf, mv->off, mv->codes, mv->chunks[k], \
+           nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts)
+    if (mixed && iso) ICF_VF(true, true, float, "k_vals_force<mixed,iso>");
+    else if (mixed) ICF_VF(true, false, float, "k_vals_force<mixed,gen>");
+    else if (iso) ICF_VF(false, true, double, "k_vals_force<fp64,iso>");
+    else ICF_VF(false, false, double, "k_vals_force<fp64,gen>");
+#undef ICF_VF
+    return true;
+}
+
+// The collision response's sums into node forces, BCSR blocks and rhs rows as one launch.
+// False when the value map's chunks are not FLAT_KEYS_ACC = 256-key ones.
+bool contact_sums_fused(icf_ctx *ctx, const SlotMap *mf, double *F, int ns, const SlotMap *mv, double *A,
+                        const SlotMap *mr, double *b, const Stage &st)
+{
+    static_assert(FLAT_KEYS_ACC == 256, "fused with 256-lane workgroups");
+    if (!mv->keys || mv->chunk_keys != FLAT_KEYS_ACC) return false;
+    const bool mixed = ctx->precision == ICF_MIXED;
+    const int k = mixed ? 0 : 1, nch = mv->nlaunch ? mv->nchunks[k] : 0;
+    const int nb0 = mf->nlaunch ? round8(blocks_for(mf->nlaunch, 256)) : 0, nb1 = round8(nch);
+    const int nb2 = mr->nlaunch ? (int)blocks_for(mr->nlaunch, 256) : 0;
+    if (nb0 + nb1 + nb2 == 0) return true;
+    const dim3 g(nb0 + nb1 + nb2), bl(256);
+#define ICF_CS(MX, L, NAME)                                                                                       \
+    LAUNCH(ctx, NAME, (k_contact_sums<MX, L, FLAT_KEYS_ACC>), g, bl, nb0, nb1, mf->off, mf->codes, mf->keys,       \
+           mf->nlaunch, F, ns, mv->off, mv->codes, mv->keys, mv->chunks[k], nch, A, mr->off, mr->codes, mr->keys,   \
+           mr->nlaunch, b, st)
+    if (mixed) ICF_CS(true, float, "k_contact_sums<mixed>");
+    else ICF_CS(false, double, "k_contact_sums<fp64>");
+#undef ICF_CS
+    return true;
+}
+
 Stage make_stage(icf_ctx *ctx, double h)
 {
     Stage sv;
@@ -768,13 +804,24 @@ Stage make_stage(icf_ctx *ctx, double h)
     return sv;
 }
 
+template <bool MX, bool IS, bool ACC, typename L, int KEYS>
+void vals_launch(icf_ctx *ctx, const char *name, const SlotMap *m, int k, double *A, const Stage &sv)
+{
+    const int n = m->nchunks[k];
+    LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L, KEYS>), dim3(n), dim3(KEYS), m->off, m->codes, m->keys,
+           m->chunks[k], n, A, sv);
+}
+
 template <bool MX, bool IS, bool ACC, typename L>
 void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, const Stage &sv)
 {
     const int k = sizeof(L) == 4 ? 0 : 1;
     if (m->nchunks[k] == 0) return;
-    LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L>), dim3(m->nchunks[k]), dim3(FLAT_KEYS), m->off, m->codes,
-           m->keys, m->chunks[k], m->nchunks[k], A, sv);
+    // the workgroup size is the key limit the map's chunk table was built with
+    if (m->chunk_keys == FLAT_KEYS_ACC)
+        vals_launch<MX, IS, ACC, L, FLAT_KEYS_ACC>(ctx, name, m, k, A, sv);
+    else
+        vals_launch<MX, IS, ACC, L, FLAT_KEYS_OW>(ctx, name, m, k, A, sv);
 }
 
 // Phase 2 of BCSR values for map m.  accumulate: add into A over the keys with contributions
@@ -843,16 +890,13 @@ int icf_create(int device, icf_ctx **out)
     ctx->device = device;
     memset(&ctx->hc, 0, sizeof(ctx->hc));
     e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream_c, hipStreamNonBlocking);
-    for (int k = 0; k < 2 && e == hipSuccess; k++) {
-        e = hipEventCreateWithFlags(&ctx->ev_fork[k], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming);
-    }
     for (int l = 0; l < 2; l++)
         for (int k = 0; k < LANE_EVENTS && e == hipSuccess; k++)
             e = hipEventCreateWithFlags(&ctx->lane_ev[l][k], hipEventDisableTiming);
+
 
     ctx->cur = ctx->stream;
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
@@ -871,7 +915,6 @@ int icf_destroy(icf_ctx *ctx)
     if (!ctx) return ICF_E_INVALID;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream_c) (void)hipStreamSynchronize(ctx->stream_c);
-    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     (void)hipStreamSynchronize(ctx->stream);
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
@@ -887,15 +930,10 @@ int icf_destroy(icf_ctx *ctx)
         ctx->ev_pool.push_back(p.b);
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
-    for (int k = 0; k < 2; k++) {
-        if (ctx->ev_fork[k]) (void)hipEventDestroy(ctx->ev_fork[k]);
-        if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
-    }
     for (int l = 0; l < 2; l++)
         for (int k = 0; k < LANE_EVENTS; k++)
             if (ctx->lane_ev[l][k]) (void)hipEventDestroy(ctx->lane_ev[l][k]);
     if (ctx->stream_c) (void)hipStreamDestroy(ctx->stream_c);
-    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
     delete ctx;
@@ -1363,9 +1401,12 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
     });
     if (l0.err) return l0.err;
     Stage st = make_stage(ctx, 1.0);
-    launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
-    launch_vals(ctx, mv, A, 1, st, true);
-    launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    double *F = dn + (size_t)nd_stride * ND_F;
+    if (!(ctx->overlap && contact_sums_fused(ctx, mf, F, nd_stride, mv, A, mr, b, st))) {
+        launch_vec3(ctx, mf, F, nd_stride, 1, 1, st);
+        launch_vals(ctx, mv, A, 1, st, true);
+        launch_vec3(ctx, mr, b, 1, 3, 1, st);
+    }
     if (int r = l0.done()) return r;
     return tm.finish();
 }
@@ -1435,37 +1476,46 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
     if (int r = stage_tets(ctx, ie, dn, 0, el_stride, nd_stride, nElems)) return r;
     if (int r = stage_czs(ctx, dcz, icz, dn, 0, cz_stride, nd_stride, nCZs)) return r;
     Timer tm(ctx);
-    LaunchOnStream on_stream{ctx};
-    // tets on the context's stream, CZs on the second one; then the node forces and rhs rows
-    // on the first, the BCSR values on the second
-    if (int r = fork(ctx, 0)) return r;
-    if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
-    use_stream2(ctx);
-    if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
-    if (int r = join(ctx, 0)) return r;
-    Stage st = make_stage(ctx, h);
-    if (int r = fork(ctx, 1)) return r;
-    use_stream2(ctx);
-    launch_vals(ctx, mv, A, 0, st);
-    ctx->cur = ctx->stream;
-    if (tsum) {
-        // node forces, keeping each node's tet sum; rhs rows start from it and add their CZs
-        LAUNCH(ctx, "k_reduce_vec3<force,acc>", (k_reduce_vec3<true, true>), dim3(blocks_for(mf->nlaunch, 256)),
-               dim3(256), mf->off, mf->codes, mf->keys, mf->nlaunch, dn + (size_t)nd_stride * ND_F, nd_stride, 1, st,
-               (double *)ctx->tsum.p);
-        if (N)
-            LAUNCH(ctx, "k_reduce_rhs_tsum", k_reduce_rhs_tsum, dim3(blocks_for(N, 256)), dim3(256), mr->off,
-                   mr->codes, mr->cz_begin, mr->row_node, (const double *)ctx->tsum.p, N, b, st);
+    int *counts = ctx->small + 2;   // the two counts of GPU_Functionality.cs:474-475
+    const int *dmg = icz ? icz + (size_t)cz_stride * CZI_TDAMAGED : nullptr;
+    const int *fld = icz ? icz + (size_t)cz_stride * CZI_TFAILED : nullptr;
+    const dim3 cnt_grid(nCZs < 64 * 256 ? blocks_for(std::max(nCZs, 1), 256) : 64);
+    const bool fuse = ctx->overlap;
+    // phase 1: tets and CZs (the counters are cleared by a kernel, so the call can be captured)
+    if (fuse) {
+        phase1_fused(ctx, ie, dn, nElems, el_stride, nd_stride, dcz, icz, nCZs, cz_stride, counts);
     } else {
-        if (mf) launch_vec3(ctx, mf, dn + (size_t)nd_stride * ND_F, nd_stride, 1, 1, st);
-        launch_vec3(ctx, mr, b, 1, 3, 0, st);
+        LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(64), counts, 2, 0);
+        if (int r = stage_tets(ctx, ie, dn, nElems, el_stride, nd_stride)) return r;
+        if (int r = stage_czs(ctx, dcz, icz, dn, nCZs, cz_stride, nd_stride)) return r;
     }
-    // the two counts of GPU_Functionality.cs:474-475 (kernel-cleared, so the call can be captured)
-    LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(1), dim3(64), ctx->small + 2, 2, 0);
-    if (nCZs)
-        LAUNCH(ctx, "k_count_flags", k_count_flags, dim3(nCZs < 64 * 256 ? blocks_for(nCZs, 256) : 64), dim3(256),
-               icz + (size_t)cz_stride * CZI_TDAMAGED, icz + (size_t)cz_stride * CZI_TFAILED, nCZs, ctx->small + 2);
-    if (int r = join(ctx, 1)) return r;
+    Stage st = make_stage(ctx, h);
+    double *F = dn + (size_t)nd_stride * ND_F;
+    if (tsum) {
+        // BCSR values and node forces (keeping each node's tet sum), then the rhs rows from
+        // those sums plus their CZ terms, and the counts
+        if (!(fuse && vals_force_fused(ctx, mv, A, mf, F, nd_stride, st))) {
+            launch_vals(ctx, mv, A, 0, st);
+            LAUNCH(ctx, "k_reduce_vec3<force,acc>", (k_reduce_vec3<true, true>), dim3(blocks_for(mf->nlaunch, 256)),
+                   dim3(256), mf->off, mf->codes, mf->keys, mf->nlaunch, F, nd_stride, 1, st, (double *)ctx->tsum.p);
+        }
+        if (fuse && (N || nCZs)) {
+            const int nbr = N ? (int)(blocks_for(N, 256) + 7) / 8 * 8 : 0;
+            LAUNCH(ctx, "k_rhs_counts", k_rhs_counts_fused, dim3(nbr + (nCZs ? cnt_grid.x : 0)), dim3(256), nbr,
+                   mr->off, mr->codes, mr->cz_begin, mr->row_node, (const double *)ctx->tsum.p, N, b, st, dmg, fld,
+                   nCZs, counts);
+        } else {
+            if (N)
+                LAUNCH(ctx, "k_reduce_rhs_tsum", k_reduce_rhs_tsum, dim3(blocks_for(N, 256)), dim3(256), mr->off,
+                       mr->codes, mr->cz_begin, mr->row_node, (const double *)ctx->tsum.p, N, b, st);
+            if (nCZs) LAUNCH(ctx, "k_count_flags", k_count_flags, cnt_grid, dim3(256), dmg, fld, nCZs, counts);
+        }
+    } else {
+        launch_vals(ctx, mv, A, 0, st);
+        if (mf) launch_vec3(ctx, mf, F, nd_stride, 1, 1, st);
+        launch_vec3(ctx, mr, b, 1, 3, 0, st);
+        if (nCZs) LAUNCH(ctx, "k_count_flags", k_count_flags, cnt_grid, dim3(256), dmg, fld, nCZs, counts);
+    }
     if (int r = ln.done()) return r;
     if (int r = tm.finish()) return r;
     if ((nCZDamaged || nCZFailed) && ctx->blocking && !ctx->capturing) {

@@ -120,12 +120,14 @@ __global__ __launch_bounds__(256) void k_map_compact(const int *__restrict__ fla
 }
 
 // Chunk limits of the staged BCSR-value reduction: keys per workgroup (one thread each) and
-// contributions staged in LDS: FLAT_CAP_F FP32 or FLAT_CAP_D FP64 values x 9 (27 KB either
-// way, so that five workgroups share a CU).  Each map has a chunk table per staging type.
-constexpr int FLAT_KEYS = 256;
-constexpr int FLAT_CAP_F = 768, FLAT_CAP_D = 384;
-template <typename L>
-__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? FLAT_CAP_F : FLAT_CAP_D; }
+// contributions staged in LDS, 3 FP32 or 1.5 FP64 values x 9 per key (27 KB for 256 keys,
+// so that five such workgroups share a CU).  Overwrite maps (every key of the BCSR, in order)
+// and accumulate maps (the keys with contributions) each have their own key limit; a map has
+// a chunk table per staging type.
+constexpr int FLAT_KEYS_OW = 256, FLAT_KEYS_ACC = 256;
+template <typename L, int KEYS>
+__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? 3 * KEYS : 3 * KEYS / 2; }
+__host__ __device__ constexpr int flat_cap_of(bool fp32, int keys) { return fp32 ? 3 * keys : 3 * keys / 2; }
 
 // Staged per-unit records of the current call.
 struct Stage {
@@ -174,8 +176,8 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
     for (int k = 0; k < 9; k++) v[k] = (L)acc[k];
 }
 
-// Phase 2 for BCSR values, one workgroup per chunk of consecutive keys (at most FLAT_KEYS
-// keys and flat_cap<L>() contributions; the chunk table, built with the map, holds each
+// Phase 2 for BCSR values, one workgroup per chunk of consecutive keys (at most KEYS keys and
+// flat_cap<L, KEYS>() contributions; the chunk table, built with the map, holds each
 // chunk's first list and code positions).  The chunk's contributions are contiguous in
 // `codes`: first every thread evaluates contributions
 // (uniform work across lanes) into LDS, then one thread per key adds its contributions from
@@ -185,16 +187,16 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
 //   keys   null: chunk positions are keys; else keys[position] (accumulate mode)
 //   ACC    add to A (single-kernel calls); otherwise overwrite, zero where nothing contributes
 // L is the staging type: float for tet and CZ blocks in mixed mode, double otherwise.
-template <bool MIXED, bool ISO, bool ACC, typename L>
-__global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
-                                                           const int *__restrict__ keys, const int2 *__restrict__ chunks,
-                                                           int nchunks, double *__restrict__ A, Stage sv)
+template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS>
+__device__ __forceinline__ void reduce_vals(int bid_, int nb_, const int *__restrict__ off,
+        const uint32_t *__restrict__ codes, const int *__restrict__ keys, const int2 *__restrict__ chunks, int nchunks,
+        double *__restrict__ A, Stage sv)
 {
-    constexpr int CAP = flat_cap<L>();
-    static_assert(CAP * 9 * sizeof(L) >= FLAT_KEYS * 9 * sizeof(double), "output staging fits the LDS");
-    constexpr int PER = (CAP + FLAT_KEYS - 1) / FLAT_KEYS;
+    constexpr int CAP = flat_cap<L, KEYS>();
+    static_assert(CAP * 9 * sizeof(L) >= KEYS * 9 * sizeof(double), "output staging fits the LDS");
+    constexpr int PER = (CAP + KEYS - 1) / KEYS;
     __shared__ L lds[CAP * 9];
-    const int ch = xcd_remap(blockIdx.x, gridDim.x);
+    const int ch = xcd_remap(bid_, nb_);
     if (ch >= nchunks) return;
     const int2 c0 = chunks[ch], c1 = chunks[ch + 1];
     const int p0 = c0.x, p1 = c1.x, cb = c0.y, ce = c1.y;
@@ -204,22 +206,30 @@ __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict
     const int p = p0 + threadIdx.x;
     const bool live = p < p1;
     int s = 0, b = 0, e = 0;
+    double acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) acc[i] = 0.;
     if (live) {
         s = keys ? keys[p] : p;
         b = off[s];
         e = off[s + 1];
+        if (ACC) {   // the block's current values, also loaded before the contributions
+            const double *src = A + (size_t)s * 9;
+#pragma unroll
+            for (int i = 0; i < 9; i++) acc[i] = src[i];
+        }
     }
     if (staged) {
         // codes first (coalesced), then the contributions, so their loads overlap
         uint32_t cd[PER];
 #pragma unroll
         for (int r = 0; r < PER; r++) {
-            const int i = threadIdx.x + r * FLAT_KEYS;
+            const int i = threadIdx.x + r * KEYS;
             cd[r] = i < nc ? codes[cb + i] : 0u;
         }
 #pragma unroll
         for (int r = 0; r < PER; r++) {
-            const int i = threadIdx.x + r * FLAT_KEYS;
+            const int i = threadIdx.x + r * KEYS;
             if (i < nc) {
                 L v[9];
                 contrib9<MIXED, ISO, L>(sv, cd[r], v);
@@ -229,15 +239,7 @@ __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict
         }
         __syncthreads();
     }
-    double acc[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) acc[i] = 0.;
     if (live) {
-        if (ACC) {
-            const double *src = A + (size_t)s * 9;
-#pragma unroll
-            for (int i = 0; i < 9; i++) acc[i] = src[i];
-        }
         if (staged) {
             for (int c = b - cb; c < e - cb; c++) {
 #pragma unroll
@@ -271,19 +273,26 @@ __global__ __launch_bounds__(FLAT_KEYS) void k_reduce_vals(const int *__restrict
     __syncthreads();
     const int nout = (p1 - p0) * 9;
     double *dst = A + (size_t)p0 * 9;
-    for (int i = threadIdx.x; i < nout; i += FLAT_KEYS) dst[i] = out[i];
+    for (int i = threadIdx.x; i < nout; i += KEYS) dst[i] = out[i];
+}
+
+template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS>
+__global__ __launch_bounds__(KEYS) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
+                                                           const int *__restrict__ keys, const int2 *__restrict__ chunks,
+                                                           int nchunks, double *__restrict__ A, Stage sv)
+{
+    reduce_vals<MIXED, ISO, ACC, L, KEYS>(blockIdx.x, gridDim.x, off, codes, keys, chunks, nchunks, A, sv);
 }
 
 // Phase 2 for rhs rows (b[row*3+k]) or node forces (dn[node + ns*(F+k)]): one thread per key.
 // TSUM (node forces of the fused call): the sum of the key's contributions from zero is also
 // written to tsum[3 key .. 3 key + 2], from which k_reduce_rhs_tsum starts the rhs rows.
-template <bool ACC, bool TSUM = false>
-__global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off, const uint32_t *__restrict__ codes,
-                                                     const int *__restrict__ keys, int nkeys, double *__restrict__ out,
-                                                     int comp_stride, int elem_stride, Stage sv,
-                                                     double *__restrict__ tsum = nullptr)
+template <bool ACC, bool TSUM>
+__device__ __forceinline__ void reduce_vec3(int bid_, int nb_, const int *__restrict__ off,
+        const uint32_t *__restrict__ codes, const int *__restrict__ keys, int nkeys, double *__restrict__ out,
+        int comp_stride, int elem_stride, Stage sv, double *__restrict__ tsum)
 {
-    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int t = xcd_remap(bid_, nb_) * blockDim.x + threadIdx.x;
     if (t >= nkeys) return;
     const int s = keys ? keys[t] : t;
     const int b = off[s], e = off[s + 1];
@@ -324,6 +333,15 @@ __global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off
     }
 }
 
+template <bool ACC, bool TSUM = false>
+__global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off, const uint32_t *__restrict__ codes,
+                                                     const int *__restrict__ keys, int nkeys, double *__restrict__ out,
+                                                     int comp_stride, int elem_stride, Stage sv,
+                                                     double *__restrict__ tsum = nullptr)
+{
+    reduce_vec3<ACC, TSUM>(blockIdx.x, gridDim.x, off, codes, keys, nkeys, out, comp_stride, elem_stride, sv, tsum);
+}
+
 // For each rhs row r of the fused call: the length of its leading run of tet codes and the
 // node those tets share.  The row's tet contributions must be exactly the force key's list
 // of that node (same codes, same order); any other case sets *err.
@@ -361,12 +379,11 @@ __global__ __launch_bounds__(256) void k_rhs_check(const int *__restrict__ roff,
 
 // rhs rows of the fused call: the node's tet sum (k_reduce_vec3 TSUM), then the row's CZ
 // contributions in order -- the sequential sum over the row's list.
-__global__ __launch_bounds__(256) void k_reduce_rhs_tsum(const int *__restrict__ off, const uint32_t *__restrict__ codes,
-                                                         const int *__restrict__ cz_begin, const int *__restrict__ row_node,
-                                                         const double *__restrict__ tsum, int N, double *__restrict__ out,
-                                                         Stage sv)
+__device__ __forceinline__ void reduce_rhs_tsum(int bid_, int nb_, const int *__restrict__ off,
+        const uint32_t *__restrict__ codes, const int *__restrict__ cz_begin, const int *__restrict__ row_node,
+        const double *__restrict__ tsum, int N, double *__restrict__ out, Stage sv)
 {
-    const int r = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int r = xcd_remap(bid_, nb_) * blockDim.x + threadIdx.x;
     if (r >= N) return;
     const int n = row_node[r];
     double a0 = 0., a1 = 0., a2 = 0.;
@@ -382,6 +399,14 @@ __global__ __launch_bounds__(256) void k_reduce_rhs_tsum(const int *__restrict__
     }
     double *o = out + (size_t)r * 3;
     o[0] = a0; o[1] = a1; o[2] = a2;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_rhs_tsum(const int *__restrict__ off, const uint32_t *__restrict__ codes,
+                                                         const int *__restrict__ cz_begin, const int *__restrict__ row_node,
+                                                         const double *__restrict__ tsum, int N, double *__restrict__ out,
+                                                         Stage sv)
+{
+    reduce_rhs_tsum(blockIdx.x, gridDim.x, off, codes, cz_begin, row_node, tsum, N, out, sv);
 }
 
 // Byte copy, 16 bytes per thread where both ends allow it (icf_d2d; a kernel so that it can
@@ -427,11 +452,11 @@ __global__ __launch_bounds__(256) void k_sum_int(const int *__restrict__ v, int 
 }
 
 // The damaged and failed counts of one CZ pass in one sweep (out[0], out[1]).
-__global__ __launch_bounds__(256) void k_count_flags(const int *__restrict__ damaged, const int *__restrict__ failed,
-                                                     int n, int *__restrict__ out)
+__device__ __forceinline__ void count_flags(int bid_, int nb_, const int *__restrict__ damaged,
+        const int *__restrict__ failed, int n, int *__restrict__ out)
 {
     int d = 0, f = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = bid_ * blockDim.x + threadIdx.x; i < n; i += nb_ * blockDim.x) {
         d += damaged[i];
         f += failed[i];
     }
@@ -441,6 +466,12 @@ __global__ __launch_bounds__(256) void k_count_flags(const int *__restrict__ dam
         if (d) atomicAdd(out, d);
         if (f) atomicAdd(out + 1, f);
     }
+}
+
+__global__ __launch_bounds__(256) void k_count_flags(const int *__restrict__ damaged, const int *__restrict__ failed,
+                                                     int n, int *__restrict__ out)
+{
+    count_flags(blockIdx.x, gridDim.x, damaged, failed, n, out);
 }
 
 }  // namespace icf
