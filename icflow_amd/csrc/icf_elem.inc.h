@@ -122,7 +122,8 @@ __device__ inline void apply_K0(const double g[12], double V, const double y[12]
 // (double, or float when MIXED).
 template <bool MIXED, bool ISO>
 __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restrict__ ie,
-        const double *__restrict__ dn, int n, int es, int ns, double *__restrict__ st_rhs, void *__restrict__ st_tan)
+        const double *__restrict__ dn, int n, int tp, int es, int ns, double *__restrict__ st_rhs,
+        void *__restrict__ st_tan)
 {
     const int nb = nb_;
     const int u = xcd_remap(bid_, nb) * blockDim.x + threadIdx.x;
@@ -217,9 +218,13 @@ __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restr
             if (r % 3 == 2) rhs[r] += gz;
         }
     }
-    double *o = st_rhs + (size_t)u * TET_RHS_REC;
+    // node i's part at plane i (tet_rhs_at), so that each store is one coalesced run
 #pragma unroll
-    for (int i = 0; i < 12; i++) o[i] = rhs[i];
+    for (int i = 0; i < 4; i++) {
+        double *o = st_rhs + ((size_t)i * tp + u) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[k] = rhs[3 * i + k];
+    }
 
     if (ISO) {
         double gt[12];
@@ -228,19 +233,20 @@ __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restr
 #pragma unroll
             for (int k = 0; k < 3; k++)
                 gt[3 * i + k] = R[k * 3 + 0] * g[3 * i] + R[k * 3 + 1] * g[3 * i + 1] + R[k * 3 + 2] * g[3 * i + 2];
-        // record: four (g~_i, V) quadruples, so that phase 2 fetches node i's gradient and the
-        // volume with one 16-byte (FP32) or two 16-byte (FP64) loads
+        // record: four (g~_i, V) quadruples, node i's at plane i, so that phase 2 fetches node
+        // i's gradient and the volume with one 16-byte (FP32) or two 16-byte (FP64) loads and
+        // phase 1 stores one coalesced run per node
         if (MIXED) {
-            float4 *t = (float4 *)st_tan + (size_t)u * 4;
+            float4 *t = (float4 *)st_tan;
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                t[i] = make_float4((float)gt[3 * i], (float)gt[3 * i + 1], (float)gt[3 * i + 2], (float)V);
+                t[(size_t)i * tp + u] = make_float4((float)gt[3 * i], (float)gt[3 * i + 1], (float)gt[3 * i + 2], (float)V);
         } else {
-            double2 *t = (double2 *)st_tan + (size_t)u * 8;
+            double2 *t = (double2 *)st_tan;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                t[2 * i] = make_double2(gt[3 * i], gt[3 * i + 1]);
-                t[2 * i + 1] = make_double2(gt[3 * i + 2], V);
+                t[((size_t)i * tp + u) * 2] = make_double2(gt[3 * i], gt[3 * i + 1]);
+                t[((size_t)i * tp + u) * 2 + 1] = make_double2(gt[3 * i + 2], V);
             }
         }
     } else {
@@ -266,10 +272,10 @@ __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restr
 
 template <bool MIXED, bool ISO>
 __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, const double *__restrict__ dn,
-                                                    int n, int es, int ns, double *__restrict__ st_rhs,
+                                                    int n, int tp, int es, int ns, double *__restrict__ st_rhs,
                                                     void *__restrict__ st_tan)
 {
-    tet_phase1<MIXED, ISO>(blockIdx.x, gridDim.x, ie, dn, n, es, ns, st_rhs, st_tan);
+    tet_phase1<MIXED, ISO>(blockIdx.x, gridDim.x, ie, dn, n, tp, es, ns, st_rhs, st_tan);
 }
 
 // Phase 2: LHS block (i,j) of an isotropic tet record.  LHS = stiff * R K R^T + mcoef V M
@@ -284,13 +290,13 @@ template <>
 struct TetIsoLoad<float> {
     float4 gi, gj;   // (g~, V)
     int ij;
-    __device__ inline void load(const float *__restrict__ base, uint32_t code)
+    __device__ inline void load(const float *__restrict__ base, int tp, uint32_t code)
     {
         const uint32_t q = code_q(code);
-        const float4 *rec = (const float4 *)base + (size_t)code_unit(code) * 4;
+        const float4 *rec = (const float4 *)base + code_unit(code);
         ij = (int)q;
-        gi = rec[q >> 2];
-        gj = rec[q & 3];
+        gi = rec[(size_t)(q >> 2) * tp];
+        gj = rec[(size_t)(q & 3) * tp];
     }
     __device__ inline void values(float w[9], float stiff, float mcoef) const
     {
@@ -313,15 +319,15 @@ template <>
 struct TetIsoLoad<double> {
     double2 i0, i1, j0, j1;   // (g~x, g~y), (g~z, V)
     int ij;
-    __device__ inline void load(const double *__restrict__ base, uint32_t code)
+    __device__ inline void load(const double *__restrict__ base, int tp, uint32_t code)
     {
         const uint32_t q = code_q(code);
-        const double2 *rec = (const double2 *)base + (size_t)code_unit(code) * 8;
+        const double2 *rec = (const double2 *)base + (size_t)code_unit(code) * 2;
         ij = (int)q;
-        i0 = rec[2 * (q >> 2)];
-        i1 = rec[2 * (q >> 2) + 1];
-        j0 = rec[2 * (q & 3)];
-        j1 = rec[2 * (q & 3) + 1];
+        i0 = rec[(size_t)(q >> 2) * tp * 2];
+        i1 = rec[(size_t)(q >> 2) * tp * 2 + 1];
+        j0 = rec[(size_t)(q & 3) * tp * 2];
+        j1 = rec[(size_t)(q & 3) * tp * 2 + 1];
     }
     __device__ inline void values(double w[9], double stiff, double mcoef) const
     {

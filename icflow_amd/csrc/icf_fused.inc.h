@@ -14,7 +14,7 @@ namespace icf {
 // two CZ counters that the last launch of the call adds up.
 template <bool MIXED, bool ISO>
 __global__ __launch_bounds__(256) void k_phase1_fused(int nbt, const int *__restrict__ ie, const double *__restrict__ dn,
-                                                      int nt, int es, int ns, double *__restrict__ tet_rhs,
+                                                      int nt, int tp, int es, int ns, double *__restrict__ tet_rhs,
                                                       void *__restrict__ tet_tan, double *__restrict__ dcz,
                                                       int *__restrict__ icz, int nc, int cs,
                                                       double *__restrict__ cz_rhs, void *__restrict__ cz_tan,
@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void k_phase1_fused(int nbt, const int *__rest
 {
     if (blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = 0;
     if ((int)blockIdx.x < nbt)
-        tet_phase1<MIXED, ISO>(blockIdx.x, nbt, ie, dn, nt, es, ns, tet_rhs, tet_tan);
+        tet_phase1<MIXED, ISO>(blockIdx.x, nbt, ie, dn, nt, tp, es, ns, tet_rhs, tet_tan);
     else
         cz_phase1<MIXED>(blockIdx.x - nbt, gridDim.x - nbt, dcz, icz, dn, nc, cs, ns, cz_rhs, cz_tan);
 }

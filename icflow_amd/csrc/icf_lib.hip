@@ -162,6 +162,7 @@ struct icf_ctx {
     DevBuf bp_f, bp_k, bp_i, bp_l;   // broad phase and NL2set scratch (doubles, keys, ints, int64)
     DevBuf tsum;                     // per-node tet sums of the fused call
     DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
+    int tplane = 64;                 // plane stride of the staged tet records (tet_plane)
     int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     int next_map_id = 0;
@@ -683,21 +684,27 @@ template <typename F> struct OnLane {
 };
 
 // reserve > n: only grow the staging buffers for `reserve` units
+// Plane stride of the tet staging records for n tets.
+int tet_plane(long long n) { return (int)((std::max(n, 1LL) + 63) / 64 * 64); }
+
 int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int ns, int reserve = 0)
 {
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const size_t rec = iso ? TET_ISO_REC : TET_GEN_REC;
-    const size_t cap = (size_t)std::max(std::max(n, reserve), 1);
+    const size_t cap = (size_t)tet_plane(std::max(n, reserve));
     if (int r = grow(ctx, ctx->tet_rhs, sizeof(double) * TET_RHS_REC * cap)) return r;
     if (int r = grow(ctx, ctx->tet_tan, (mixed ? 4 : 8) * rec * cap)) return r;
     if (n == 0) return 0;
+    const int tp = ctx->tplane = tet_plane(n);
     dim3 g(blocks_for(n, 256)), b(256);
     double *sr = (double *)ctx->tet_rhs.p;
     void *st = ctx->tet_tan.p;
-    if (mixed && iso) LAUNCH(ctx, "k_tet_phase1<mixed,iso>", (k_tet_phase1<true, true>), g, b, ie, dn, n, es, ns, sr, st);
-    else if (mixed) LAUNCH(ctx, "k_tet_phase1<mixed,gen>", (k_tet_phase1<true, false>), g, b, ie, dn, n, es, ns, sr, st);
-    else if (iso) LAUNCH(ctx, "k_tet_phase1<fp64,iso>", (k_tet_phase1<false, true>), g, b, ie, dn, n, es, ns, sr, st);
-    else LAUNCH(ctx, "k_tet_phase1<fp64,gen>", (k_tet_phase1<false, false>), g, b, ie, dn, n, es, ns, sr, st);
+#define ICF_T1(MX, IS, NAME) LAUNCH(ctx, NAME, (k_tet_phase1<MX, IS>), g, b, ie, dn, n, tp, es, ns, sr, st)
+    if (mixed && iso) ICF_T1(true, true, "k_tet_phase1<mixed,iso>");
+    else if (mixed) ICF_T1(true, false, "k_tet_phase1<mixed,gen>");
+    else if (iso) ICF_T1(false, true, "k_tet_phase1<fp64,iso>");
+    else ICF_T1(false, false, "k_tet_phase1<fp64,gen>");
+#undef ICF_T1
     return 0;
 }
 
@@ -733,8 +740,10 @@ void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es,
     const dim3 g(std::max(nbt + nbc, 1)), b(256);
     double *tr = (double *)ctx->tet_rhs.p, *cr = (double *)ctx->cz_rhs.p;
     void *tt = ctx->tet_tan.p, *ct = ctx->cz_tan.p;
+    const int tp = ctx->tplane = tet_plane(nt);
 #define ICF_P1(MX, IS, NAME)                                                                                      \
-    LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS>), g, b, nbt, ie, dn, nt, es, ns, tr, tt, dcz, icz, nc, cs, cr, ct, counts)
+    LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS>), g, b, nbt, ie, dn, nt, tp, es, ns, tr, tt, dcz, icz, nc, cs, cr, ct, \
+           counts)
     if (mixed && iso) ICF_P1(true, true, "k_phase1<mixed,iso>");
     else if (mixed) ICF_P1(true, false, "k_phase1<mixed,gen>");
     else if (iso) ICF_P1(false, true, "k_phase1<fp64,iso>");
@@ -794,6 +803,7 @@ Stage make_stage(icf_ctx *ctx, double h)
     Stage sv;
     sv.tet_rhs = (const double *)ctx->tet_rhs.p;
     sv.tet_tan = ctx->tet_tan.p;
+    sv.tplane = ctx->tplane;
     sv.cz_rhs = (const double *)ctx->cz_rhs.p;
     sv.cz_tan = ctx->cz_tan.p;
     sv.ct_tan = ctx->ct.p;

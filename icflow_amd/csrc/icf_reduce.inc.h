@@ -131,8 +131,9 @@ __host__ __device__ constexpr int flat_cap_of(bool fp32, int keys) { return fp32
 
 // Staged per-unit records of the current call.
 struct Stage {
-    const double *tet_rhs;
-    const void *tet_tan;
+    const double *tet_rhs;  // node part i of tet u at (i * tplane + u) * 3
+    const void *tet_tan;    // isotropic: node i's (g~, V) of tet u at record i * tplane + u
+    int tplane;             // tet plane stride (the staged tet count, rounded up to 64)
     const double *cz_rhs;
     const void *cz_tan;
     const void *ct_tan;     // impact tangent blocks (float in mixed mode)
@@ -143,7 +144,7 @@ struct Stage {
 __device__ inline const double *rhs_of(const Stage &sv, uint32_t code)
 {
     const uint32_t kind = code_kind(code), u = code_unit(code), q = code_q(code);
-    if (kind == KIND_TET) return sv.tet_rhs + (size_t)u * TET_RHS_REC + 3 * q;
+    if (kind == KIND_TET) return sv.tet_rhs + ((size_t)q * sv.tplane + u) * 3;
     if (kind == KIND_CZ) return sv.cz_rhs + (size_t)u * CZ_RHS_REC + 3 * q;
     return sv.ct_f + (size_t)u * CT_F_REC + 3 * q;
 }
@@ -158,7 +159,7 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
     const T stiff = (T)sv.stiff, mcoef = (T)sv.mcoef;
     if (kind == KIND_TET && ISO) {
         TetIsoLoad<T> l;
-        l.load((const T *)sv.tet_tan, code);
+        l.load((const T *)sv.tet_tan, sv.tplane, code);
         T w[9];
         l.values(w, stiff, mcoef);
 #pragma unroll
