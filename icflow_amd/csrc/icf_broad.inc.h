@@ -84,67 +84,120 @@ __global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, 
     idx[t] = t;
 }
 
-// Visit the overlapping partners b > a of tet a; count them or write (a << 32 | b) keys.
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_bp_pairs(const double *__restrict__ lo, const double *__restrict__ hi, int nT,
-                                                  const unsigned long long *__restrict__ skey,
-                                                  const int *__restrict__ sidx, const int *__restrict__ ie, int es,
-                                                  int exclude_shared, const double *__restrict__ red,
-                                                  long long *__restrict__ count, const long long *__restrict__ off,
-                                                  unsigned long long *__restrict__ out)
+// The k-DOPs and node ids in cell order: sorted position j holds tet sidx[j], so that the tets
+// of one cell, and of neighbouring cells, are adjacent in memory for k_bp_pairs.  Per position:
+// the k-DOP as (lo, hi) pairs per direction (skd, 24 doubles), the node ids (snode), and the
+// axis-aligned box in FP32 rounded outward, lo down and hi up (box): a prefilter that never
+// rejects a pair whose FP64 boxes overlap.
+__global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ lo, const double *__restrict__ hi,
+                                                   int nT, const int *__restrict__ sidx, const int *__restrict__ ie,
+                                                   int es, double2 *__restrict__ skd, int4 *__restrict__ snode,
+                                                   float4 *__restrict__ box)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nT) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nT) return;
+    const int t = sidx[j];
+    double l[BP_DIRS], h[BP_DIRS];
+#pragma unroll
+    for (int d = 0; d < BP_DIRS; d++) {
+        l[d] = lo[(size_t)d * nT + t];
+        h[d] = hi[(size_t)d * nT + t];
+        skd[(size_t)j * BP_DIRS + d] = make_double2(l[d], h[d]);
+    }
+    snode[j] = make_int4(ie[t], ie[t + (size_t)es], ie[t + 2 * (size_t)es], ie[t + 3 * (size_t)es]);
+    box[2 * (size_t)j] = make_float4(__double2float_rd(l[0]), __double2float_rd(l[1]), __double2float_rd(l[2]), 0.f);
+    box[2 * (size_t)j + 1] = make_float4(__double2float_ru(h[0]), __double2float_ru(h[1]), __double2float_ru(h[2]), 0.f);
+}
+
+// Visit each overlapping pair once, from the partner with the lower sorted position (one
+// thread per sorted position, over the cell-ordered copies); count the pairs, and write
+// (min << 32 | max) keys of the original indices: at off[ja] (WRITE), or, counting, into the
+// thread's own slots out[ja * cap .. ja * cap + cap) when out is given (pairs beyond cap are
+// only counted; the caller then runs the WRITE pass).  The 27 neighbouring cells are 9 rows of
+// three: a row (x - 1 .. x + 1; x is the key's low field) is one key range, so it takes one
+// binary search, and only positions after the thread's own are visited.  A candidate passes
+// the outward-rounded FP32 box prefilter, then (when asked) the shared-node exclusion, which
+// removes most neighbours of a conforming mesh before the FP64 12-direction test (the
+// predicate of the reference's k-DOP overlap).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_bp_pairs(const double2 *__restrict__ skd, int nT,
+                                                  const unsigned long long *__restrict__ skey,
+                                                  const int *__restrict__ sidx, const int4 *__restrict__ snode,
+                                                  const float4 *__restrict__ box, int exclude_shared,
+                                                  const double *__restrict__ red, long long *__restrict__ count,
+                                                  const long long *__restrict__ off,
+                                                  unsigned long long *__restrict__ out, int cap)
+{
+    const int ja = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ja >= nT) return;
+    const int a = sidx[ja];
     const double cs = bp_cell_size(red);
     double la[BP_DIRS], ha[BP_DIRS];
 #pragma unroll
     for (int d = 0; d < BP_DIRS; d++) {
-        la[d] = lo[(size_t)d * nT + a];
-        ha[d] = hi[(size_t)d * nT + a];
+        const double2 v = skd[(size_t)ja * BP_DIRS + d];
+        la[d] = v.x;
+        ha[d] = v.y;
     }
+    const float4 bl = box[2 * (size_t)ja], bh = box[2 * (size_t)ja + 1];
     int ca[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) ca[k] = (int)floor((la[k] - red[1 + k]) / cs);
-    int na[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) na[i] = ie[a + (size_t)es * i];
-    long long pos = WRITE ? off[a] : 0;
+    const int4 na = snode[ja];
+    long long pos = WRITE ? off[ja] : 0;
     int cnt = 0;
     for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int cx = ca[0] + dx, cy = ca[1] + dy, cz = ca[2] + dz;
-                if (cx < 0 || cy < 0 || cz < 0) continue;
-                const unsigned long long key = (unsigned long long)cx | ((unsigned long long)cy << BP_CELL_BITS) |
-                                               ((unsigned long long)cz << (2 * BP_CELL_BITS));
-                int lo_i = 0, hi_i = nT;
-                while (lo_i < hi_i) {
-                    const int mid = (lo_i + hi_i) >> 1;
-                    if (skey[mid] < key) lo_i = mid + 1;
-                    else hi_i = mid;
-                }
-                for (int j = lo_i; j < nT && skey[j] == key; j++) {
-                    const int b = sidx[j];
-                    if (b <= a) continue;
-                    bool ov = true;
-#pragma unroll
-                    for (int d = 0; d < BP_DIRS; d++)
-                        ov = ov && !(la[d] > hi[(size_t)d * nT + b]) && !(ha[d] < lo[(size_t)d * nT + b]);
-                    if (!ov) continue;
-                    if (exclude_shared) {
-                        bool sh = false;
-#pragma unroll
-                        for (int j2 = 0; j2 < 4; j2++) {
-                            const int nb = ie[b + (size_t)es * j2];
-                            sh = sh || nb == na[0] || nb == na[1] || nb == na[2] || nb == na[3];
-                        }
-                        if (sh) continue;
-                    }
-                    if (WRITE) out[pos + cnt] = ((unsigned long long)a << 32) | (unsigned)b;
-                    cnt++;
-                }
+        for (int dy = -1; dy <= 1; dy++) {
+            const int cy = ca[1] + dy, cz = ca[2] + dz;
+            if (cy < 0 || cz < 0) continue;
+            const unsigned long long row = ((unsigned long long)cy << BP_CELL_BITS) |
+                                           ((unsigned long long)cz << (2 * BP_CELL_BITS));
+            const unsigned long long k0 = row | (unsigned long long)(ca[0] > 0 ? ca[0] - 1 : 0);
+            const unsigned long long k1 = row | (unsigned long long)(ca[0] + 1);
+            int lo_i = 0, hi_i = nT;
+            while (lo_i < hi_i) {
+                const int mid = (lo_i + hi_i) >> 1;
+                if (skey[mid] < k0) lo_i = mid + 1;
+                else hi_i = mid;
             }
-    if (!WRITE) count[a] = cnt;
+            for (int j = lo_i > ja ? lo_i : ja + 1; j < nT && skey[j] <= k1; j++) {
+                const float4 ol = box[2 * (size_t)j], oh = box[2 * (size_t)j + 1];
+                if (bl.x > oh.x || bl.y > oh.y || bl.z > oh.z || bh.x < ol.x || bh.y < ol.y || bh.z < ol.z) continue;
+                if (exclude_shared) {
+                    const int4 nb = snode[j];
+                    const int m[4] = {nb.x, nb.y, nb.z, nb.w};
+                    bool sh = false;
+#pragma unroll
+                    for (int j2 = 0; j2 < 4; j2++) sh = sh || m[j2] == na.x || m[j2] == na.y || m[j2] == na.z || m[j2] == na.w;
+                    if (sh) continue;
+                }
+                bool ov = true;
+#pragma unroll
+                for (int d = 0; d < BP_DIRS; d++) {
+                    const double2 v = skd[(size_t)j * BP_DIRS + d];
+                    ov = ov && !(la[d] > v.y) && !(ha[d] < v.x);
+                }
+                if (!ov) continue;
+                if (WRITE || (out && cnt < cap)) {
+                    const unsigned b = (unsigned)sidx[j];
+                    const unsigned p = (unsigned)a < b ? (unsigned)a : b, q = (unsigned)a < b ? b : (unsigned)a;
+                    out[WRITE ? pos + cnt : (long long)ja * cap + cnt] = ((unsigned long long)p << 32) | q;
+                }
+                cnt++;
+            }
+        }
+    if (!WRITE) count[ja] = cnt;
+}
+
+// The counting pass's slot pairs, packed at their offsets (threads whose pairs all fit).
+__global__ __launch_bounds__(256) void k_bp_pack(const unsigned long long *__restrict__ slots, int cap,
+                                                 const long long *__restrict__ count, const long long *__restrict__ off,
+                                                 int nT, unsigned long long *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ja = (int)(i / cap), k = (int)(i % cap);
+    if (ja >= nT || k >= count[ja]) return;
+    out[off[ja] + k] = slots[i];
 }
 
 __global__ __launch_bounds__(256) void k_bp_unpack(const unsigned long long *__restrict__ keys, long long n,

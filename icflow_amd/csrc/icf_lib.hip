@@ -1696,15 +1696,18 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     *nOut = 0;
     if (nTets == 0) return 0;
     const int nT = nTets;
-    // scratch: lo, hi (12 nT each), ext (nT), mn (3 nT), red (4)
-    if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)28 * nT + 4))) return r;
+    // scratch: lo, hi (12 nT each), ext (nT), mn (3 nT), red (4), cell-ordered lo, hi (12 nT each)
+    if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)60 * nT + 4))) return r;
     if (int r = grow(ctx, ctx->bp_k, sizeof(unsigned long long) * 2 * (size_t)nT)) return r;
-    if (int r = grow(ctx, ctx->bp_i, sizeof(int) * 2 * (size_t)nT)) return r;
-    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2 * ((size_t)nT + 1))) return r;
+    if (int r = grow(ctx, ctx->bp_i, sizeof(int) * (6 * (size_t)nT + 4))) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * (2 * ((size_t)nT + 1) + 1))) return r;
     double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT, *ext = hi + (size_t)12 * nT,
            *mn = ext + nT, *red = mn + (size_t)3 * nT;
+    double2 *skd = (double2 *)(red + 4);                  // 12 double2 = 24 doubles per tet
+    float4 *box = (float4 *)(red + 4 + (size_t)24 * nT);  // 2 float4 = 4 doubles per tet
     auto *key = (unsigned long long *)ctx->bp_k.p, *skey = key + nT;
     int *idx = (int *)ctx->bp_i.p, *sidx = idx + nT;
+    int4 *snode = (int4 *)(sidx + nT + ((4 - ((2 * nT) & 3)) & 3));   // 16-byte aligned, 4 ints per tet
     long long *cnt = (long long *)ctx->bp_l.p, *off = cnt + nT + 1;
     int *d_err = ctx->small + 1;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
@@ -1716,7 +1719,9 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b3, key, skey, idx, sidx, nT, 0, 3 * BP_CELL_BITS,
                                                    ctx->stream));
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, cnt, off, nT + 1, ctx->stream));
-    size_t tb = std::max(std::max(b1, b2), std::max(b3, b4));
+    size_t b6 = 0;
+    HIPCHK(ctx, hipcub::DeviceReduce::Max(nullptr, b6, cnt, off + nT + 1, nT, ctx->stream));
+    size_t tb = std::max(std::max(std::max(b1, b2), std::max(b3, b4)), b6);
     if (int r = grow(ctx, ctx->scan_tmp, tb)) return r;
     HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b1, ext, red, nT, ctx->stream));
     for (int k = 0; k < 3; k++)
@@ -1725,13 +1730,22 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, b3, key, skey, idx, sidx, nT, 0,
                                                    3 * BP_CELL_BITS, ctx->stream));
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)(cnt + nT), 2, 0);
-    hipLaunchKernelGGL((k_bp_pairs<false>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, skey,
-                       sidx, ie, el_all_stride, exclude_shared, red, cnt, nullptr, nullptr);
+    hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, sidx, ie,
+                       el_all_stride, skd, snode, box);
+    // counting pass; each thread also keeps its first BP_SLOTS pairs, so that one pass suffices
+    // when no thread has more
+    constexpr int BP_SLOTS = 64;
+    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * BP_SLOTS * (size_t)nT)) return r;
+    auto *slots = (unsigned long long *)ctx->pat_a.p;
+    hipLaunchKernelGGL((k_bp_pairs<false>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skd, nT, skey, sidx,
+                       snode, box, exclude_shared, red, cnt, nullptr, slots, BP_SLOTS);
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b4, cnt, off, nT + 1, ctx->stream));
-    long long total = 0;
+    HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b6, cnt, off + nT + 1, nT, ctx->stream));
+    long long total = 0, most = 0;
     int err = 0;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(&total, off + nT, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&most, off + nT + 1, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
@@ -1739,11 +1753,19 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     if (total > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
     if (total == 0) return 0;
     if (total >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 pairs");
-    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)total)) return r;
     if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)total)) return r;
     auto *pa = (unsigned long long *)ctx->pat_a.p, *pb = (unsigned long long *)ctx->pat_b.p;
-    hipLaunchKernelGGL((k_bp_pairs<true>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, skey,
-                       sidx, ie, el_all_stride, exclude_shared, red, nullptr, off, pa);
+    if (most <= BP_SLOTS) {
+        // every pair is in a slot: pack them at their offsets (into pb), then sort into pa
+        hipLaunchKernelGGL(k_bp_pack, dim3(blocks_for((long long)nT * BP_SLOTS, 256)), dim3(256), 0, ctx->stream, pa,
+                           BP_SLOTS, cnt, off, nT, pb);
+        std::swap(pa, pb);
+    } else {
+        if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)total)) return r;
+        pa = (unsigned long long *)ctx->pat_a.p;
+        hipLaunchKernelGGL((k_bp_pairs<true>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skd, nT, skey,
+                           sidx, snode, box, exclude_shared, red, nullptr, off, pa, 0);
+    }
     size_t b5 = 0;
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, b5, pa, pb, (int)total, 0, 64, ctx->stream));
     if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;

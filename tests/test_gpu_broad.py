@@ -59,3 +59,42 @@ def test_device_broad_phase_and_nl2set_match_host(specimen_consts, exclude_share
     r = lib.icf_nl2set(ctx, d_nl.p, d_pairs.p, Q, gf.g_ie.p, gf.el_all_stride, small.p, 1, ctypes.byref(nd))
     assert r == _lib.ICF_E_CAPACITY and nd.value == nh.value
     gf.close()
+
+
+def test_device_broad_phase_crowded_cells():
+    """100 tets on top of one another (distinct node ids) and 20 apart: every one of the 100
+    overlaps every other, so some tets have more partners than the counting pass keeps per
+    thread, which takes the second (writing) pass.  Same pairs, same order as the host grid."""
+    from icflow_amd.gpu_functionality import GPUFunctionality
+    rng = np.random.default_rng(3)
+    unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.01
+    tets = [unit + 1e-6 * k for k in range(100)]
+    tets += [unit + np.array([0.05 * (k + 1), 0.0, 0.0]) + 1e-4 * rng.random((4, 3)) for k in range(20)]
+    nT = len(tets)
+    nN = 4 * nT
+    ns, es = stride_of(nN), stride_of(nT)
+    c_dn = np.zeros(18 * ns)
+    for t, x in enumerate(tets):
+        for i in range(4):
+            for k in range(3):
+                c_dn[4 * t + i + ns * (15 + k)] = x[i, k]
+    c_ie = np.zeros(5 * es, np.int32)
+    for t in range(nT):
+        for i in range(4):
+            c_ie[t + es * i] = 4 * t + i
+    gf = GPUFunctionality(0)
+    lib, ctx = gf.lib, gf.ctx
+    d_dn, d_ie = gf.buf("cr_dn", c_dn.size, np.float64), gf.buf("cr_ie", c_ie.size, np.int32)
+    d_dn.upload(c_dn)
+    d_ie.upload(c_ie)
+    for excl in (0, 1):
+        host = broad_phase(gf, c_dn, ns, c_ie, es, nT, exclude_shared=bool(excl))
+        Q = len(host)
+        assert Q >= 100 * 99 // 2
+        d_pairs = gf.buf(f"cr_pairs{excl}", 2 * Q, np.int32)
+        n = ctypes.c_longlong()
+        _lib.check(lib.icf_broad_phase(ctx, d_dn.p, ns, d_ie.p, es, nT, excl, d_pairs.p, Q, ctypes.byref(n)), ctx,
+                   "broad_phase")
+        assert n.value == Q
+        assert np.array_equal(d_pairs.download(2 * Q).reshape(-1, 2), host)
+    gf.close()
