@@ -125,6 +125,27 @@ __device__ inline double point_tri_dsq(const double f[9], const double p[3])
     return q0 * q0 + q1 * q1 + q2 * q2;
 }
 
+// Compact copy of the surface tets' adjacent-face lists (ie row 4 = count, rows 5.. = face
+// ids; one SoA row per list position): tet t's list at ids[off[t] .. off[t+1]).  The closest
+// face search then reads a tuple's list from one place instead of one row per face.
+__global__ __launch_bounds__(256) void k_adj_count(const int *__restrict__ ie, int es, int max_rows,
+                                                   int *__restrict__ cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= es) return;
+    const int n = ie[t + (size_t)es * 4];
+    cnt[t] = n < 0 ? 0 : (n > max_rows ? max_rows : n);
+}
+
+__global__ __launch_bounds__(256) void k_adj_fill(const int *__restrict__ ie, int es, const int *__restrict__ off,
+                                                  int *__restrict__ ids)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= es) return;
+    const int o = off[t], n = off[t + 1] - o;
+    for (int k = 0; k < n; k++) ids[o + k] = ie[t + (size_t)es * (5 + k)];
+}
+
 // (node, tet) tuples (narrowphase.h:159-201), CF_GROUP lanes per tuple: lane l evaluates the
 // adjacent faces k = l, l + CF_GROUP, ...; the group then takes the lexicographic minimum of
 // (distance, k).  That is the reference's sequential "first strict minimum" rule: a NaN
@@ -134,8 +155,10 @@ __device__ inline double point_tri_dsq(const double f[9], const double p[3])
 constexpr int CF_GROUP = 8;
 
 __global__ __launch_bounds__(256) void k_closest_face(int nPairs, int ts, int *__restrict__ nl2,
-                                                      const double *__restrict__ dn, const int *__restrict__ ie,
-                                                      const int *__restrict__ faces, int ns, int es, int fs)
+                                                      const double *__restrict__ dn,
+                                                      const int *__restrict__ adj_off,
+                                                      const int *__restrict__ adj_ids,
+                                                      const int *__restrict__ faces, int ns, int fs)
 {
     const int u = (blockIdx.x * blockDim.x + threadIdx.x) / CF_GROUP;
     const int lane = threadIdx.x % CF_GROUP;
@@ -147,9 +170,9 @@ __global__ __launch_bounds__(256) void k_closest_face(int nPairs, int ts, int *_
         double p[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) p[k] = dn[node + ns * (ND_XC + k)];
-        const int nf = ie[tet + es * 4];
+        const int a0 = adj_off[tet], nf = adj_off[tet + 1] - a0;
         for (int k = lane; k < nf; k += CF_GROUP) {
-            const int fid = ie[tet + es * (5 + k)];
+            const int fid = adj_ids[a0 + k];
             double fc[9];
 #pragma unroll
             for (int v = 0; v < 3; v++) {

@@ -163,6 +163,15 @@ struct icf_ctx {
     DevBuf tsum;                     // per-node tet sums of the fused call
     DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
     int tplane = 64;                 // plane stride of the staged tet records (tet_plane)
+    // compact adjacent-face lists of the surface tets (k_adj_*), built from `ie` on first use and
+    // dropped when a library write or free touches that allocation (or icf_pattern_changed)
+    struct Adj {
+        const int *ie = nullptr;
+        int es = 0;
+        uintptr_t lo = 0, hi = 0;    // the allocation holding ie
+        DevBuf off, ids;
+        bool valid = false;
+    } adj;
     int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     int next_map_id = 0;
@@ -238,6 +247,7 @@ void free_map(SlotMap &m)
 // marked suspect, to be compared against their snapshot before the next use.
 void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi, bool erase = false)
 {
+    if (ctx->adj.valid && ctx->adj.lo < hi && lo < ctx->adj.hi) ctx->adj.valid = false;
     for (auto it = ctx->maps.begin(); it != ctx->maps.end();) {
         bool hit = false;
         for (const SrcKey &s : it->srcs) {
@@ -929,7 +939,7 @@ int icf_destroy(icf_ctx *ctx)
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
                       &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l,
-                      &ctx->tsum,    &ctx->ct_f};
+                      &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->small) (void)hipFree(ctx->small);
@@ -1121,6 +1131,7 @@ int icf_pattern_changed(icf_ctx *ctx)
     if (int r = check_ctx(ctx)) return r;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_pattern_changed during capture");
     for (SlotMap &m : ctx->maps) m.suspect = true;
+    ctx->adj.valid = false;
     return 0;
 }
 
@@ -1337,6 +1348,44 @@ int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, 
     return tm.finish();
 }
 
+// The compact adjacency lists of ie (valid on return), built synchronously when missing.
+int get_adjacency(icf_ctx *ctx, const int *ie, int es)
+{
+    icf_ctx::Adj &a = ctx->adj;
+    if (a.valid && a.ie == ie && a.es == es) return 0;
+    if (ctx->capturing)
+        return fail(ctx, ICF_E_CAPTURE, "the adjacency lists must be built: run the call once uncaptured");
+    void *base = nullptr;
+    size_t sz = 0;
+    HIPCHK(ctx, hipMemGetAddressRange(&base, &sz, const_cast<int *>(ie)));
+    const uintptr_t end = (uintptr_t)base + sz;
+    const long long rows = ((long long)(end - (uintptr_t)ie)) / (4LL * std::max(es, 1));
+    if (rows < 5) return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: ie holds fewer than 5 rows");
+    const int max_rows = (int)std::min(rows - 5, 1LL << 20);
+    if (int r = lane_barrier(ctx)) return r;
+    if (int r = grow(ctx, a.off, sizeof(int) * ((size_t)es + 1))) return r;
+    int *off = (int *)a.off.p;
+    size_t tb = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, off, off, es + 1, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, tb)) return r;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, off + es, 1, 0);
+    hipLaunchKernelGGL(k_adj_count, dim3(blocks_for(es, 256)), dim3(256), 0, ctx->stream, ie, es, max_rows, off);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tb, off, off, es + 1, ctx->stream));
+    int total = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&total, off + es, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int r = grow(ctx, a.ids, sizeof(int) * (size_t)std::max(total, 1))) return r;
+    hipLaunchKernelGGL(k_adj_fill, dim3(blocks_for(es, 256)), dim3(256), 0, ctx->stream, ie, es, off, (int *)a.ids.p);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    a.ie = ie;
+    a.es = es;
+    a.lo = (uintptr_t)base;
+    a.hi = end;
+    a.valid = true;
+    return 0;
+}
+
 int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, const double *dn, const int *ie,
                           const int *faces, int nd_stride, int el_all_stride, int fc_stride)
 {
@@ -1345,18 +1394,21 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
         (nPairs && (!nl2 || !dn || !ie || !faces)))
         return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: invalid argument");
     if (nPairs == 0) return 0;
-    // the adjacency rows of ie are as many as the mesh needs: the whole allocation is read
+    // the surface tets' adjacency lists, read from their compact copy
+    if (int r = get_adjacency(ctx, ie, el_all_stride)) return r;
     OnLane ln(ctx, contact_lane(ctx), [&] {
         return Access()
             .wr(nl2, 4 * (size_t)tet_stride)
             .rd(dn + (size_t)nd_stride * ND_XC, 3 * (size_t)nd_stride)
-            .alloc(ie, false)
+            .rd((const char *)ctx->adj.off.p, ctx->adj.off.bytes)
+            .rd((const char *)ctx->adj.ids.p, ctx->adj.ids.bytes)
             .rd(faces, 3 * (size_t)fc_stride);
     });
     if (ln.err) return ln.err;
     Timer tm(ctx);
-    LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256), nPairs, tet_stride,
-           nl2, dn, ie, faces, nd_stride, el_all_stride, fc_stride);
+    LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256),
+           nPairs, tet_stride, nl2, dn, (const int *)ctx->adj.off.p, (const int *)ctx->adj.ids.p, faces, nd_stride,
+           fc_stride);
     if (int r = ln.done()) return r;
     return tm.finish();
 }

@@ -276,3 +276,51 @@ def test_mixed_precision_with_contact(orc, specimen_consts):
     errs = _compare(got, ref, tol_vals=1e-5, tol_rhs=TOL64)
     assert errs["vals"] > 0
     p.gf.close()
+
+
+def test_closest_face_follows_adjacency_changes(orc, specimen_consts):
+    """The closest-face search reads a compact copy of the surface tets' adjacent-face lists,
+    built on first use.  Re-uploading different lists through icf_h2d must drop that copy:
+    with every list reversed (the first strict minimum then picks the other face of a tie)
+    and one face removed per list, the device still matches the oracle on the new lists."""
+    col = small_specimen(cells=(5, 4, 4), size=(0.05, 0.04, 0.04), grains=5, seed=4)
+    st = make_state(col, specimen_consts, seed=5, platen_dz_frac=0.3)
+    p = Pipeline(col, specimen_consts, st, contact=True)
+    gf, lib, ctx = p.gf, p.gf.lib, p.gf.ctx
+    nT, ts, es = gf.nImpacts, gf.tet_stride, gf.el_all_stride
+    assert nT > 0
+    tuples = np.zeros(4 * ts, np.int32)
+    tuples[:2 * ts] = gf.c_itet[:2 * ts]
+    d_nl2 = gf.buf("cf_nl2", 4 * ts, np.int32)
+
+    def device(ie_host):
+        d_nl2.upload(tuples)
+        _lib.check(lib.icf_find_closest_face(ctx, nT, ts, d_nl2.p, gf.g_dn.p, gf.g_ie.p, gf.g_ifc.p, gf.nd_stride, es,
+                                             gf.fc_stride), ctx, "closest")
+        return d_nl2.download(4 * ts)
+
+    def oracle(ie_host):
+        out = tuples.copy()
+        orc.find_closest_face(nT, ts, out, gf.c_dn, ie_host, gf.c_ifc, gf.nd_stride, es, gf.fc_stride)
+        return out
+
+    first = device(gf.c_ie)
+    assert np.array_equal(first[:4 * ts], oracle(gf.c_ie)[:4 * ts])
+    ie2 = gf.c_ie.copy()
+    nS = len(col.surf)
+    for t in range(nS):
+        n = ie2[t + es * 4]
+        lst = [ie2[t + es * (5 + k)] for k in range(n)][::-1]
+        if n > 1:
+            lst = lst[1:]
+        ie2[t + es * 4] = len(lst)
+        for k, f in enumerate(lst):
+            ie2[t + es * (5 + k)] = f
+    gf.g_ie.upload(ie2)
+    second = device(ie2)
+    ref2 = oracle(ie2)
+    for k in range(4):
+        assert np.array_equal(second[k * ts:k * ts + nT], ref2[k * ts:k * ts + nT]), k
+    # the changed lists change the answer, so a stale copy would have failed the check above
+    assert not np.array_equal(ref2[ts:4 * ts], oracle(gf.c_ie)[ts:4 * ts])
+    gf.close()
