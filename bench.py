@@ -223,6 +223,29 @@ def timed(runner, steps, warmup, how):
     return dt / steps, runner.profile()
 
 
+def autotune(r, rounds, name):
+    """Blocking iterations under icf_set_autotune(rounds) until the key limits of this mode's
+    value reductions are chosen (set-up, before any timed step)."""
+    gf = r.gf
+    stage = "fp32" if name == "mixed" else "fp64"
+    roles = [f"overwrite,{stage}"] + ([f"accumulate,{stage}"] if r.contact else [])
+    gf.set_autotune(rounds)
+    r.lib.icf_set_blocking(gf.ctx, 1)
+    n = 0
+    while n < 4 * rounds * len(gf.KEY_CANDIDATES) + 2:
+        r.step()
+        n += 1
+        state = gf.autotune_state()
+        if not any(state[k][1] for k in roles):
+            break
+    gf.set_autotune(0)
+    state = gf.autotune_state()
+    got = {k: dict(keys=state[k][0], best_us={c: round(1e3 * ms, 1) for c, ms in state[k][2].items()})
+           for k in roles}
+    log(f"autotune {name} ({n} iterations): {got}")
+    return got
+
+
 def algorithmic_bytes(r):
     """Compulsory HBM bytes per launch of each kernel (every byte counted once; staging records
     and slot maps are implementation traffic and are not counted), keyed by the profile name
@@ -405,6 +428,8 @@ def main():
                          "launches, no second stream)")
     ap.add_argument("--timings", default="dropin,async,graph,profile",
                     help="ways of issuing the timed steps (profile is always run: it gives the kernel times)")
+    ap.add_argument("--tune-rounds", type=int, default=3,
+                    help="icf_set_autotune rounds before timing each mode (0: keep the default key limit)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run icf_d2d over this many MiB (a plain copy of known size, for "
                          "calibrating hardware byte counters)")
@@ -441,11 +466,14 @@ def main():
     log(f"mixed vs fp64 normwise rel err: {errs}")
 
     out = {}
+    tuned = {}
     modes = [m for m in args.modes.split(",") if m]
     for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
         if name not in modes:
             continue
         r.set_mode(mode)
+        if args.tune_rounds > 0:
+            tuned[name] = autotune(r, args.tune_rounds, name)
         log(f"timing {name}: {args.warmup} warmup + {args.steps} steps each way")
         res = {how: timed(r, args.steps, args.warmup, how) for how in hows}
         out[name] = res
@@ -538,6 +566,7 @@ def main():
                                "frac": round(it_bytes / sec / 1e9 / HBM_PEAK_GBS, 4)},
         "mixed_rel_err_vs_fp64": errs,
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
+        "autotune": tuned or None,
         "roofline": roof,
         "device_broad_phase": bp,
         "precision_study": study,

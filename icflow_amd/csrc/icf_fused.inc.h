@@ -28,7 +28,8 @@ __global__ __launch_bounds__(256) void k_phase1_fused(int nbt, const int *__rest
 }
 
 // Node forces with each node's tet sum (k_reduce_vec3<true, true>; workgroups [0, nbf), the
-// short phase first), then the BCSR values of the fused call (overwrite).
+// short phase first), then the BCSR values of the fused call (overwrite).  Workgroups of KEYS
+// lanes: one key per lane in both phases.
 template <bool MIXED, bool ISO, typename L, int KEYS>
 __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *__restrict__ off,
                                                            const uint32_t *__restrict__ codes,
@@ -39,7 +40,6 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *_
                                                            double *__restrict__ F, int ns, Stage sv,
                                                            double *__restrict__ tsum)
 {
-    static_assert(KEYS == 256, "the force phase runs 256-lane workgroups");
     if ((int)blockIdx.x < nbf)
         reduce_vec3<true, true>(blockIdx.x, nbf, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, tsum);
     else
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_rhs_counts_fused(int nbr, const int *__
 }
 
 // The impact sums of the collision response, added into node forces (workgroups [0, nb0)),
-// BCSR blocks (the next nb1) and rhs rows (the rest).
+// BCSR blocks (the next nb1) and rhs rows (the rest); KEYS lanes per workgroup, one key each.
 template <bool MIXED, typename L, int KEYS>
 __global__ __launch_bounds__(KEYS) void k_contact_sums(int nb0, int nb1, const int *__restrict__ foff,
                                                        const uint32_t *__restrict__ fcodes,
@@ -79,7 +79,6 @@ __global__ __launch_bounds__(KEYS) void k_contact_sums(int nb0, int nb1, const i
                                                        const int *__restrict__ rkeys, int nrkeys,
                                                        double *__restrict__ b, Stage sv)
 {
-    static_assert(KEYS == 256, "the vector phases run 256-lane workgroups");
     const int w = blockIdx.x;
     if (w < nb0)
         reduce_vec3<true, false>(w, nb0, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, nullptr);

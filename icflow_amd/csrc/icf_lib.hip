@@ -120,9 +120,10 @@ struct SlotMap {
     uint32_t *codes = nullptr;     // ncodes
     int *keys = nullptr;           // accumulate mode: the keys with contributions (null: all keys)
     int nlaunch = 0;               // keys a reduction visits
-    int2 *chunks[2] = {nullptr, nullptr};  // BCSR-value maps, per staging type (0: FP32 values, 1:
-    int nchunks[2] = {0, 0};               // FP64 values): chunk starts (list position, code position)
-    int chunk_keys = 0;                    // key limit of the chunks (one thread per key)
+    // BCSR-value maps, per candidate key limit KEY_CANDS[c] and staging type (0: FP32 values,
+    // 1: FP64 values): chunk starts (list position, code position)
+    int2 *chunks[NKC][2] = {};
+    int nchunks[NKC][2] = {};
     int id = 0;                    // unique per map built
     // rhs maps of the fused call: the force map (by id) whose per-node tet sums seed the rows,
     // the outcome of that check, and per row the node and the first non-tet code
@@ -180,6 +181,18 @@ struct icf_ctx {
     bool blocking = true;          // icf_set_blocking
     bool capturing = false;        // between icf_graph_begin and icf_graph_end
     bool overlap = true;           // icf_set_overlap: independent kernels on more than one stream
+    // key limit (workgroup size) of the staged BCSR-value reductions, per role
+    // [accumulate map][FP64 staging]: the candidate in use and, while tuning, the timings
+    struct KeyTune {
+        int sel = KC_DEFAULT;      // index into KEY_CANDS
+        bool active = false;       // timing the candidates in turn
+        int turn = 0;              // launches started while tuning
+        int samples = 0;           // launches timed
+        int cand = -1;             // candidate of the launch being timed, -1: none
+        float best[NKC] = {};      // fastest launch per candidate (ms), 0: not timed
+        hipEvent_t ev[2] = {nullptr, nullptr};
+    } tune[2][2];
+    int tune_rounds = 0;           // timed launches per candidate (icf_set_autotune)
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
     struct Pending {
@@ -232,11 +245,10 @@ void free_map(SlotMap &m)
     if (m.keys) (void)hipFree(m.keys);
     if (m.row_node) (void)hipFree(m.row_node);
     if (m.cz_begin) (void)hipFree(m.cz_begin);
-    for (int k = 0; k < 2; k++)
-        if (m.chunks[k]) (void)hipFree(m.chunks[k]);
+    if (m.chunks[0][0]) (void)hipFree(m.chunks[0][0]);   // the start of the one allocation
+    for (int c = 0; c < NKC; c++) m.chunks[c][0] = m.chunks[c][1] = nullptr;
     if (m.snap) (void)hipFree(m.snap);
     m.snap = nullptr;
-    m.chunks[0] = m.chunks[1] = nullptr;
     m.off = nullptr;
     m.codes = nullptr;
     m.keys = nullptr;
@@ -448,10 +460,9 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
     m.keys = list;                 // null when overwriting: every key, in order
     if (target == TGT_VALS && nlist > 0) {
         // chunks of consecutive list positions for the staged reduction of BCSR values, one
-        // table per staging type: at most chunk_keys keys and flat_cap_of(FP32?, chunk_keys)
-        // contributions each (a single key with more contributions gets a chunk of its own,
-        // reduced without staging)
-        m.chunk_keys = list ? FLAT_KEYS_ACC : FLAT_KEYS_OW;
+        // table per candidate key limit and staging type: at most `keys` keys and
+        // flat_cap_of(FP32?, keys) contributions each (a single key with more contributions gets
+        // a chunk of its own, reduced without staging)
         std::vector<int> h_off((size_t)nkeys + 1), h_list(list ? (size_t)nlist : 0);
         HIPCHK(ctx, hipMemcpyAsync(h_off.data(), m.off, sizeof(int) * ((size_t)nkeys + 1), hipMemcpyDeviceToHost,
                                    ctx->stream));
@@ -462,14 +473,18 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         // a chunk's codes are contiguous (keys in between, if any, have none): each entry is the
         // chunk's first list position and first code position; the last entry closes the table
         auto key_of = [&](int p) { return list ? h_list[p] : p; };
-        for (int kind = 0; kind < 2; kind++) {
-            const int cap = flat_cap_of(kind == 0, m.chunk_keys);
-            std::vector<int2> ch(1, make_int2(0, h_off[key_of(0)]));
+        std::vector<int2> ch;          // every table, one after the other, in one allocation
+        size_t at[2 * NKC];
+        for (int kc = 0; kc < 2 * NKC; kc++) {
+            const int kind = kc & 1, keys = KEY_CANDS[kc >> 1];
+            const int cap = flat_cap_of(kind == 0, keys);
+            at[kc] = ch.size();
+            ch.push_back(make_int2(0, h_off[key_of(0)]));
             int cnt = 0, nk = 0;
             for (int p = 0; p < nlist; p++) {
                 const int k = key_of(p);
                 const int c = h_off[k + 1] - h_off[k];
-                if (nk > 0 && (nk == m.chunk_keys || cnt + c > cap)) {
+                if (nk > 0 && (nk == keys || cnt + c > cap)) {
                     ch.push_back(make_int2(p, h_off[k]));
                     cnt = 0;
                     nk = 0;
@@ -478,12 +493,12 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
                 nk++;
             }
             ch.push_back(make_int2(nlist, h_off[key_of(nlist - 1) + 1]));
-            m.nchunks[kind] = (int)ch.size() - 1;
-            HIPCHK(ctx, hipMalloc(&m.chunks[kind], sizeof(int2) * ch.size()));
-            HIPCHK(ctx, hipMemcpyAsync(m.chunks[kind], ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice,
-                                       ctx->stream));
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            m.nchunks[kc >> 1][kind] = (int)(ch.size() - at[kc]) - 1;
         }
+        int2 *d = nullptr;
+        HIPCHK(ctx, hipMalloc(&d, sizeof(int2) * ch.size()));
+        for (int kc = 0; kc < 2 * NKC; kc++) m.chunks[kc >> 1][kc & 1] = d + at[kc];
+        HIPCHK(ctx, hipMemcpyAsync(d, ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
     m.nlaunch = nlist;
@@ -761,50 +776,115 @@ void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es,
 #undef ICF_P1
 }
 
+// ---- autotuner of the value reductions' key limit --------------------------------------------
+// While a role is being tuned, its launches cycle through the candidate key limits, each timed
+// with a pair of events around the launch; the next launch of the role reads the previous
+// timing (waiting for it if it has not finished), and after tune_rounds timings per candidate
+// the fastest launch decides.  Captured calls are not timed; they use the current choice.
+
+void tune_fold(icf_ctx *ctx, icf_ctx::KeyTune &t)
+{
+    if (t.cand >= 0) {
+        float ms = 0;
+        if (hipEventSynchronize(t.ev[1]) == hipSuccess && hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) == hipSuccess &&
+            ms > 0) {
+            if (t.best[t.cand] == 0 || ms < t.best[t.cand]) t.best[t.cand] = ms;
+            t.samples++;
+        }
+        t.cand = -1;
+    }
+    if (t.active && (t.samples >= ctx->tune_rounds * NKC || t.turn >= 4 * ctx->tune_rounds * NKC)) {
+        t.active = false;
+        for (int c = 0; c < NKC; c++)
+            if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
+    }
+}
+
+// The candidate for the next value reduction of role (acc, fp64); starts its timing if tuning.
+int tune_begin(icf_ctx *ctx, bool acc, bool fp64)
+{
+    icf_ctx::KeyTune &t = ctx->tune[acc][fp64];
+    if (!t.active || ctx->capturing) return t.sel;
+    tune_fold(ctx, t);
+    if (!t.active) return t.sel;
+    t.cand = t.turn++ % NKC;
+    (void)hipEventRecord(t.ev[0], ctx->cur);
+    return t.cand;
+}
+
+void tune_end(icf_ctx *ctx, bool acc, bool fp64)
+{
+    icf_ctx::KeyTune &t = ctx->tune[acc][fp64];
+    if (t.cand >= 0 && !ctx->capturing) (void)hipEventRecord(t.ev[1], ctx->cur);
+}
+
+// Calls f(std::integral_constant<int, KEY_CANDS[c]>()).
+template <typename F>
+void with_keys(int c, F &&f)
+{
+    static_assert(NKC == 4 && KEY_CANDS[0] == 64 && KEY_CANDS[1] == 128 && KEY_CANDS[2] == 256 &&
+                      KEY_CANDS[3] == 512, "one case per candidate");
+    switch (c) {
+    case 0: f(std::integral_constant<int, 64>()); break;
+    case 1: f(std::integral_constant<int, 128>()); break;
+    case 2: f(std::integral_constant<int, 256>()); break;
+    default: f(std::integral_constant<int, 512>()); break;
+    }
+}
+
 // BCSR values (overwrite) and node forces with their tet sums as one launch.  False when the
-// value map is not an overwrite map of FLAT_KEYS_OW-key chunks: the caller launches them apart.
+// value map is an accumulate map: the caller launches them apart.
 bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap *mf, double *F, int ns,
                       const Stage &st)
 {
-    static_assert(FLAT_KEYS_OW == 256, "fused with 256-lane workgroups");
-    if (mv->keys || mv->chunk_keys != FLAT_KEYS_OW) return false;
+    if (mv->keys) return false;
+    if (mv->nlaunch == 0 && mf->nlaunch == 0) return true;
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
-    const int k = mixed ? 0 : 1, nch = mv->nlaunch ? mv->nchunks[k] : 0;
-    const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, 256)) : 0, nbv = nch;
-    if (nbv + nbf == 0) return true;
-    const dim3 g(nbv + nbf), b(256);
+    const int k = mixed ? 0 : 1;
     double *ts = (double *)ctx->tsum.p;
+    const int c = tune_begin(ctx, false, !mixed);
+    with_keys(c, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
+        const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
+        const dim3 g(nch + nbf), b(K);   // neither is 0: at least one map has keys
 #define ICF_VF(MX, IS, L, NAME)                                                                                   \
-    LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, FLAT_KEYS_OW>), g, b, nbf, mv->off, mv->codes, mv->chunks[k], \
-           nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts)
-    if (mixed && iso) ICF_VF(true, true, float, "k_vals_force<mixed,iso>");
-    else if (mixed) ICF_VF(true, false, float, "k_vals_force<mixed,gen>");
-    else if (iso) ICF_VF(false, true, double, "k_vals_force<fp64,iso>");
-    else ICF_VF(false, false, double, "k_vals_force<fp64,gen>");
+    LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, K>), g, b, nbf, mv->off, mv->codes, mv->chunks[c][k], nch, A, \
+           mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts)
+        if (mixed && iso) ICF_VF(true, true, float, "k_vals_force<mixed,iso>");
+        else if (mixed) ICF_VF(true, false, float, "k_vals_force<mixed,gen>");
+        else if (iso) ICF_VF(false, true, double, "k_vals_force<fp64,iso>");
+        else ICF_VF(false, false, double, "k_vals_force<fp64,gen>");
 #undef ICF_VF
+    });
+    tune_end(ctx, false, !mixed);
     return true;
 }
 
 // The collision response's sums into node forces, BCSR blocks and rhs rows as one launch.
-// False when the value map's chunks are not FLAT_KEYS_ACC = 256-key ones.
+// False when the value map is an overwrite map.
 bool contact_sums_fused(icf_ctx *ctx, const SlotMap *mf, double *F, int ns, const SlotMap *mv, double *A,
                         const SlotMap *mr, double *b, const Stage &st)
 {
-    static_assert(FLAT_KEYS_ACC == 256, "fused with 256-lane workgroups");
-    if (!mv->keys || mv->chunk_keys != FLAT_KEYS_ACC) return false;
+    if (!mv->keys) return false;
+    if (mf->nlaunch == 0 && mv->nlaunch == 0 && mr->nlaunch == 0) return true;
     const bool mixed = ctx->precision == ICF_MIXED;
-    const int k = mixed ? 0 : 1, nch = mv->nlaunch ? mv->nchunks[k] : 0;
-    const int nb0 = mf->nlaunch ? round8(blocks_for(mf->nlaunch, 256)) : 0, nb1 = round8(nch);
-    const int nb2 = mr->nlaunch ? (int)blocks_for(mr->nlaunch, 256) : 0;
-    if (nb0 + nb1 + nb2 == 0) return true;
-    const dim3 g(nb0 + nb1 + nb2), bl(256);
+    const int k = mixed ? 0 : 1;
+    const int c = tune_begin(ctx, true, !mixed);
+    with_keys(c, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
+        const int nb0 = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0, nb1 = round8(nch);
+        const int nb2 = mr->nlaunch ? (int)blocks_for(mr->nlaunch, K) : 0;
+        const dim3 g(nb0 + nb1 + nb2), bl(K);
 #define ICF_CS(MX, L, NAME)                                                                                       \
-    LAUNCH(ctx, NAME, (k_contact_sums<MX, L, FLAT_KEYS_ACC>), g, bl, nb0, nb1, mf->off, mf->codes, mf->keys,       \
-           mf->nlaunch, F, ns, mv->off, mv->codes, mv->keys, mv->chunks[k], nch, A, mr->off, mr->codes, mr->keys,   \
-           mr->nlaunch, b, st)
-    if (mixed) ICF_CS(true, float, "k_contact_sums<mixed>");
-    else ICF_CS(false, double, "k_contact_sums<fp64>");
+    LAUNCH(ctx, NAME, (k_contact_sums<MX, L, K>), g, bl, nb0, nb1, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, \
+           mv->off, mv->codes, mv->keys, mv->chunks[c][k], nch, A, mr->off, mr->codes, mr->keys, mr->nlaunch, b, st)
+        if (mixed) ICF_CS(true, float, "k_contact_sums<mixed>");
+        else ICF_CS(false, double, "k_contact_sums<fp64>");
 #undef ICF_CS
+    });
+    tune_end(ctx, true, !mixed);
     return true;
 }
 
@@ -824,24 +904,19 @@ Stage make_stage(icf_ctx *ctx, double h)
     return sv;
 }
 
-template <bool MX, bool IS, bool ACC, typename L, int KEYS>
-void vals_launch(icf_ctx *ctx, const char *name, const SlotMap *m, int k, double *A, const Stage &sv)
-{
-    const int n = m->nchunks[k];
-    LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L, KEYS>), dim3(n), dim3(KEYS), m->off, m->codes, m->keys,
-           m->chunks[k], n, A, sv);
-}
-
 template <bool MX, bool IS, bool ACC, typename L>
 void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, const Stage &sv)
 {
     const int k = sizeof(L) == 4 ? 0 : 1;
-    if (m->nchunks[k] == 0) return;
-    // the workgroup size is the key limit the map's chunk table was built with
-    if (m->chunk_keys == FLAT_KEYS_ACC)
-        vals_launch<MX, IS, ACC, L, FLAT_KEYS_ACC>(ctx, name, m, k, A, sv);
-    else
-        vals_launch<MX, IS, ACC, L, FLAT_KEYS_OW>(ctx, name, m, k, A, sv);
+    if (m->nlaunch == 0) return;   // then no chunks for any key limit
+    const int c = tune_begin(ctx, ACC, k == 1);
+    with_keys(c, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const int n = m->nchunks[c][k];
+        LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L, K>), dim3(n), dim3(K), m->off, m->codes, m->keys,
+               m->chunks[c][k], n, A, sv);
+    });
+    tune_end(ctx, ACC, k == 1);
 }
 
 // Phase 2 of BCSR values for map m.  accumulate: add into A over the keys with contributions
@@ -950,6 +1025,10 @@ int icf_destroy(icf_ctx *ctx)
         ctx->ev_pool.push_back(p.b);
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto &row : ctx->tune)
+        for (auto &t : row)
+            for (hipEvent_t e : t.ev)
+                if (e) (void)hipEventDestroy(e);
     for (int l = 0; l < 2; l++)
         for (int k = 0; k < LANE_EVENTS; k++)
             if (ctx->lane_ev[l][k]) (void)hipEventDestroy(ctx->lane_ev[l][k]);
@@ -1249,6 +1328,61 @@ int icf_set_overlap(icf_ctx *ctx, int on)
     if (!ctx) return ICF_E_INVALID;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_set_overlap during capture");
     ctx->overlap = on != 0;
+    return 0;
+}
+
+int icf_set_autotune(icf_ctx *ctx, int rounds)
+{
+    if (!ctx || rounds < 0 || rounds > 1000) return ICF_E_INVALID;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_set_autotune during capture");
+    if (int r = check_ctx(ctx)) return r;
+    for (auto &row : ctx->tune)
+        for (icf_ctx::KeyTune &t : row) {
+            if (rounds == 0) {         // stop: fold what was timed, keep the fastest so far
+                if (t.active) {
+                    tune_fold(ctx, t);
+                    for (int c = 0; c < NKC; c++)
+                        if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
+                }
+                t.active = false;
+                continue;
+            }
+            for (hipEvent_t &e : t.ev)
+                if (!e) HIPCHK(ctx, hipEventCreate(&e));
+            t.active = true;
+            t.turn = t.samples = 0;
+            t.cand = -1;
+            for (float &b : t.best) b = 0;
+        }
+    ctx->tune_rounds = rounds;
+    return 0;
+}
+
+int icf_get_autotune(icf_ctx *ctx, int *keys, int *tuning, float *best_ms)
+{
+    if (!ctx) return ICF_E_INVALID;
+    for (int r = 0; r < 4; r++) {
+        icf_ctx::KeyTune &t = ctx->tune[r >> 1][r & 1];
+        if (t.active && !ctx->capturing) tune_fold(ctx, t);   // a finished last timing counts
+        if (keys) keys[r] = KEY_CANDS[t.sel];
+        if (tuning) tuning[r] = t.active ? 1 : 0;
+        if (best_ms)
+            for (int c = 0; c < NKC; c++) best_ms[r * NKC + c] = t.best[c];
+    }
+    return 0;
+}
+
+int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys)
+{
+    if (!ctx) return ICF_E_INVALID;
+    int c = 0;
+    while (c < NKC && KEY_CANDS[c] != keys) c++;
+    if (c == NKC) return fail(ctx, ICF_E_INVALID, "icf_set_chunk_keys: keys must be 64, 128, 256 or 512");
+    icf_ctx::KeyTune &t = ctx->tune[accumulate != 0][fp64 != 0];
+    if (t.active && !ctx->capturing) tune_fold(ctx, t);
+    t.active = false;
+    t.cand = -1;
+    t.sel = c;
     return 0;
 }
 

@@ -121,10 +121,14 @@ __global__ __launch_bounds__(256) void k_map_compact(const int *__restrict__ fla
 
 // Chunk limits of the staged BCSR-value reduction: keys per workgroup (one thread each) and
 // contributions staged in LDS, 3 FP32 or 1.5 FP64 values x 9 per key (27 KB for 256 keys,
-// so that five such workgroups share a CU).  Overwrite maps (every key of the BCSR, in order)
-// and accumulate maps (the keys with contributions) each have their own key limit; a map has
-// a chunk table per staging type.
-constexpr int FLAT_KEYS_OW = 256, FLAT_KEYS_ACC = 256;
+// so that five such workgroups share a CU).  The key limit is the workgroup size; a value map
+// has a chunk table for every candidate limit and staging type, and the library's autotuner
+// picks the limit per role (overwrite or accumulate map, FP32 or FP64 staging) by timing the
+// candidates.  Every limit gives the same values: each key's contributions are added in list
+// order whatever the chunking.
+constexpr int NKC = 4;                                   // candidate key limits
+constexpr int KEY_CANDS[NKC] = {64, 128, 256, 512};
+constexpr int KC_DEFAULT = 2;                            // 256 keys until tuned
 template <typename L, int KEYS>
 __host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? 3 * KEYS : 3 * KEYS / 2; }
 __host__ __device__ constexpr int flat_cap_of(bool fp32, int keys) { return fp32 ? 3 * keys : 3 * keys / 2; }

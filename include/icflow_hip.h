@@ -166,12 +166,35 @@ int icf_set_profiling(icf_ctx *ctx, int on);
  *    runs on a second stream of the context beside the assembly.  Each call waits for the
  *    earlier calls that write bytes it reads, or read or write bytes it writes, so results are
  *    those of one stream; device memory touched outside the library needs icf_synchronize().
- *  - captured: the fused call runs tets beside CZs and the BCSR values beside the node forces
- *    and rhs rows, joined back before it returns.
- * 0 keeps every kernel on the context's stream.  While profiling is on everything runs on one
+ *  - every mode: independent phases share one launch -- icf_assemble_elems_and_czs runs its
+ *    tets with its CZs, its BCSR values with its node forces and its rhs rows with its CZ
+ *    counts; icf_collision_response adds its node forces, BCSR blocks and rhs rows in one.
+ * 0 keeps every kernel on the context's stream, one phase per launch.  While profiling is on everything runs on one
  * stream, so each kernel's events time it alone.  No reference counterpart (the reference's
  * Run() issues one stream, GPU_Functionality.cs:451-476). */
 int icf_set_overlap(icf_ctx *ctx, int on);
+
+/* Autotuner of the BCSR-value reductions.  Their workgroups reduce chunks of consecutive keys
+ * (BCSR blocks), one key per lane; the key limit (64, 128, 256 or 512 = ICF_KEY_CANDIDATES
+ * sizes, 256 until tuned) trades staging-buffer size (LDS per workgroup, hence workgroups per
+ * CU) against the tail of a chunk.  There are four roles, r = 2 * accumulate + fp64:
+ * overwrite maps (icf_assemble_elems_and_czs) and accumulate maps (icf_collision_response and
+ * the single-kernel calls), each with FP32 (mixed) or FP64 staging.
+ * icf_set_autotune(ctx, rounds > 0) starts tuning every role: its next launches take the
+ * candidates in turn, each timed with events (waiting for the previous timed launch of the
+ * role), until every candidate has `rounds` timings; then the fastest (least time over its
+ * timings) is kept.  rounds = 0 stops tuning and keeps the fastest timed so far.  Captured
+ * calls are never timed.  Time it in blocking mode or with overlap off for clean timings.
+ * icf_get_autotune: per role, keys[r] = the key limit in use, tuning[r] = 1 while tuning,
+ * best_ms[r * ICF_KEY_CANDIDATES + c] = fastest timing of candidate c (0 = none); any of the
+ * three may be null.  icf_set_chunk_keys fixes one role's key limit (ends its tuning).
+ * Results do not depend on the key limit: each key's contributions are added in list order
+ * whatever the chunking.  No reference counterpart (the reference launches every kernel with
+ * a fixed block_size = 128, GPU_Functionality.cs:38,123-136). */
+#define ICF_KEY_CANDIDATES 4
+int icf_set_autotune(icf_ctx *ctx, int rounds);
+int icf_get_autotune(icf_ctx *ctx, int *keys, int *tuning, float *best_ms);
+int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys);
 int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);
 int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches);
