@@ -347,10 +347,22 @@ def _rocprof_symbol(name):
     return name
 
 
-def profiled_traffic(name, path=os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")):
+def key_limit(gf, name):
+    """The workgroup size (key limit) the autotuner chose for a value-reduction kernel, or None
+    for kernels without one."""
+    base = name.split("<")[0]
+    if base not in ("k_vals_force", "k_contact_sums", "k_reduce_vals"):
+        return None
+    stage = "fp64" if "fp64" in name else "fp32"
+    acc = base == "k_contact_sums" or "acc" in name or "contact" in name
+    return gf.autotune_state()[f"{'accumulate' if acc else 'overwrite'},{stage}"][0]
+
+
+def profiled_traffic(name, keys=None, path=os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-probe factors), or None."""
-    sym = _rocprof_symbol(name)
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-probe factors), or None.  keys: the
+    workgroup size of a kernel templated on it (its last template argument)."""
+    sym = _rocprof_symbol(name) + (f", {keys}>" if keys else "")
     try:
         for line in open(path):
             head = f"| `{sym}"
@@ -509,8 +521,9 @@ def main():
     dbytes = kernel_bytes(alg, dname) or 0
     avg_s = dms / dn / 1e3
     achieved = dbytes / avg_s / 1e9
-    traffic = profiled_traffic(dname) if contact else None
-    roof = dict(kernel=dname, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+    dkeys = key_limit(r.gf, dname)
+    traffic = profiled_traffic(dname, dkeys) if contact else None
+    roof = dict(kernel=dname + (f" ({dkeys}-lane workgroups)" if dkeys else ""), bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=int(traffic) if traffic else None,
                 alg_bytes=int(dbytes), avg_us=round(avg_s * 1e6, 2),
                 traffic_source="profiles/assembly_compression40_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
