@@ -86,6 +86,14 @@ class Runner:
             log(f"pattern: N={gf.csr.N} nnzb={gf.csr.nnz} ({time.time() - t:.1f} s)")
             self.vals = gf.buf("vals", 2 * gf.csr.nnz * 9, np.float64)
             self.rhs = gf.buf("rhs", gf.csr.N * 3, np.float64)
+            # specimens of a batched collection: per-specimen offsets for icf_assemble_batched
+            self.worklist = None
+            if len(col.meshes) > 1:
+                from icflow_amd import specimens
+                tet, cz, row = specimens.specimen_offsets(col)
+                offs = [tet, cz, row, gf.csr.rows[row].astype(np.int32)]
+                self.worklist_arrays = offs
+                self.worklist = _lib.IcfWorklist(len(tet) - 1, *[_lib.ptr(a) for a in offs])
             return
         t = time.time()
         self.pairs = broad_phase(gf, gf.c_dn, gf.nd_stride, gf.c_ie, gf.el_all_stride, len(col.surf))
@@ -136,6 +144,12 @@ class Runner:
         """One iteration's device calls.  Blocking by default (the drop-in sequence); with
         icf_set_blocking(0) they only enqueue, and between icf_graph_begin/end they are captured."""
         gf, lib, ctx, chk = self.gf, self.lib, self.gf.ctx, self.check
+        if not self.contact and self.worklist is not None:
+            chk(lib.icf_assemble_batched(
+                ctx, ctypes.byref(self.worklist), gf.g_ie_pcsr.p, gf.el_elastic_stride, gf.g_dcz.p, gf.g_icz.p,
+                gf.cz_stride, gf.g_dn.p, gf.nd_stride, self.h, self.vals.p, self.rhs.p, None, None), ctx,
+                "icf_assemble_batched")
+            return
         if not self.contact:
             chk(lib.icf_assemble_elems_and_czs(
                 ctx, gf.g_ie_pcsr.p, len(self.col.elastic), gf.el_elastic_stride, gf.g_dcz.p, gf.g_icz.p,

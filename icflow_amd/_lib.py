@@ -40,6 +40,12 @@ class IcfModelPrms(ctypes.Structure):
         "alpha", "beta", "lambda_n", "lambda_t", "phi_n", "phi_t", "sigma_max", "tau_max")]
 
 
+class IcfWorklist(ctypes.Structure):
+    """Mirror of icf_worklist (include/icflow_hip.h): per-specimen offsets, host arrays."""
+    _fields_ = [("nSpecimens", c_int), ("tet_off", c_void_p), ("cz_off", c_void_p), ("row_off", c_void_p),
+                ("nnzb_off", c_void_p)]
+
+
 _SIGS = {
     "icf_create": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
     "icf_destroy": (c_int, [c_void_p]),
@@ -64,6 +70,9 @@ _SIGS = {
     "icf_assemble_elems_and_czs": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                            c_void_p, c_int, c_double, c_void_p, c_int, c_void_p, c_int,
                                            P_int, P_int]),
+    "icf_assemble_batched": (c_int, [c_void_p, ctypes.POINTER(IcfWorklist), c_void_p, c_int, c_void_p, c_void_p,
+                                     c_int, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "icf_batch_cz_counts": (c_int, [c_void_p, c_void_p, c_void_p]),
     "icf_sum_int": (c_int, [c_void_p, c_void_p, c_int, P_int]),
     "icf_pattern_changed": (c_int, [c_void_p]),
     "icf_last_device_ms": (c_double, [c_void_p]),

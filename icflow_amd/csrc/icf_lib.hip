@@ -28,6 +28,7 @@ __constant__ DevConsts g_c;
 #include "icf_contact_resp.inc.h"
 #include "icf_reduce.inc.h"
 #include "icf_fused.inc.h"
+#include "icf_batch.inc.h"
 #include "icf_nodes.inc.h"
 #include "icf_pattern.inc.h"
 #include "icf_broad.inc.h"
@@ -193,6 +194,14 @@ struct icf_ctx {
         hipEvent_t ev[2] = {nullptr, nullptr};
     } tune[2][2];
     int tune_rounds = 0;           // timed launches per candidate (icf_set_autotune)
+    // icf_assemble_batched: the last offsets (host copy; device copy followed by 2 counts per
+    // specimen) and the value and rhs maps whose units were last checked against them
+    struct Batch {
+        std::vector<int> off;      // 4 (n + 1): tets, CZs, rows, blocks
+        int n = 0;
+        DevBuf d;
+        int checked_v = 0, checked_r = 0;
+    } batch;
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
     struct Pending {
@@ -1014,7 +1023,7 @@ int icf_destroy(icf_ctx *ctx)
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
                       &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l,
-                      &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids};
+                      &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids, &ctx->batch.d};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->small) (void)hipFree(ctx->small);
@@ -1623,9 +1632,10 @@ int icf_sum_int(icf_ctx *ctx, const int *d, int n, int *out)
     return 0;
 }
 
-int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_stride, double *dcz, int *icz, int nCZs,
-                               int cz_stride, double *dn, int nd_stride, double h, double *A, int nnzb, double *b,
-                               int N, int *nCZDamaged, int *nCZFailed)
+// The fused call; map_ids (optional) receives the ids of its value and rhs maps.
+static int assemble_impl(icf_ctx *ctx, const int *ie, int nElems, int el_stride, double *dcz, int *icz, int nCZs,
+                         int cz_stride, double *dn, int nd_stride, double h, double *A, int nnzb, double *b, int N,
+                         int *nCZDamaged, int *nCZFailed, int *map_ids)
 {
     if (int r = check_ctx(ctx)) return r;
     if (bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) || nd_stride < 0 || nnzb < 0 || N < 0 ||
@@ -1661,6 +1671,10 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
     if (int r = get_map(ctx, TGT_RHS, N, sr, &mr)) return r;
     if (nElems)
         if (int r = get_map(ctx, TGT_FORCE, -1, sfv, &mf)) return r;
+    if (map_ids) {
+        map_ids[0] = mv->id;
+        map_ids[1] = mr->id;
+    }
     bool tsum = false;
     if (mf) {
         MapSrc tet{ie, nElems, el_stride, EL_NODES, 4, KIND_TET};
@@ -1720,6 +1734,115 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         if (nCZDamaged) *nCZDamaged = sums[0];
         if (nCZFailed) *nCZFailed = sums[1];
+    }
+    return 0;
+}
+
+int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_stride, double *dcz, int *icz, int nCZs,
+                               int cz_stride, double *dn, int nd_stride, double h, double *A, int nnzb, double *b,
+                               int N, int *nCZDamaged, int *nCZFailed)
+{
+    return assemble_impl(ctx, ie, nElems, el_stride, dcz, icz, nCZs, cz_stride, dn, nd_stride, h, A, nnzb, b, N,
+                         nCZDamaged, nCZFailed, nullptr);
+}
+
+int icf_assemble_batched(icf_ctx *ctx, const icf_worklist *wl, const int *ie, int el_stride, double *dcz, int *icz,
+                         int cz_stride, double *dn, int nd_stride, double h, double *A, double *b, int *nCZDamaged,
+                         int *nCZFailed)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (!wl || wl->nSpecimens < 1 || wl->nSpecimens > (1 << 24) || !wl->tet_off || !wl->cz_off || !wl->row_off ||
+        !wl->nnzb_off)
+        return fail(ctx, ICF_E_INVALID, "icf_assemble_batched: invalid worklist");
+    const int n = wl->nSpecimens;
+    const int *tabs[4] = {wl->tet_off, wl->cz_off, wl->row_off, wl->nnzb_off};
+    std::vector<int> off(4 * ((size_t)n + 1));
+    for (int k = 0; k < 4; k++) {
+        if (tabs[k][0] != 0) return fail(ctx, ICF_E_INVALID, "icf_assemble_batched: offsets must start at 0");
+        for (int s = 0; s <= n; s++) {
+            if (s && tabs[k][s] < tabs[k][s - 1])
+                return fail(ctx, ICF_E_INVALID, "icf_assemble_batched: offsets must not decrease");
+            off[(size_t)k * (n + 1) + s] = tabs[k][s];
+        }
+    }
+    const int nE = wl->tet_off[n], nC = wl->cz_off[n], N = wl->row_off[n], nnzb = wl->nnzb_off[n];
+    icf_ctx::Batch &B = ctx->batch;
+    if (off != B.off) {
+        if (ctx->capturing)
+            return fail(ctx, ICF_E_CAPTURE, "icf_assemble_batched: new offsets cannot be uploaded during capture");
+        if (int r = lane_barrier(ctx)) return r;
+        if (int r = grow(ctx, B.d, sizeof(int) * (off.size() + 2 * (size_t)n))) return r;
+        HIPCHK(ctx, hipMemcpyAsync(B.d.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        B.off = off;
+        B.n = n;
+        B.checked_v = B.checked_r = 0;
+    }
+    int ids[2] = {0, 0};
+    if (int r = assemble_impl(ctx, ie, nE, el_stride, dcz, icz, nC, cz_stride, dn, nd_stride, h, A, nnzb, b, N,
+                              nullptr, nullptr, ids))
+        return r;
+    const int *d = (const int *)B.d.p;
+    const BatchOff bo{d, d + (n + 1), d + 2 * (n + 1), d + 3 * (n + 1), n};
+    int *cnt = (int *)B.d.p + 4 * (n + 1);
+    OnLane ln(ctx, 0, [&] {
+        return Access()
+            .rd(nE ? ie : nullptr, (EL_ROWS + 4) * (size_t)el_stride)
+            .rd(nC ? icz : nullptr, (CZI_ROWS + 6) * (size_t)cz_stride);
+    });
+    if (ln.err) return ln.err;
+    if (ids[0] != B.checked_v || ids[1] != B.checked_r) {
+        // the block-diagonal layout, checked once per pair of maps
+        if (ctx->capturing)
+            return fail(ctx, ICF_E_CAPTURE, "icf_assemble_batched: run the batch once uncaptured before capture");
+        int *d_err = ctx->small + 1;
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->cur, d_err, 1, 0);
+        if (nE + nC)
+            hipLaunchKernelGGL(k_batch_check, dim3(blocks_for((long long)nE + nC, 256)), dim3(256), 0, ctx->cur, ie, nE,
+                               el_stride, icz, nC, cz_stride, bo, d_err);
+        int e = 0;
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(&e, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->cur));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->cur));
+        if (e)
+            return fail(ctx, ICF_E_INDEX,
+                        "icf_assemble_batched: a unit writes a BCSR block or rhs row outside its specimen's ranges");
+        B.checked_v = ids[0];
+        B.checked_r = ids[1];
+    }
+    LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(blocks_for(2 * (long long)n, 256)), dim3(256), cnt, 2 * n, 0);
+    if (nC) {
+        const int *dmg = icz + (size_t)cz_stride * CZI_TDAMAGED, *fld = icz + (size_t)cz_stride * CZI_TFAILED;
+        LAUNCH(ctx, "k_batch_counts", k_batch_counts, dim3(blocks_for(nC, 256)), dim3(256), dmg, fld, nC, bo.cz, n,
+               cnt);
+    }
+    if (int r = ln.done()) return r;
+    if ((nCZDamaged || nCZFailed) && ctx->blocking && !ctx->capturing) {
+        std::vector<int> c(2 * (size_t)n);
+        HIPCHK(ctx, hipMemcpyAsync(c.data(), cnt, sizeof(int) * c.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (int k = 0; k < n; k++) {
+            if (nCZDamaged) nCZDamaged[k] = c[2 * (size_t)k];
+            if (nCZFailed) nCZFailed[k] = c[2 * (size_t)k + 1];
+        }
+    }
+    return 0;
+}
+
+int icf_batch_cz_counts(icf_ctx *ctx, int *nCZDamaged, int *nCZFailed)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_batch_cz_counts during capture");
+    const int n = ctx->batch.n;
+    if (n == 0) return fail(ctx, ICF_E_INVALID, "icf_batch_cz_counts: no batched call yet");
+    if (int r = sync_lanes(ctx)) return r;
+    std::vector<int> c(2 * (size_t)n);
+    HIPCHK(ctx, hipMemcpyAsync(c.data(), (int *)ctx->batch.d.p + 4 * (n + 1), sizeof(int) * c.size(),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < n; k++) {
+        if (nCZDamaged) nCZDamaged[k] = c[2 * (size_t)k];
+        if (nCZFailed) nCZFailed[k] = c[2 * (size_t)k + 1];
     }
     return 0;
 }

@@ -259,6 +259,24 @@ class GPUFunctionality:
         self.nCZDamaged, self.nCZFailedThisStep = dmg.value, fld.value
         return self.nCZDamaged, self.nCZFailedThisStep
 
+    def AssembleBatched(self, h, tet_off, cz_off, row_off, nnzb_off):
+        """icf_assemble_batched over the collection's specimens (per-specimen offsets, each
+        n + 1 ints from 0); returns the per-specimen damaged and failed CZ counts."""
+        nnz, N = self.csr.nnz, self.csr.N
+        self.g_dvals = self.buf("vals", max(1, 2 * nnz * 9), np.float64)
+        self.g_drhs = self.buf("rhs", max(1, N * 3), np.float64)
+        offs = [np.ascontiguousarray(a, np.int32) for a in (tet_off, cz_off, row_off, nnzb_off)]
+        n = len(offs[0]) - 1
+        wl = _lib.IcfWorklist(n, *[ptr(a) for a in offs])
+        dmg, fld = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        nC = int(offs[1][-1])
+        check(self.lib.icf_assemble_batched(
+            self.ctx, ctypes.byref(wl), self.g_ie_pcsr.p, self.el_elastic_stride,
+            self.g_dcz.p if nC else None, self.g_icz.p if nC else None, self.cz_stride,
+            self.g_dn.p, self.nd_stride, h, self.g_dvals.p, self.g_drhs.p, ptr(dmg), ptr(fld)),
+            self.ctx, "icf_assemble_batched")
+        return dmg, fld
+
     def last_device_ms(self):
         return self.lib.icf_last_device_ms(self.ctx)
 

@@ -356,6 +356,24 @@ def batch(collections):
     return out.finalize()
 
 
+def specimen_offsets(col):
+    """Unit offsets per specimen of a batched collection whose meshes are its specimens: tets,
+    CZs and active rows (n + 1 int32 each, from 0).  Units are mesh-ordered (finalize), so each
+    specimen's units are one run."""
+    n = len(col.meshes)
+    nm = col.node_mesh
+
+    def off(first_nodes):
+        k = nm[first_nodes] if len(first_nodes) else np.zeros(0, np.int64)
+        assert np.all(np.diff(k) >= 0), "units are not ordered by specimen"
+        return np.r_[0, np.cumsum(np.bincount(k, minlength=n))].astype(np.int32)
+
+    tet = off(col.elastic[:, 0])
+    cz = off(col.czs[:, 0]) if len(col.czs) else np.zeros(n + 1, np.int32)
+    row = off(np.flatnonzero(~col.anchored))
+    return tet, cz, row
+
+
 def lognormal_cubes(n_specimens, seed, median_tets=12000, min_tets=2000, max_tets=40000, sigma=0.6,
                     grains=(8, 64), size=0.02):
     """The batched workload: cubes whose tet counts are lognormal (median `median_tets`,

@@ -122,6 +122,33 @@ int icf_assemble_elems_and_czs(icf_ctx *ctx, const int *ie, int nElems, int el_s
                                double *A, int nnzb, double *b, int N,
                                int *nCZDamaged, int *nCZFailed);
 
+/* Batched specimens: one icf_assemble_elems_and_czs over many independent specimens whose
+ * units lie back to back in its buffers.  Specimen s owns the elastic tets
+ * [tet_off[s], tet_off[s+1]), the CZs [cz_off[s], cz_off[s+1]), the rhs rows
+ * [row_off[s], row_off[s+1]) and the BCSR blocks [nnzb_off[s], nnzb_off[s+1]) (host arrays of
+ * nSpecimens + 1 ints, from 0, non-decreasing); the totals tet_off[n], cz_off[n], row_off[n],
+ * nnzb_off[n] are the call's nElems, nCZs, N and nnzb.  Every phase is one launch over all
+ * specimens, and the results are those of icf_assemble_elems_and_czs over the whole lists, bit
+ * for bit.  Beyond that call:
+ *  - nCZDamaged[s], nCZFailed[s] (host arrays of nSpecimens, may be NULL; filled in blocking
+ *    mode): the flag sums of GPU_Functionality.cs:470-475 per specimen; icf_batch_cz_counts
+ *    reads those of the last batched call at any time (it waits for the device);
+ *  - the layout is checked on the first call and whenever the slot or row fields change: a
+ *    unit that writes a block or row outside its specimen's ranges fails with ICF_E_INDEX.
+ * New offsets are copied to the device (not during capture).  No reference counterpart: the
+ * reference runs one specimen per process (ImplicitModel3.Step, ImplicitModel3.cs:377-435). */
+typedef struct icf_worklist {
+    int nSpecimens;
+    const int *tet_off;
+    const int *cz_off;
+    const int *row_off;
+    const int *nnzb_off;
+} icf_worklist;
+int icf_assemble_batched(icf_ctx *ctx, const icf_worklist *wl, const int *ie, int el_stride,
+                         double *dcz, int *icz, int cz_stride, double *dn, int nd_stride,
+                         double h, double *A, double *b, int *nCZDamaged, int *nCZFailed);
+int icf_batch_cz_counts(icf_ctx *ctx, int *nCZDamaged, int *nCZFailed);
+
 /* Sum of n ints in device memory (GPU_Functionality.Sum, :93-107). */
 int icf_sum_int(icf_ctx *ctx, const int *d, int n, int *out);
 
