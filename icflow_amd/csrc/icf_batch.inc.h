@@ -57,32 +57,23 @@ __global__ __launch_bounds__(256) void k_batch_check(const int *__restrict__ ie,
     if (bad) atomicOr(err, 1);
 }
 
-// counts[2 s] += sum of damaged[u], counts[2 s + 1] += sum of failed[u] over the CZs u of
-// specimen s.  A wavefront whose CZs all belong to one specimen adds once; otherwise each lane
-// adds its own.  Integer sums: the result does not depend on the order.
+// counts[2 s], counts[2 s + 1] = sums of damaged[u] and failed[u] over the CZs u of specimen
+// s = blockIdx.x (one 256-thread workgroup per specimen; no atomics, every count written).
 __global__ __launch_bounds__(256) void k_batch_counts(const int *__restrict__ damaged, const int *__restrict__ failed,
-                                                      int nC, const int *__restrict__ cz_off, int n,
-                                                      int *__restrict__ counts)
+                                                      const int *__restrict__ cz_off, int *__restrict__ counts)
 {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = u < nC;
-    const int s = live ? specimen_of(cz_off, n, u) : -1;
-    int d = live ? damaged[u] : 0, f = live ? failed[u] : 0;
-    const int s0 = __shfl(s, 0, 64);
-    const bool uniform = __all(!live || s == s0);
-    if (uniform) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            d += __shfl_xor(d, o, 64);
-            f += __shfl_xor(f, o, 64);
-        }
-        if ((threadIdx.x & 63) == 0 && s0 >= 0) {
-            if (d) atomicAdd(counts + 2 * s0, d);
-            if (f) atomicAdd(counts + 2 * s0 + 1, f);
-        }
-    } else if (live) {
-        if (d) atomicAdd(counts + 2 * s, d);
-        if (f) atomicAdd(counts + 2 * s + 1, f);
+    const int s = blockIdx.x;
+    const int a = cz_off[s], b = cz_off[s + 1];
+    int d = 0, f = 0;
+    for (int u = a + (int)threadIdx.x; u < b; u += 256) {
+        d += damaged[u];
+        f += failed[u];
+    }
+    d = block_sum256(d);
+    f = block_sum256(f);
+    if (threadIdx.x == 0) {
+        counts[2 * s] = d;
+        counts[2 * s + 1] = f;
     }
 }
 

@@ -1810,11 +1810,11 @@ int icf_assemble_batched(icf_ctx *ctx, const icf_worklist *wl, const int *ie, in
         B.checked_v = ids[0];
         B.checked_r = ids[1];
     }
-    LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(blocks_for(2 * (long long)n, 256)), dim3(256), cnt, 2 * n, 0);
     if (nC) {
         const int *dmg = icz + (size_t)cz_stride * CZI_TDAMAGED, *fld = icz + (size_t)cz_stride * CZI_TFAILED;
-        LAUNCH(ctx, "k_batch_counts", k_batch_counts, dim3(blocks_for(nC, 256)), dim3(256), dmg, fld, nC, bo.cz, n,
-               cnt);
+        LAUNCH(ctx, "k_batch_counts", k_batch_counts, dim3(n), dim3(256), dmg, fld, bo.cz, cnt);
+    } else {
+        LAUNCH(ctx, "k_fill_int", k_fill_int, dim3(blocks_for(2 * (long long)n, 256)), dim3(256), cnt, 2 * n, 0);
     }
     if (int r = ln.done()) return r;
     if ((nCZDamaged || nCZFailed) && ctx->blocking && !ctx->capturing) {
