@@ -302,6 +302,7 @@ def algorithmic_bytes(r):
     alg["k_phase1"] = alg["k_tet_phase1"] + alg["k_cz_phase1"]
     alg["k_vals_force"] = alg["k_reduce_vals"] + alg["k_reduce_vec3<force>"]
     alg["k_rhs_counts"] = alg["k_reduce_vec3<rhs>"] + alg["k_count_flags"]
+    alg["k_vals_force_rhs"] = alg["k_vals_force"] + alg["k_rhs_counts"]
     alg["k_contact_sums"] = alg["k_reduce_vals+acc"] + alg["k_reduce_vec3<rhs+acc>"] + alg["k_reduce_vec3<force+acc>"]
     return alg
 
@@ -352,9 +353,9 @@ def _rocprof_symbol(name):
         return f"{base}<{'float' if args[0] == 'mixed' else 'double'}>"
     if base == "k_phase1":
         return f"k_phase1_fused<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}>"
-    if base == "k_vals_force":
+    if base in ("k_vals_force", "k_vals_force_rhs"):
         mixed = args[0] == "mixed"
-        return f"k_vals_force_fused<{b(mixed)}, {b(args[1] == 'iso')}, {'float' if mixed else 'double'}"
+        return f"{base}_fused<{b(mixed)}, {b(args[1] == 'iso')}, {'float' if mixed else 'double'}"
     if base == "k_contact_sums":
         mixed = args[0] == "mixed"
         return f"k_contact_sums<{b(mixed)}, {'float' if mixed else 'double'}"
@@ -365,7 +366,7 @@ def key_limit(gf, name):
     """The workgroup size (key limit) the autotuner chose for a value-reduction kernel, or None
     for kernels without one."""
     base = name.split("<")[0]
-    if base not in ("k_vals_force", "k_contact_sums", "k_reduce_vals"):
+    if base not in ("k_vals_force", "k_vals_force_rhs", "k_contact_sums", "k_reduce_vals"):
         return None
     stage = "fp64" if "fp64" in name else "fp32"
     acc = base == "k_contact_sums" or "acc" in name or "contact" in name
@@ -385,6 +386,28 @@ def profiled_traffic(name, keys=None, path=os.path.join(ROOT, "profiles", "assem
     except OSError:
         return None
     return None
+
+
+def copy_bandwidth(r, mib=1024, reps=10):
+    """A STREAM-like copy through icf_d2d (the library's 16-byte-per-thread copy kernel): GB/s
+    of bytes read plus bytes written, best of `reps` blocking copies of `mib` MiB."""
+    gf, lib, ctx = r.gf, r.lib, r.gf.ctx
+    nb = mib << 20
+    a, b = gf.buf("bw_a", nb // 8, np.float64), gf.buf("bw_b", nb // 8, np.float64)
+    a.zero()
+    lib.icf_set_blocking(ctx, 1)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r.check(lib.icf_d2d(ctx, b.p, 0, a.p, 0, nb), ctx, "icf_d2d")
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    a.free()
+    b.free()
+    gf._bufs.pop("bw_a", None)
+    gf._bufs.pop("bw_b", None)
+    return dict(bytes=2 * nb, GBs=round(2 * nb / best / 1e9, 1), reps=reps,
+                note="best blocking icf_d2d copy, host-timed (includes the call's launch and wait)")
 
 
 def broad_phase_timing(r, reps=5):
@@ -546,6 +569,9 @@ def main():
 
     # device broad phase over the same surface tets (reported beside the step; the reference's
     # broad phase is a host BVH outside the assembly path)
+    bw = copy_bandwidth(r)
+    log(f"copy probe: {bw}")
+    roof["frac_of_copy_probe"] = round(achieved / bw["GBs"], 4)
     bp = broad_phase_timing(r) if contact else None
     if bp:
         log(f"device broad phase: {bp}")
@@ -595,6 +621,7 @@ def main():
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
         "autotune": tuned or None,
         "roofline": roof,
+        "hbm_copy_probe": bw,
         "device_broad_phase": bp,
         "precision_study": study,
         "cpu_baseline": cpu,

@@ -47,6 +47,27 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *_
                                                 nchunks, A, sv);
 }
 
+// Node forces with the rhs rows (reduce_force_rhs; workgroups [0, nbf)), the CZ counters (the
+// next nbc), then the BCSR values of the fused call (overwrite): the whole second half of the
+// fused call as one launch.  Workgroups of KEYS lanes.
+template <bool MIXED, bool ISO, typename L, int KEYS>
+__global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc, const int *__restrict__ off,
+        const uint32_t *__restrict__ codes, const int2 *__restrict__ chunks, int nchunks, double *__restrict__ A,
+        const int *__restrict__ foff, const uint32_t *__restrict__ fcodes, const int *__restrict__ fkeys, int nfkeys,
+        double *__restrict__ F, int ns, const int *__restrict__ node_row, const int *__restrict__ roff,
+        const uint32_t *__restrict__ rcodes, const int *__restrict__ cz_begin, double *__restrict__ b,
+        const int *__restrict__ damaged, const int *__restrict__ failed, int nc, int *__restrict__ counts, Stage sv)
+{
+    const int w = blockIdx.x;
+    if (w < nbf)
+        reduce_force_rhs(w, nbf, foff, fcodes, fkeys, nfkeys, F, ns, node_row, roff, rcodes, cz_begin, b, sv);
+    else if (w < nbf + nbc)
+        count_flags_n<KEYS>(w - nbf, nbc, damaged, failed, nc, counts);
+    else
+        reduce_vals<MIXED, ISO, false, L, KEYS>(w - nbf - nbc, gridDim.x - nbf - nbc, off, codes, nullptr, chunks,
+                                                nchunks, A, sv);
+}
+
 // rhs rows from the tet sums plus their CZ terms (workgroups [0, nbr)), then the CZ counters.
 __global__ __launch_bounds__(256) void k_rhs_counts_fused(int nbr, const int *__restrict__ off,
                                                           const uint32_t *__restrict__ codes,

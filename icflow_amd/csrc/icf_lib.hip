@@ -131,6 +131,10 @@ struct SlotMap {
     int tsum_with = 0;
     bool tsum_ok = false;
     int *row_node = nullptr, *cz_begin = nullptr;
+    // ... and per node of the force map its rhs row (-1: none), when every row has one tet node
+    // (then the node-force pass also writes the rhs rows)
+    int *node_row = nullptr;
+    bool merge_ok = false;
     // copy of the key fields the map was built from; after a write that may have touched them
     // (suspect) the map is kept if they still compare equal
     int *snap = nullptr;
@@ -254,6 +258,9 @@ void free_map(SlotMap &m)
     if (m.keys) (void)hipFree(m.keys);
     if (m.row_node) (void)hipFree(m.row_node);
     if (m.cz_begin) (void)hipFree(m.cz_begin);
+    if (m.node_row) (void)hipFree(m.node_row);
+    m.node_row = nullptr;
+    m.merge_ok = false;
     if (m.chunks[0][0]) (void)hipFree(m.chunks[0][0]);   // the start of the one allocation
     for (int c = 0; c < NKC; c++) m.chunks[c][0] = m.chunks[c][1] = nullptr;
     if (m.snap) (void)hipFree(m.snap);
@@ -555,8 +562,9 @@ int get_tsum_link(icf_ctx *ctx, SlotMap *mr, const SlotMap *mf, const MapSrc &te
             return fail(ctx, ICF_E_CAPTURE, "a slot map would be built during capture: run the call once uncaptured");
         if (mr->row_node) (void)hipFree(mr->row_node);
         if (mr->cz_begin) (void)hipFree(mr->cz_begin);
-        mr->row_node = mr->cz_begin = nullptr;
-        mr->tsum_ok = false;
+        if (mr->node_row) (void)hipFree(mr->node_row);
+        mr->row_node = mr->cz_begin = mr->node_row = nullptr;
+        mr->tsum_ok = mr->merge_ok = false;
         mr->tsum_with = mf->id;
         if (mr->nkeys > 0) {
             HIPCHK(ctx, hipMalloc(&mr->row_node, sizeof(int) * (size_t)mr->nkeys));
@@ -571,6 +579,19 @@ int get_tsum_link(icf_ctx *ctx, SlotMap *mr, const SlotMap *mf, const MapSrc &te
             HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
             mr->tsum_ok = err == 0;
+            if (mr->tsum_ok && mf->nkeys > 0) {
+                HIPCHK(ctx, hipMalloc(&mr->node_row, sizeof(int) * (size_t)mf->nkeys));
+                hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+                hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(mf->nkeys, 256)), dim3(256), 0, ctx->stream,
+                                   mr->node_row, mf->nkeys, -1);
+                hipLaunchKernelGGL(k_node_row, dim3(blocks_for(mr->nkeys, 256)), dim3(256), 0, ctx->stream,
+                                   mr->row_node, mr->nkeys, mr->node_row, d_err);
+                err = 1;
+                HIPCHK(ctx, hipGetLastError());
+                HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+                mr->merge_ok = err == 0;
+            }
         }
     }
     *ok = mr->tsum_ok;
@@ -865,6 +886,38 @@ bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap 
         else if (iso) ICF_VF(false, true, double, "k_vals_force<fp64,iso>");
         else ICF_VF(false, false, double, "k_vals_force<fp64,gen>");
 #undef ICF_VF
+    });
+    tune_end(ctx, false, !mixed);
+    return true;
+}
+
+// The second half of the fused call as one launch: node forces with the rhs rows, the CZ
+// counters and the BCSR values (overwrite).  False unless the rhs map's rows each have their
+// tet node (merge_ok) and the value map is an overwrite map.
+bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap *mf, double *F, int ns,
+                          const SlotMap *mr, double *b, const int *dmg, const int *fld, int nCZs, int *counts,
+                          const Stage &st)
+{
+    if (mv->keys || !mr->merge_ok || !mr->node_row) return false;
+    if (mv->nlaunch == 0 && mf->nlaunch == 0 && nCZs == 0) return true;
+    const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
+    const int k = mixed ? 0 : 1;
+    const int c = tune_begin(ctx, false, !mixed);
+    with_keys(c, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
+        const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
+        const int nbc = nCZs ? round8(std::min<long long>(blocks_for(nCZs, K), 64)) : 0;
+        const dim3 g(nbf + nbc + nch), bl(K);   // not empty: something has keys or CZs
+#define ICF_VFR(MX, IS, L, NAME)                                                                                   \
+    LAUNCH(ctx, NAME, (k_vals_force_rhs_fused<MX, IS, L, K>), g, bl, nbf, nbc, mv->off, mv->codes, mv->chunks[c][k], \
+           nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, mr->node_row, mr->off, mr->codes, mr->cz_begin, \
+           b, dmg, fld, nCZs, counts, st)
+        if (mixed && iso) ICF_VFR(true, true, float, "k_vals_force_rhs<mixed,iso>");
+        else if (mixed) ICF_VFR(true, false, float, "k_vals_force_rhs<mixed,gen>");
+        else if (iso) ICF_VFR(false, true, double, "k_vals_force_rhs<fp64,iso>");
+        else ICF_VFR(false, false, double, "k_vals_force_rhs<fp64,gen>");
+#undef ICF_VFR
     });
     tune_end(ctx, false, !mixed);
     return true;
@@ -1701,7 +1754,9 @@ static int assemble_impl(icf_ctx *ctx, const int *ie, int nElems, int el_stride,
     }
     Stage st = make_stage(ctx, h);
     double *F = dn + (size_t)nd_stride * ND_F;
-    if (tsum) {
+    if (tsum && fuse && vals_force_rhs_fused(ctx, mv, A, mf, F, nd_stride, mr, b, dmg, fld, nCZs, counts, st)) {
+        // done: node forces with the rhs rows, the counts and the BCSR values in one launch
+    } else if (tsum) {
         // BCSR values and node forces (keeping each node's tet sum), then the rhs rows from
         // those sums plus their CZ terms, and the counts
         if (!(fuse && vals_force_fused(ctx, mv, A, mf, F, nd_stride, st))) {

@@ -338,6 +338,55 @@ __device__ __forceinline__ void reduce_vec3(int bid_, int nb_, const int *__rest
     }
 }
 
+// Node forces and rhs rows of the fused call in one pass, one thread per node key: F[node] gets
+// the node's tet contributions added in list order (as reduce_vec3<true, TSUM>), and the node's
+// rhs row -- node_row[node], when not negative -- is the sum of the same contributions from zero
+// followed by the row's CZ contributions in list order (as reduce_rhs_tsum from that tet sum).
+// Used when every rhs row's tet contributions are its node's force list and every row has such
+// a node (the host checks both once per pair of maps).
+__device__ __forceinline__ void reduce_force_rhs(int bid_, int nb_, const int *__restrict__ foff,
+        const uint32_t *__restrict__ fcodes, const int *__restrict__ fkeys, int nfkeys, double *__restrict__ F,
+        int ns, const int *__restrict__ node_row, const int *__restrict__ roff, const uint32_t *__restrict__ rcodes,
+        const int *__restrict__ cz_begin, double *__restrict__ rhs, Stage sv)
+{
+    const int t = xcd_remap(bid_, nb_) * blockDim.x + threadIdx.x;
+    if (t >= nfkeys) return;
+    const int s = fkeys ? fkeys[t] : t;
+    const int b = foff[s], e = foff[s + 1];
+    const int r = node_row[s];
+    double *dst = F + s;
+    double a0 = dst[0], a1 = dst[ns], a2 = dst[2 * (size_t)ns], t0 = 0., t1 = 0., t2 = 0.;
+    int c = b;
+    for (; c + 4 <= e; c += 4) {          // loads of four contributions in flight, added in order
+        const double *r0 = rhs_of(sv, fcodes[c]), *r1 = rhs_of(sv, fcodes[c + 1]);
+        const double *r2 = rhs_of(sv, fcodes[c + 2]), *r3 = rhs_of(sv, fcodes[c + 3]);
+        const double x0 = r0[0], y0 = r0[1], z0 = r0[2], x1 = r1[0], y1 = r1[1], z1 = r1[2];
+        const double x2 = r2[0], y2 = r2[1], z2 = r2[2], x3 = r3[0], y3 = r3[1], z3 = r3[2];
+        a0 += x0; a1 += y0; a2 += z0;
+        a0 += x1; a1 += y1; a2 += z1;
+        a0 += x2; a1 += y2; a2 += z2;
+        a0 += x3; a1 += y3; a2 += z3;
+        t0 += x0; t1 += y0; t2 += z0;
+        t0 += x1; t1 += y1; t2 += z1;
+        t0 += x2; t1 += y2; t2 += z2;
+        t0 += x3; t1 += y3; t2 += z3;
+    }
+    for (; c < e; c++) {
+        const double *q = rhs_of(sv, fcodes[c]);
+        a0 += q[0]; a1 += q[1]; a2 += q[2];
+        t0 += q[0]; t1 += q[1]; t2 += q[2];
+    }
+    dst[0] = a0; dst[ns] = a1; dst[2 * (size_t)ns] = a2;
+    if (r < 0) return;
+    const int re = roff[r + 1];
+    for (int k = cz_begin[r]; k < re; k++) {
+        const double *q = rhs_of(sv, rcodes[k]);
+        t0 += q[0]; t1 += q[1]; t2 += q[2];
+    }
+    double *o = rhs + (size_t)r * 3;
+    o[0] = t0; o[1] = t1; o[2] = t2;
+}
+
 template <bool ACC, bool TSUM = false>
 __global__ __launch_bounds__(256) void k_reduce_vec3(const int *__restrict__ off, const uint32_t *__restrict__ codes,
                                                      const int *__restrict__ keys, int nkeys, double *__restrict__ out,
@@ -380,6 +429,17 @@ __global__ __launch_bounds__(256) void k_rhs_check(const int *__restrict__ roff,
             return;
         }
     row_node[r] = n;
+}
+
+// node_row[row_node[r]] = r for every rhs row r (node_row filled with -1 before).  A row
+// without a tet node, or a node with two rows, sets *err.
+__global__ __launch_bounds__(256) void k_node_row(const int *__restrict__ row_node, int N, int *__restrict__ node_row,
+                                                  int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int n = row_node[r];
+    if (n < 0 || atomicExch(node_row + n, r) != -1) atomicOr(err, 1);
 }
 
 // rhs rows of the fused call: the node's tet sum (k_reduce_vec3 TSUM), then the row's CZ
@@ -468,6 +528,37 @@ __device__ __forceinline__ void count_flags(int bid_, int nb_, const int *__rest
     d = block_sum256(d);
     f = block_sum256(f);
     if (threadIdx.x == 0) {
+        if (d) atomicAdd(out, d);
+        if (f) atomicAdd(out + 1, f);
+    }
+}
+
+// count_flags for workgroups of NT threads (a multiple of 64, at most 1024).
+template <int NT>
+__device__ __forceinline__ void count_flags_n(int bid_, int nb_, const int *__restrict__ damaged,
+        const int *__restrict__ failed, int n, int *__restrict__ out)
+{
+    __shared__ int part[2][NT / 64];
+    int d = 0, f = 0;
+    for (int i = bid_ * NT + threadIdx.x; i < n; i += nb_ * NT) {
+        d += damaged[i];
+        f += failed[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        d += __shfl_xor(d, o);
+        f += __shfl_xor(f, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = d;
+        part[1][threadIdx.x >> 6] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        d = f = 0;
+        for (int w = 0; w < NT / 64; w++) {
+            d += part[0][w];
+            f += part[1][w];
+        }
         if (d) atomicAdd(out, d);
         if (f) atomicAdd(out + 1, f);
     }
