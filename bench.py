@@ -470,6 +470,9 @@ def main():
                     help="compression: the 40^3 cube with platens and contact; batched: --specimens cubes, "
                          "lognormal sizes (seed 2), no contact")
     ap.add_argument("--specimens", type=int, default=256)
+    ap.add_argument("--newton-damage-steps", type=int, default=26,
+                    help="simulated time (in 1e-4 s steps) of the FP64-vs-mixed three-point-bend run into fracture "
+                         "(2 mm deflection ramped over 30 steps, the reference's time-step control; 0: skip)")
     ap.add_argument("--newton-steps", type=int, default=10,
                     help="steps of the FP64-vs-mixed three-point-bend Newton run (0: skip)")
     ap.add_argument("--no-overlap", action="store_true",
@@ -582,6 +585,20 @@ def main():
         study = newton.precision_study(consts, steps=args.newton_steps, log=log)
         study["wall_s"] = round(time.time() - t0, 1)
         log(f"precision study: {study}")
+    study_damage = None
+    if args.newton_damage_steps > 0:
+        from icflow_amd import newton
+        t0 = time.time()
+        last = [t0]
+
+        def every_15s(msg):                 # progress of a two-minute run, a line at a time
+            if time.time() - last[0] > 15:
+                last[0] = time.time()
+                log(msg)
+        study_damage = newton.precision_study(consts, steps=args.newton_damage_steps, ramp_steps=30, adapt=True,
+                                              log=every_15s)
+        study_damage["wall_s"] = round(time.time() - t0, 1)
+        log(f"precision study into fracture: {study_damage}")
     cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -624,6 +641,7 @@ def main():
         "hbm_copy_probe": bw,
         "device_broad_phase": bp,
         "precision_study": study,
+        "precision_study_fracture": study_damage,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -14,8 +14,14 @@ Per step:
 The linear solve stands in for MKL PARDISO (LinearSystem.Solve, LinearSystem.cs:45-64,
 mtype -2 on the upper-triangular BCSR): the symmetric matrix is expanded from the upper
 blocks and factored with SuperLU (scipy.sparse.linalg.splu).  It is a host substitute and not
-PARDISO.  The time step is fixed (the reference's step-size adaptation, :258-288, is not
-restated), so every step is accepted.
+PARDISO.
+
+Time-step control (StepParams.adapt): the reference's _checkDamage (:198-211), _adjustTimeStep
+(:258-288), the acceptance rule of Step (:415-431) and FrameInfo.IncrementTime
+(FrameInfo.cs:159-175).  An attempt runs at h0 / tsf (tsf a power of two up to PARTS); an
+attempt whose Newton loop diverges, does not converge, or fails or damages more than 1% of the
+initial CZs is discarded and retried at twice the factor, and after converged steps the factor
+is halved again.  Without it every attempt is accepted at h0.
 """
 import ctypes
 from dataclasses import dataclass, field
@@ -24,6 +30,10 @@ import numpy as np
 
 from . import _lib
 from .gpu_functionality import GPUFunctionality, stride_of
+
+
+PARTS = 1024 * 32      # FrameInfo.Parts (= Factor2): the finest step is h0 / PARTS
+HOLD_FACTOR_DELAY = 4  # ImplicitModel3.holdFactorDelay
 
 
 @dataclass
@@ -36,6 +46,9 @@ class StepParams:
     conv_cutoff: float = 1e-8    # ConvergenceCutoff
     min_iter: int = 3
     max_iter: int = 10
+    max_damage: float = 0.01     # maxDamagePerStep
+    max_fail: float = 0.01       # maxFailPerStep
+    adapt: bool = False          # the reference's time-step control (module docstring)
 
 
 @dataclass
@@ -45,6 +58,9 @@ class StepRecord:
     failed_czs: int
     damaged: int
     norms: list = field(default_factory=list)
+    accepted: bool = True
+    h: float = 0.0
+    explodes: bool = False
 
 
 def upper_bcsr_to_csc(rows, cols, vals):
@@ -162,6 +178,10 @@ class ImplicitModel:
         self.cz_alive = np.arange(len(self.cz_nodes))   # indices into the original CZ list
         self.t = 0.0
         self.records = []
+        # FrameInfo fields of the current frame (time-step control)
+        self.tsf, self.hold, self.int_time, self.frames = 1, 0, 0, 0
+        self.nCZ_initial = len(self.cz_nodes)
+        self.prev_damaged = 0
         self._setup_czs()
 
     # -- MeshCollection.UpdateStaticStructureData + CreateStructure + TransferPCSR --
@@ -205,9 +225,9 @@ class ImplicitModel:
             d[(3 + j) * self.cs:(3 + j) * self.cs + self.C] = tm[:, j]
         self.g_dcz.upload(d)
 
-    def _begin_step(self, h):
-        """_beginStep (ImplicitModel3.cs:117-156): du of anchored nodes = target - u; du = v h
-        for active nodes."""
+    def _begin_step(self, h, t_new):
+        """_beginStep (ImplicitModel3.cs:117-156): du of anchored nodes = target(t_new) - u;
+        du = v h for active nodes."""
         ss = self.st_stride
         st = self.g_st.download(12 * ss)
         u = np.stack([st[k * ss:k * ss + self.n] for k in range(3)], 1)
@@ -216,17 +236,32 @@ class ImplicitModel:
         act = self.alt >= 0
         du[act] = v[act] * h
         if self.prescribed is not None and len(self.anchored_ids):
-            tgt = self.prescribed(self.t + h)
+            tgt = self.prescribed(t_new)
             du[self.anchored_ids] = tgt - u[self.anchored_ids]
         for k in range(3):
             st[(9 + k) * ss:(9 + k) * ss + self.n] = du[:, k]
         self.g_st.upload(st)
 
+    def _check_damage(self, damaged, failed_now):
+        """_checkDamage (ImplicitModel3.cs:198-211): this attempt's new damage or failures above
+        maxDamagePerStep / maxFailPerStep of the initial CZ count."""
+        if self.tsf == PARTS or self.nCZ_initial == 0:
+            return False
+        dn_damaged = (damaged - self.prev_damaged) / self.nCZ_initial
+        dn_failed = failed_now / self.nCZ_initial
+        return dn_damaged > self.p.max_damage or dn_failed > self.p.max_fail
+
     def step(self):
+        """One attempt of ImplicitModel3.Step; its record says whether it was accepted."""
         p, lib, ctx = self.p, self.lib, self.ctx
-        h = p.h
-        self._begin_step(h)
-        it, err0, converged, diverges = 0, None, False, False
+        if p.adapt:
+            ticks = PARTS // self.tsf
+            h = p.h * ticks / PARTS
+            t_new = p.h * (self.int_time + ticks) / PARTS
+        else:
+            h, t_new = p.h, self.t + p.h
+        self._begin_step(h, t_new)
+        it, err0, converged, diverges, explodes = 0, None, False, False, False
         norms = []
         dmg = ctypes.c_int()
         fld = ctypes.c_int()
@@ -243,6 +278,8 @@ class ImplicitModel:
                 ctx, self.g_ie.p, self.T, self.es, self.g_dcz.p if self.C else None,
                 self.g_icz.p if self.C else None, self.C, self.cs, self.g_dn.p, self.ns, h, self.g_vals.p, self.nnz,
                 self.g_rhs.p, self.N, ctypes.byref(dmg), ctypes.byref(fld)), ctx, "assemble")
+            if p.adapt:
+                explodes = self._check_damage(dmg.value, fld.value)
             vals = self.g_vals.download(9 * self.nnz)
             rhs = self.g_rhs.download(3 * self.N)
             dx = self.solver.solve(vals, rhs)
@@ -261,8 +298,33 @@ class ImplicitModel:
             _lib.check(lib.icf_nodes_add_dx(ctx, self.g_st.p, self.st_stride, self.g_alt.p, self.n, self.g_dx.p),
                        ctx, "icf_nodes_add_dx")
             it += 1
-            if not (it < p.min_iter or (not diverges and not converged and it < p.max_iter)):
+            if not (it < p.min_iter or (not explodes and not diverges and not converged and it < p.max_iter)):
                 break
+        accepted = True
+        if p.adapt:
+            # _adjustTimeStep: cf's factor for a retry, tcf0's for the next frame
+            cf_tsf, cf_hold, t_tsf, t_hold = self.tsf, self.hold, self.tsf, self.hold
+            if explodes:
+                cf_tsf, cf_hold = min(cf_tsf * 2, PARTS), HOLD_FACTOR_DELAY
+            elif (diverges or not converged) and t_tsf < PARTS and p.max_iter > 1:
+                cf_tsf, cf_hold = cf_tsf * 2, HOLD_FACTOR_DELAY
+            elif converged and cf_tsf > 1:
+                if t_hold > 0:
+                    t_hold -= 1
+                else:
+                    t_tsf, t_hold = max(1, t_tsf // 2), 2
+            accepted = (p.max_iter == 1 or t_tsf == PARTS or (not converged and t_tsf >= PARTS)
+                        or (not explodes and not diverges and converged))
+            if not accepted:
+                self.tsf, self.hold = cf_tsf, cf_hold
+                rec = StepRecord(it, converged, 0, dmg.value, norms, accepted=False, h=h, explodes=explodes)
+                self.records.append(rec)
+                return rec
+            self.tsf, self.hold = t_tsf, t_hold
+            self.int_time += ticks
+            self.frames += 1
+            if self.frames == 1:
+                self.tsf = 1                            # _acceptFrame (:322)
         _lib.check(lib.icf_nodes_accept(ctx, self.g_st.p, self.st_stride, self.g_dn.p, self.ns, self.n, h), ctx,
                    "icf_nodes_accept")
         failed_now = 0
@@ -277,8 +339,9 @@ class ImplicitModel:
             if failed_now:
                 self.cz_alive = self.cz_alive[~failed]
                 self._setup_czs()
-        self.t += h
-        rec = StepRecord(it, converged, failed_now, dmg.value, norms)
+        self.t = t_new
+        self.prev_damaged = dmg.value
+        rec = StepRecord(it, converged, failed_now, dmg.value, norms, accepted=True, h=h, explodes=explodes)
         self.records.append(rec)
         return rec
 
@@ -323,29 +386,38 @@ def three_point_bend(consts, cells=(40, 8, 8), size=(0.40, 0.05, 0.05), grains=1
     return col, prescribed
 
 
-def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-4, ramp_steps=200, log=None):
-    """The same three-point-bend run in FP64 and in mixed precision: relative displacement
-    error of the mixed run after `steps` steps (normwise over all nodes), the failed-CZ sets,
-    and Newton iterations per step."""
+def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-4, ramp_steps=200, adapt=False,
+                    max_attempts=4000, log=None):
+    """The same three-point-bend run in FP64 and in mixed precision up to time steps * h:
+    relative displacement error of the mixed run at the end (normwise over all nodes), the
+    failed-CZ sets, Newton iterations, and with adapt (the reference's time-step control) the
+    attempts, discarded attempts and the finest step used."""
     out = {}
     runs = {}
     for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
         col, presc = three_point_bend(consts, cells=cells, grains=grains, seed=seed, ramp_steps=ramp_steps, h=h)
-        m = ImplicitModel(col, consts, precision=mode, step=StepParams(h=h), prescribed=presc)
-        for k in range(steps):
+        m = ImplicitModel(col, consts, precision=mode, step=StepParams(h=h, adapt=adapt), prescribed=presc)
+        attempts, frames, max_tsf = 0, 0, 1
+        while attempts < max_attempts and (m.int_time < steps * PARTS if adapt else frames < steps):
+            max_tsf = max(max_tsf, m.tsf)
             rec = m.step()
-            if log and (k + 1) % max(1, steps // 5) == 0:
-                log(f"newton {name}: step {k + 1}/{steps}, {rec.iterations} iterations, "
-                    f"{len(m.failed_set())} CZs failed")
-        runs[name] = (m.displacements(), m.failed_set(), [r.iterations for r in m.records],
-                      [r.converged for r in m.records])
-        out[f"nodes"] = int(m.n)
-        out[f"czs"] = int(len(m.cz_nodes))
+            attempts += 1
+            frames += rec.accepted
+            if log and rec.accepted and (frames % max(1, steps // 5) == 0 or rec.failed_czs):
+                log(f"newton {name}: frame {frames} t={m.t:.6g} (attempt {attempts}, tsf {max_tsf}), "
+                    f"{rec.iterations} iterations, {len(m.failed_set())} CZs failed")
+        runs[name] = (m.displacements(), m.failed_set(), [r.iterations for r in m.records if r.accepted],
+                      [r.converged for r in m.records if r.accepted], m.int_time if adapt else frames, m.t)
+        out[f"attempts_{name}"] = attempts
+        out[f"discarded_{name}"] = attempts - frames
+        out[f"finest_step_{name}"] = h / max_tsf
+        out["nodes"] = int(m.n)
+        out["czs"] = int(len(m.cz_nodes))
         m.close()
-    u64, f64, it64, c64 = runs["fp64"]
-    umx, fmx, itmx, cmx = runs["mixed"]
+    u64, f64, it64, c64, k64, t64 = runs["fp64"]
+    umx, fmx, itmx, cmx, kmx, tmx = runs["mixed"]
     scale = float(np.linalg.norm(u64))
-    out.update(steps=steps, dt=h,
+    out.update(steps=steps, dt=h, adapt=adapt, t_end_fp64=t64, t_end_mixed=tmx, same_end_time=bool(k64 == kmx),
                rel_err_displacement=float(np.linalg.norm(umx - u64) / scale) if scale > 0 else 0.0,
                max_abs_displacement=float(np.abs(u64).max()),
                failed_czs_fp64=int(len(f64)), failed_czs_mixed=int(len(fmx)),
@@ -353,3 +425,24 @@ def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-
                newton_iterations_fp64=int(sum(it64)), newton_iterations_mixed=int(sum(itmx)),
                steps_converged_fp64=int(sum(c64)), steps_converged_mixed=int(sum(cmx)))
     return out
+
+
+def main(argv=None):
+    """python -m icflow_amd.newton [--steps N] [--ramp R] [--adapt]: the precision study as one
+    JSON line (GPU required)."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=10, help="simulated time in units of the initial step h")
+    ap.add_argument("--ramp", type=int, default=200, help="steps of the deflection ramp (2 mm)")
+    ap.add_argument("--adapt", action="store_true", help="the reference's time-step control")
+    ap.add_argument("--max-attempts", type=int, default=4000)
+    a = ap.parse_args(argv)
+    consts = _lib.consts_from_prms(**_lib.SPECIMEN_PRMS)
+    r = precision_study(consts, steps=a.steps, ramp_steps=a.ramp, adapt=a.adapt, max_attempts=a.max_attempts,
+                        log=lambda m: print(m, flush=True))
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
