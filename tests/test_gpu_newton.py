@@ -104,3 +104,30 @@ def test_newton_driver_matches_oracle_loop(orc, specimen_consts):
     print("history differences", rel_err(m.pmax, pmax), rel_err(m.tmax, tmax), flush=True)
     assert rel_err(m.pmax, pmax) <= 1e-9 and rel_err(m.tmax, tmax) <= 1e-9
     m.close()
+
+
+def test_time_step_control_discards_and_refines(specimen_consts):
+    """The reference's time-step control (StepParams.adapt): a deflection ramped fast enough to
+    damage or fail many CZs at once makes attempts explode or diverge; a discarded attempt leaves the
+    accepted state (displacements, failed set, time) as it was, the retry runs at half the
+    step, and the simulated time only advances by accepted steps."""
+    col, presc = newton.three_point_bend(specimen_consts, cells=(20, 4, 4), size=(0.2, 0.025, 0.025), grains=4,
+                                         seed=0, deflection=2e-3, ramp_steps=4, h=1e-4)
+    # thresholds of 5% (the default 1% is less than one CZ of this small beam)
+    m = newton.ImplicitModel(col, specimen_consts, precision=_lib.ICF_FP64,
+                             step=newton.StepParams(h=1e-4, adapt=True, max_damage=0.05, max_fail=0.05),
+                             prescribed=presc)
+    discarded = 0
+    for _ in range(40):
+        u0, f0, k0, tsf0 = m.displacements(), m.failed_set(), m.int_time, m.tsf
+        rec = m.step()
+        assert rec.h == 1e-4 * (newton.PARTS // tsf0) / newton.PARTS
+        if rec.accepted:
+            assert m.int_time == k0 + newton.PARTS // tsf0
+        else:
+            discarded += 1
+            assert np.array_equal(m.displacements(), u0) and np.array_equal(m.failed_set(), f0)
+            assert m.int_time == k0 and m.tsf == 2 * tsf0
+    print("attempts 40, discarded", discarded, "failed", len(m.failed_set()), "t", m.t, "tsf", m.tsf, flush=True)
+    assert 0 < discarded < 40 and m.t > 0
+    m.close()
