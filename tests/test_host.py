@@ -198,3 +198,29 @@ def test_specimen_cz_conventions():
     assert (np.einsum("ij,ij->i", n, cent0 - p.mean(axis=1)) > 0).all()
     # each node in the grid appears once per grain of its tets when it lies on a CZ face
     assert len(col.x0) > 4 * 5 * 4 * 4 / 4
+
+
+def test_batched_specimen_offsets(lib):
+    """specimen_offsets of a batch: each specimen's tets, CZs and active rows form one run, in
+    specimen order, and the rows' BCSR blocks stay inside the specimen's block range (the
+    block-diagonal layout icf_assemble_batched checks on the device)."""
+    from icflow_amd import specimens
+    from icflow_amd.gpu_functionality import CSR
+    col = specimens.lognormal_cubes(4, 3, median_tets=800, min_tets=300, max_tets=3000, sigma=0.9, grains=(3, 6),
+                                    size=0.01)
+    tet, cz, row = specimens.specimen_offsets(col)
+    n = len(col.meshes)
+    assert len(tet) == len(cz) == len(row) == n + 1
+    assert tet[-1] == len(col.elastic) and cz[-1] == len(col.czs) and row[-1] == int((~col.anchored).sum())
+    for s in range(n):
+        assert np.all(col.node_mesh[col.elastic[tet[s]:tet[s + 1]]] == s)
+        assert np.all(col.node_mesh[col.czs[cz[s]:cz[s + 1]]] == s)
+        act = np.flatnonzero((col.node_mesh == s) & ~col.anchored)
+        assert np.array_equal(np.sort(col.alt[act]), np.arange(row[s], row[s + 1]))
+    csr = CSR(lib).build(col.alt[col.elastic], col.alt[col.czs])
+    br = np.repeat(np.arange(csr.N), np.diff(csr.rows))
+    blk = csr.rows[row]
+    for s in range(n):
+        cols = csr.cols[blk[s]:blk[s + 1]]
+        assert np.all((cols >= row[s]) & (cols < row[s + 1])), s
+        assert np.all((br[blk[s]:blk[s + 1]] >= row[s]) & (br[blk[s]:blk[s + 1]] < row[s + 1]))
