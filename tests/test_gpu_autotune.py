@@ -2,12 +2,12 @@
 icf_set_chunk_keys).  The key limit changes how the keys are chunked over workgroups, never the
 order in which a key's contributions are added: the fused iteration with contact gives the same
 bits (BCSR values, rhs rows, node forces, CZ flags) under every candidate of every role, and
-after tuning."""
+after tuning; so do the reference's separate launches (accumulate maps)."""
 import numpy as np
 import pytest
 
 from icflow_amd import _lib
-from workloads import Pipeline, make_state, small_specimen
+from workloads import DIST_EPS, H, PENALTY_K, Pipeline, make_state, small_specimen
 
 pytestmark = pytest.mark.gpu
 
@@ -18,6 +18,25 @@ def _iteration(p):
     gf = p.gf
     gf.g_dn.upload(np.zeros(3 * gf.nd_stride), offset_elems=12 * gf.nd_stride)
     return p.assemble()
+
+
+def _separate(p):
+    """The reference's separate launches (icf_elem_force, icf_cz_force into zeroed arrays, then
+    the collision response): accumulate-map reductions."""
+    gf, lib, col = p.gf, p.gf.lib, p.col
+    ns = gf.nd_stride
+    gf.g_dvals.zero()
+    gf.g_drhs.zero()
+    gf.g_dn.upload(np.zeros(3 * ns), offset_elems=12 * ns)
+    _lib.check(lib.icf_elem_force(gf.ctx, gf.g_ie_pcsr.p, gf.g_dn.p, H, gf.g_dvals.p, gf.g_drhs.p, len(col.elastic),
+                                  gf.el_elastic_stride, ns), gf.ctx, "icf_elem_force")
+    _lib.check(lib.icf_cz_force(gf.ctx, gf.g_dcz.p, gf.g_icz.p, gf.g_dn.p, gf.g_dvals.p, gf.g_drhs.p, H,
+                                len(col.czs), gf.cz_stride, ns), gf.ctx, "icf_cz_force")
+    _lib.check(lib.icf_collision_response(gf.ctx, gf.g_icr.p, gf.g_dn.p, H, gf.g_dvals.p, gf.g_drhs.p, gf.nImpacts,
+                                          gf.cr_stride, ns, PENALTY_K, DIST_EPS), gf.ctx, "icf_collision_response")
+    vals, rhs = gf.TransferLinearSystemToHost()
+    out = gf.TransferUpdatedStateToHost()
+    return dict(vals=vals, rhs=rhs, F=out["F"], icz=out["icz"])
 
 
 def _same(a, b, what):
@@ -41,6 +60,10 @@ def test_every_key_limit_gives_the_same_bits(specimen_consts, precision):
             gf.set_chunk_keys(acc, fp64, keys)
         assert gf.autotune_state()[f"overwrite,{'fp64' if fp64 else 'fp32'}"][0] == keys
         _same(_iteration(p), ref, keys)
+    # the separate single-kernel launches (accumulate maps) under every key limit
+    for keys in CANDS:
+        gf.set_chunk_keys(1, fp64, keys)
+        _same(_separate(p), ref, ("separate", keys))
     # mixed choices (overwrite and accumulate maps tuned apart)
     gf.set_chunk_keys(0, fp64, 64)
     gf.set_chunk_keys(1, fp64, 512)
