@@ -488,7 +488,7 @@ def main():
                          "calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}).",
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (requested --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 

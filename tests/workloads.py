@@ -5,7 +5,7 @@ import numpy as np
 from icflow_amd import _lib, specimens
 from icflow_amd.gpu_functionality import GPUFunctionality, broad_phase
 
-H = 1e-4            # time step of the three-point-bend specimen
+H = 1e-4            # time step of the three-point-bend beam (0.40 x 0.05 x 0.05 m, 15,360 tets)
 PENALTY_K = 1e11 * (0.1 / 40) ** 2   # 1e11 N/m^3 times the platen cell area (N/m per pair)
 DIST_EPS = 1e-15    # ModelPrms.DistanceEpsilon default
 
