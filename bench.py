@@ -474,6 +474,9 @@ def main():
     ap.add_argument("--newton-damage-steps", type=int, default=26,
                     help="simulated time (in 1e-4 s steps) of the FP64-vs-mixed three-point-bend run into fracture "
                          "(2 mm deflection ramped over 30 steps, the reference's time-step control; 0: skip)")
+    ap.add_argument("--newton-contact-attempts", type=int, default=200,
+                    help="attempts of the FP64-vs-mixed Newton run with contact (a cube between rigid platens, "
+                         "the reference's time-step control; 0: skip)")
     ap.add_argument("--newton-steps", type=int, default=10,
                     help="steps of the FP64-vs-mixed three-point-bend Newton run (0: skip)")
     ap.add_argument("--no-overlap", action="store_true",
@@ -603,6 +606,14 @@ def main():
                                               log=every_15s)
         study_damage["wall_s"] = round(time.time() - t0, 1)
         log(f"precision study into fracture: {study_damage}")
+    study_contact = None
+    if args.newton_contact_attempts > 0:
+        from icflow_amd import newton
+        t0 = time.time()
+        study_contact = newton.contact_precision_study(consts, steps=20, adapt=True,
+                                                       max_attempts=args.newton_contact_attempts)
+        study_contact["wall_s"] = round(time.time() - t0, 1)
+        log(f"precision study with contact: {study_contact}")
     cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -647,6 +658,7 @@ def main():
         "device_broad_phase": bp,
         "precision_study": study,
         "precision_study_fracture": study_damage,
+        "precision_study_contact": study_contact,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

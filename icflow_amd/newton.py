@@ -139,7 +139,8 @@ class ImplicitModel:
     anchored deformable nodes, the target displacement (n_anchored, 3) at time t (the role
     of the anchored SurfaceFragments' dx, dy, dz with their application time)."""
 
-    def __init__(self, col, consts, precision=_lib.ICF_FP64, step=None, prescribed=None):
+    def __init__(self, col, consts, precision=_lib.ICF_FP64, step=None, prescribed=None, prescribed_nodes=None,
+                 contact=False, penalty=None, dist_eps=1e-15):
         self.col, self.consts = col, consts
         self.p = step or StepParams()
         self.prescribed = prescribed
@@ -150,6 +151,14 @@ class ImplicitModel:
         n = len(col.x0)
         self.n = n
         self.ns = stride_of(n)
+        # contact (ImplicitModel3.cs:386-398): the collection's surface tets, adjacent faces and
+        # faces on the device; candidate pairs, narrow phase, NL2set, closest face and the
+        # impact records are all device calls each iteration
+        self.contact = contact
+        self.penalty, self.dist_eps = penalty, dist_eps
+        self.nImpacts = 0
+        if contact:
+            gf.TransferStaticDataToDevice(col)
         # node buffer: x0 (x_current follows from the state)
         dn = np.zeros(18 * self.ns)
         for k in range(3):
@@ -163,14 +172,15 @@ class ImplicitModel:
         self.g_alt = gf.buf("alt", max(1, n), np.int32)
         self.g_alt.upload(self.alt)
         self.N = int(self.alt.max()) + 1
-        self.anchored_ids = np.flatnonzero(col.anchored & (col.node_mesh == 0))
+        self.anchored_ids = (np.flatnonzero(col.anchored & (col.node_mesh == 0)) if prescribed_nodes is None
+                             else np.asarray(prescribed_nodes, np.int64))
         # elements (static), CZs (shrinks as they fail)
         T = len(col.elastic)
         self.T, self.es = T, stride_of(T)
         ie = np.zeros(28 * self.es, np.int32)
         for i in range(4):
             ie[i * self.es:i * self.es + T] = col.elastic[:, i]
-        self.g_ie = gf.buf("ie", ie.size, np.int32)
+        self.g_ie = gf.buf("ie_el", ie.size, np.int32)
         self.g_ie.upload(ie)
         self.cz_nodes = np.asarray(col.czs, np.int64)
         self.pmax = np.zeros((len(self.cz_nodes), 3))
@@ -214,6 +224,77 @@ class ImplicitModel:
         self.g_vals = gf.buf("vals", 9 * self.nnz, np.float64)
         self.g_rhs = gf.buf("rhs", 3 * self.N, np.float64)
         self.g_dx = gf.buf("dx", 3 * self.N, np.float64)
+
+    def _detect_contacts(self):
+        """bvh.ConstructAndTraverse + NarrowPhaseCollisionDetection + the impact packing of
+        collisionResponse (ImplicitModel3.cs:389-391, GPU_Functionality.cs:539-616,627-665), on
+        the device: icf_broad_phase, icf_narrow_phase on a copy of the pairs, icf_nl2set,
+        icf_find_closest_face, and the impacts' node rows copied into icr."""
+        gf, lib, ctx = self.gf, self.lib, self.ctx
+        nS, es = len(self.col.surf), gf.el_all_stride
+        n = ctypes.c_longlong()
+        cap = getattr(self, "_pair_cap", 1024)
+        while True:
+            d_pairs = gf.buf("nw_pairs", 2 * cap, np.int32)
+            r = lib.icf_broad_phase(ctx, self.g_dn.p, self.ns, gf.g_ie.p, es, nS, 1, d_pairs.p, cap, ctypes.byref(n))
+            if r == _lib.ICF_E_CAPACITY:
+                cap = int(n.value * 1.25) + 1024
+                continue
+            _lib.check(r, ctx, "icf_broad_phase")
+            break
+        self._pair_cap = cap
+        Q = int(n.value)
+        self.nImpacts = 0
+        if Q == 0:
+            return
+        d_list = gf.buf("nw_list", 2 * cap, np.int32)
+        _lib.check(lib.icf_d2d(ctx, d_list.p, 0, d_pairs.p, 0, 8 * Q), ctx, "icf_d2d")
+        _lib.check(lib.icf_narrow_phase(ctx, Q, self.g_dn.p, gf.g_ie.p, d_list.p, es, self.ns), ctx,
+                   "icf_narrow_phase")
+        nt = ctypes.c_int()
+        ts = getattr(self, "_tuple_stride", stride_of(1024))
+        while True:
+            d_nl2 = gf.buf("nw_nl2", 4 * ts, np.int32)
+            r = lib.icf_nl2set(ctx, d_list.p, d_pairs.p, Q, gf.g_ie.p, es, d_nl2.p, ts, ctypes.byref(nt))
+            if r == _lib.ICF_E_CAPACITY:
+                ts = stride_of(int(nt.value * 1.25) + 1024)
+                continue
+            _lib.check(r, ctx, "icf_nl2set")
+            break
+        self._tuple_stride = ts
+        nT = int(nt.value)
+        if nT == 0:
+            return
+        _lib.check(lib.icf_find_closest_face(ctx, nT, ts, d_nl2.p, self.g_dn.p, gf.g_ie.p, gf.g_ifc.p, self.ns, es,
+                                             gf.fc_stride), ctx, "icf_find_closest_face")
+        # icr: rows 0-3 the impact's nodes (point, face), slots and rows packed with the pattern
+        self.cr_stride = ts
+        self.g_icr = gf.buf("nw_icr", 28 * ts, np.int32)
+        _lib.check(lib.icf_d2d(ctx, self.g_icr.p, 0, d_nl2.p, 0, 16 * ts), ctx, "icf_d2d")
+        self.nImpacts = nT
+
+    def _pattern_with_impacts(self):
+        """CreateStructure with the colliding nodes and TransferPCSR of every unit (elements,
+        CZs, impacts) against it, on the device; the host solver is set up for the pattern."""
+        gf, lib, ctx = self.gf, self.lib, self.ctx
+        C, nT = self.C, self.nImpacts
+        icr = self.g_icr.p if nT else None
+        cr = self.cr_stride if nT else 0
+        cap = 10 * self.T + 21 * C + 6 * nT + 1
+        self.g_cols = gf.buf("cols", cap, np.int32)
+        nnz = ctypes.c_int()
+        _lib.check(lib.icf_build_pattern(ctx, self.g_alt.p, self.n, self.g_ie.p, self.T, self.es,
+                                         self.g_icz.p if C else None, C, self.cs, icr, nT, cr, self.N, self.g_rows.p,
+                                         self.g_cols.p, cap, ctypes.byref(nnz)), ctx, "icf_build_pattern")
+        _lib.check(lib.icf_pack_pcsr(ctx, self.g_alt.p, self.n, self.g_rows.p, self.g_cols.p, self.N, self.g_ie.p,
+                                     self.T, self.es, self.g_icz.p if C else None, C, self.cs, icr, nT, cr), ctx,
+                   "icf_pack_pcsr")
+        rows = self.g_rows.download(self.N + 1)
+        cols = self.g_cols.download(nnz.value)
+        if nnz.value != self.nnz or not np.array_equal(rows, self.rows) or not np.array_equal(cols, self.cols):
+            self.nnz, self.rows, self.cols = nnz.value, rows, cols
+            self.solver = UpperBcsrSolver(rows, cols)
+            self.g_vals = gf.buf("vals", 9 * self.nnz, np.float64)
 
     def _upload_history(self):
         if not self.C:
@@ -268,7 +349,11 @@ class ImplicitModel:
         while True:
             _lib.check(lib.icf_nodes_infer(ctx, self.g_st.p, self.st_stride, self.g_dn.p, self.ns, self.n, h, p.beta,
                                            p.gamma), ctx, "icf_nodes_infer")
-            if self.C:
+            if self.contact:
+                self._detect_contacts()
+                self._upload_history()
+                self._pattern_with_impacts()
+            elif self.C:
                 # TransferPCSR, each iteration: current history in, tentative fields and the CZ
                 # flags cleared (GPU_Functionality.cs:376-412)
                 self._upload_history()
@@ -278,6 +363,10 @@ class ImplicitModel:
                 ctx, self.g_ie.p, self.T, self.es, self.g_dcz.p if self.C else None,
                 self.g_icz.p if self.C else None, self.C, self.cs, self.g_dn.p, self.ns, h, self.g_vals.p, self.nnz,
                 self.g_rhs.p, self.N, ctypes.byref(dmg), ctypes.byref(fld)), ctx, "assemble")
+            if self.contact and self.nImpacts:
+                _lib.check(lib.icf_collision_response(ctx, self.g_icr.p, self.g_dn.p, h, self.g_vals.p, self.g_rhs.p,
+                                                      self.nImpacts, self.cr_stride, self.ns, self.penalty,
+                                                      self.dist_eps), ctx, "icf_collision_response")
             if p.adapt:
                 explodes = self._check_damage(dmg.value, fld.value)
             vals = self.g_vals.download(9 * self.nnz)
@@ -386,6 +475,69 @@ def three_point_bend(consts, cells=(40, 8, 8), size=(0.40, 0.05, 0.05), grains=1
     return col, prescribed
 
 
+def platen_compression(consts, cells=(8, 8, 8), size=(0.02, 0.02, 0.02), grains=8, seed=0, rate=5.0):
+    """A cube between two rigid platens of its own footprint, one cell thick, meshed with 4/5 of
+    the cube's cells per side (as 32 for 40) so that their vertices do not fall on the cube's
+    tet faces: the bottom platen fixed, the top one moving down at rate * size_z per second,
+    the specimen free (it meets both through point-triangle contact, and its grains meet each
+    other).  Returns the collection, the prescribed displacement of the platen nodes at time t,
+    those nodes, and the penalty stiffness per impact (1e11 N/m^3 times the platen cell
+    area)."""
+    from . import specimens
+    pc = (max(1, cells[0] * 4 // 5), max(1, cells[1] * 4 // 5))
+    col = specimens.make_specimen(cells, size, grains, seed, platens=pc).finalize()
+    nodes = np.flatnonzero(col.node_mesh > 0)
+    top = col.node_mesh[nodes] == 2
+    v = rate * size[2]
+
+    def prescribed(t):
+        out = np.zeros((len(nodes), 3))
+        out[top, 2] = -v * t
+        return out
+
+    return col, prescribed, nodes, 1e11 * (size[0] / pc[0]) * (size[1] / pc[1])
+
+
+def contact_precision_study(consts, steps=20, h=1e-4, adapt=True, max_attempts=400, log=None, **scenario):
+    """platen_compression in FP64 and in mixed precision up to time steps * h: relative
+    displacement error at the end, the failed-CZ sets, impacts and Newton iterations."""
+    out = {}
+    runs = {}
+    for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
+        col, presc, nodes, k = platen_compression(consts, **scenario)
+        m = ImplicitModel(col, consts, precision=mode, step=StepParams(h=h, adapt=adapt), prescribed=presc,
+                          prescribed_nodes=nodes, contact=True, penalty=k)
+        attempts, frames, impacts = 0, 0, []
+        while attempts < max_attempts and (m.int_time < steps * PARTS if adapt else frames < steps):
+            rec = m.step()
+            attempts += 1
+            if rec.accepted:
+                frames += 1
+                impacts.append(m.nImpacts)
+                if log:
+                    log(f"contact newton {name}: frame {frames} t={m.t:.6g} impacts {m.nImpacts}, "
+                        f"{rec.iterations} iterations, {len(m.failed_set())} CZs failed")
+        runs[name] = (m.displacements(), m.failed_set(), m.int_time if adapt else frames, m.t,
+                      sum(r.iterations for r in m.records if r.accepted))
+        out[f"attempts_{name}"] = attempts
+        out[f"impacts_last_{name}"] = impacts[-1] if impacts else 0
+        out[f"impacts_max_{name}"] = max(impacts) if impacts else 0
+        out["nodes"] = int(m.n)
+        out["tets"] = int(m.T)
+        out["czs"] = int(len(m.cz_nodes))
+        m.close()
+    u64, f64, k64, t64, it64 = runs["fp64"]
+    umx, fmx, kmx, tmx, itmx = runs["mixed"]
+    spec = np.flatnonzero(col.node_mesh == 0)
+    scale = float(np.linalg.norm(u64[spec]))
+    out.update(steps=steps, dt=h, adapt=adapt, t_end_fp64=t64, t_end_mixed=tmx, same_end_time=bool(k64 == kmx),
+               rel_err_displacement=float(np.linalg.norm(umx[spec] - u64[spec]) / scale) if scale > 0 else 0.0,
+               failed_czs_fp64=int(len(f64)), failed_czs_mixed=int(len(fmx)),
+               failed_set_identical=bool(np.array_equal(f64, fmx)),
+               newton_iterations_fp64=int(it64), newton_iterations_mixed=int(itmx))
+    return out
+
+
 def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-4, ramp_steps=200, adapt=False,
                     max_attempts=4000, log=None):
     """The same three-point-bend run in FP64 and in mixed precision up to time steps * h:
@@ -437,10 +589,16 @@ def main(argv=None):
     ap.add_argument("--ramp", type=int, default=200, help="steps of the deflection ramp (2 mm)")
     ap.add_argument("--adapt", action="store_true", help="the reference's time-step control")
     ap.add_argument("--max-attempts", type=int, default=4000)
+    ap.add_argument("--contact", action="store_true",
+                    help="platen compression with contact (platen_compression) instead of the three-point bend")
     a = ap.parse_args(argv)
     consts = _lib.consts_from_prms(**_lib.SPECIMEN_PRMS)
-    r = precision_study(consts, steps=a.steps, ramp_steps=a.ramp, adapt=a.adapt, max_attempts=a.max_attempts,
-                        log=lambda m: print(m, flush=True))
+    if a.contact:
+        r = contact_precision_study(consts, steps=a.steps, adapt=a.adapt, max_attempts=a.max_attempts,
+                                    log=lambda m: print(m, flush=True))
+    else:
+        r = precision_study(consts, steps=a.steps, ramp_steps=a.ramp, adapt=a.adapt, max_attempts=a.max_attempts,
+                            log=lambda m: print(m, flush=True))
     print(json.dumps(r), flush=True)
 
 

@@ -131,3 +131,45 @@ def test_time_step_control_discards_and_refines(specimen_consts):
     print("attempts 40, discarded", discarded, "failed", len(m.failed_set()), "t", m.t, "tsf", m.tsf, flush=True)
     assert 0 < discarded < 40 and m.t > 0
     m.close()
+
+
+def test_newton_with_contact(specimen_consts):
+    """The Newton driver with contact (platen_compression: a free cube between rigid platens,
+    the top one moving): every iteration's candidate pairs, narrow phase, NL2set, closest face
+    and impact records are device calls.  The impacts of the last iteration equal the host
+    pipeline's (k-DOP grid, narrow phase, host NL2set, closest face) on the same positions, two
+    FP64 runs give the same bits, and the mixed run stays within 1e-5 of FP64 with the same
+    impact count."""
+    from icflow_amd.gpu_functionality import broad_phase
+    steps = 3
+
+    def run(mode):
+        col, presc, nodes, k = newton.platen_compression(specimen_consts, cells=(6, 6, 6), size=(0.015,) * 3,
+                                                         grains=5, seed=2)
+        m = newton.ImplicitModel(col, specimen_consts, precision=mode, step=newton.StepParams(h=1e-4),
+                                 prescribed=presc, prescribed_nodes=nodes, contact=True, penalty=k)
+        for _ in range(steps):
+            m.step()
+        return m
+
+    m = run(_lib.ICF_FP64)
+    gf = m.gf
+    assert m.nImpacts > 0
+    got = np.stack([m.g_icr.download(4 * m.cr_stride)[r * m.cr_stride:r * m.cr_stride + m.nImpacts]
+                    for r in range(4)], 1)
+    gf.c_dn = m.g_dn.download(18 * m.ns)
+    pairs = broad_phase(gf, gf.c_dn, m.ns, gf.c_ie, gf.el_all_stride, len(m.col.surf))
+    host = gf.NarrowPhaseCollisionDetection(pairs)
+    print("impacts", m.nImpacts, "host", len(host), flush=True)
+    assert np.array_equal(got, host)
+    u64 = m.displacements()
+    m.close()
+    m2 = run(_lib.ICF_FP64)
+    assert np.array_equal(m2.displacements(), u64)
+    m2.close()
+    mx = run(_lib.ICF_MIXED)
+    spec = np.flatnonzero(mx.col.node_mesh == 0)
+    err = np.linalg.norm(mx.displacements()[spec] - u64[spec]) / np.linalg.norm(u64[spec])
+    print("mixed vs fp64", err, "impacts", mx.nImpacts, flush=True)
+    assert err <= 1e-5 and mx.nImpacts == len(host)
+    mx.close()
