@@ -120,8 +120,12 @@ __global__ __launch_bounds__(256) void k_map_compact(const int *__restrict__ fla
 }
 
 // Chunk limits of the staged BCSR-value reduction: keys per workgroup (one thread each) and
-// contributions staged in LDS, 3 FP32 or 1.5 FP64 values x 9 per key (27 KB for 256 keys,
-// so that five such workgroups share a CU).  The key limit is the workgroup size; a value map
+// contributions staged in LDS, 2 FP32 or 1.5 FP64 values x 9 per key (18 KB and 27 KB for
+// 256 keys).  FP32: the least that still holds the chunk's output blocks (9 doubles per key)
+// when they leave through the same LDS; chunks end on the contribution limit (keys average
+// about 5 contributions), and the small buffer lets more workgroups share a CU (3 per key
+// measured 4% slower, 4 and 5 slower still).  FP64: 1 per key measured 4% slower than 1.5.
+// The key limit is the workgroup size; a value map
 // has a chunk table for every candidate limit and staging type, and the library's autotuner
 // picks the limit per role (overwrite or accumulate map, FP32 or FP64 staging) by timing the
 // candidates.  Every limit gives the same values: each key's contributions are added in list
@@ -130,8 +134,8 @@ constexpr int NKC = 4;                                   // candidate key limits
 constexpr int KEY_CANDS[NKC] = {64, 128, 256, 512};
 constexpr int KC_DEFAULT = 2;                            // 256 keys until tuned
 template <typename L, int KEYS>
-__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? 3 * KEYS : 3 * KEYS / 2; }
-__host__ __device__ constexpr int flat_cap_of(bool fp32, int keys) { return fp32 ? 3 * keys : 3 * keys / 2; }
+__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? 2 * KEYS : 3 * KEYS / 2; }
+__host__ __device__ constexpr int flat_cap_of(bool fp32, int keys) { return fp32 ? 2 * keys : 3 * keys / 2; }
 
 // Staged per-unit records of the current call.
 struct Stage {
