@@ -245,7 +245,7 @@ def autotune(r, rounds, name):
     gf.set_autotune(rounds)
     r.lib.icf_set_blocking(gf.ctx, 1)
     n = 0
-    while n < 4 * rounds * len(gf.KEY_CANDIDATES) + 2:
+    while n < 4 * rounds * len(gf.CANDIDATES) + 2:
         r.step()
         n += 1
         state = gf.autotune_state()
@@ -253,7 +253,8 @@ def autotune(r, rounds, name):
             break
     gf.set_autotune(0)
     state = gf.autotune_state()
-    got = {k: dict(keys=state[k][0], best_us={c: round(1e3 * ms, 1) for c, ms in state[k][2].items()})
+    got = {k: dict(keys=state[k][0][0], stage=state[k][0][1],
+                   best_us={f"{c[0]}/{c[1]}": round(1e3 * ms, 1) for c, ms in state[k][2].items()})
            for k in roles}
     log(f"autotune {name} ({n} iterations): {got}")
     return got
@@ -362,8 +363,8 @@ def _rocprof_symbol(name):
 
 
 def key_limit(gf, name):
-    """The workgroup size (key limit) the autotuner chose for a value-reduction kernel, or None
-    for kernels without one."""
+    """The chunk shape (workgroup size, staged contributions per key) the autotuner chose for a
+    value-reduction kernel, or None for kernels without one."""
     base = name.split("<")[0]
     if base not in ("k_vals_force", "k_vals_force_rhs", "k_contact_sums", "k_reduce_vals"):
         return None
@@ -372,11 +373,11 @@ def key_limit(gf, name):
     return gf.autotune_state()[f"{'accumulate' if acc else 'overwrite'},{stage}"][0]
 
 
-def profiled_traffic(name, keys=None, path=os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")):
+def profiled_traffic(name, shape=None, path=os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-probe factors), or None.  keys: the
-    workgroup size of a kernel templated on it (its last template argument)."""
-    sym = _rocprof_symbol(name) + (f", {keys}>" if keys else "")
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-probe factors), or None.  shape: the
+    (workgroup size, staging) of a kernel templated on them (its last two template arguments)."""
+    sym = _rocprof_symbol(name) + (f", {shape[0]}, {shape[1]}>" if shape else "")
     try:
         for line in open(path):
             head = f"| `{sym}"
@@ -570,7 +571,8 @@ def main():
     # the committed counter summary covers the default compression workload only
     profiled = contact and (args.cells, args.grains, args.seed, args.size, args.platen_cells) == (40, 500, 1, 0.1, 32)
     traffic = profiled_traffic(dname, dkeys) if profiled else None
-    roof = dict(kernel=dname + (f" ({dkeys}-lane workgroups)" if dkeys else ""), bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+    roof = dict(kernel=dname + (f" ({dkeys[0]}-lane workgroups, {dkeys[1]} staged per key)" if dkeys else ""),
+                bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=int(traffic) if traffic else None,
                 alg_bytes=int(dbytes), avg_us=round(avg_s * 1e6, 2),
                 traffic_source="profiles/assembly_compression40_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "

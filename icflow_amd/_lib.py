@@ -86,8 +86,8 @@ _SIGS = {
     "icf_set_profiling": (c_int, [c_void_p, c_int]),
     "icf_set_overlap": (c_int, [c_void_p, c_int]),
     "icf_set_autotune": (c_int, [c_void_p, c_int]),
-    "icf_get_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
-    "icf_set_chunk_keys": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "icf_get_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "icf_set_chunk_keys": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "icf_profile_reset": (c_int, [c_void_p]),
     "icf_profile_count": (c_int, [c_void_p]),
     "icf_profile_entry": (c_int, [c_void_p, c_int, ctypes.c_char_p, c_int, ctypes.POINTER(c_double),

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void k_phase1_fused(int nbt, const int *__rest
 // Node forces with each node's tet sum (k_reduce_vec3<true, true>; workgroups [0, nbf), the
 // short phase first), then the BCSR values of the fused call (overwrite).  Workgroups of KEYS
 // lanes: one key per lane in both phases.
-template <bool MIXED, bool ISO, typename L, int KEYS>
+template <bool MIXED, bool ISO, typename L, int KEYS, int STAGE>
 __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *__restrict__ off,
                                                            const uint32_t *__restrict__ codes,
                                                            const int2 *__restrict__ chunks, int nchunks,
@@ -43,14 +43,14 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *_
     if ((int)blockIdx.x < nbf)
         reduce_vec3<true, true>(blockIdx.x, nbf, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, tsum);
     else
-        reduce_vals<MIXED, ISO, false, L, KEYS>(blockIdx.x - nbf, gridDim.x - nbf, off, codes, nullptr, chunks,
+        reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(blockIdx.x - nbf, gridDim.x - nbf, off, codes, nullptr, chunks,
                                                 nchunks, A, sv);
 }
 
 // Node forces with the rhs rows (reduce_force_rhs; workgroups [0, nbf)), the CZ counters (the
 // next nbc), then the BCSR values of the fused call (overwrite): the whole second half of the
 // fused call as one launch.  Workgroups of KEYS lanes.
-template <bool MIXED, bool ISO, typename L, int KEYS>
+template <bool MIXED, bool ISO, typename L, int KEYS, int STAGE>
 __global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc, const int *__restrict__ off,
         const uint32_t *__restrict__ codes, const int2 *__restrict__ chunks, int nchunks, double *__restrict__ A,
         const int *__restrict__ foff, const uint32_t *__restrict__ fcodes, const int *__restrict__ fkeys, int nfkeys,
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc,
     else if (w < nbf + nbc)
         count_flags_n<KEYS>(w - nbf, nbc, damaged, failed, nc, counts);
     else
-        reduce_vals<MIXED, ISO, false, L, KEYS>(w - nbf - nbc, gridDim.x - nbf - nbc, off, codes, nullptr, chunks,
+        reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(w - nbf - nbc, gridDim.x - nbf - nbc, off, codes, nullptr, chunks,
                                                 nchunks, A, sv);
 }
 
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_rhs_counts_fused(int nbr, const int *__
 
 // The impact sums of the collision response, added into node forces (workgroups [0, nb0)),
 // BCSR blocks (the next nb1) and rhs rows (the rest); KEYS lanes per workgroup, one key each.
-template <bool MIXED, typename L, int KEYS>
+template <bool MIXED, typename L, int KEYS, int STAGE>
 __global__ __launch_bounds__(KEYS) void k_contact_sums(int nb0, int nb1, const int *__restrict__ foff,
                                                        const uint32_t *__restrict__ fcodes,
                                                        const int *__restrict__ fkeys, int nfkeys,
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(KEYS) void k_contact_sums(int nb0, int nb1, const i
     if (w < nb0)
         reduce_vec3<true, false>(w, nb0, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, nullptr);
     else if (w < nb0 + nb1)
-        reduce_vals<MIXED, false, true, L, KEYS>(w - nb0, nb1, voff, vcodes, vkeys, chunks, nchunks, A, sv);
+        reduce_vals<MIXED, false, true, L, KEYS, STAGE>(w - nb0, nb1, voff, vcodes, vkeys, chunks, nchunks, A, sv);
     else
         reduce_vec3<true, false>(w - nb0 - nb1, gridDim.x - nb0 - nb1, roff, rcodes, rkeys, nrkeys, b, 1, 3, sv,
                                  nullptr);

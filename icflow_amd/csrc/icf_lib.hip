@@ -121,10 +121,10 @@ struct SlotMap {
     uint32_t *codes = nullptr;     // ncodes
     int *keys = nullptr;           // accumulate mode: the keys with contributions (null: all keys)
     int nlaunch = 0;               // keys a reduction visits
-    // BCSR-value maps, per candidate key limit KEY_CANDS[c] and staging type (0: FP32 values,
-    // 1: FP64 values): chunk starts (list position, code position)
-    int2 *chunks[NKC][2] = {};
-    int nchunks[NKC][2] = {};
+    // BCSR-value maps, per candidate chunk shape c (cand_keys(c), cand_stage(c)) and staging
+    // type (0: FP32 values, 1: FP64 values): chunk starts (list position, code position)
+    int2 *chunks[NCAND][2] = {};
+    int nchunks[NCAND][2] = {};
     int id = 0;                    // unique per map built
     // rhs maps of the fused call: the force map (by id) whose per-node tet sums seed the rows,
     // the outcome of that check, and per row the node and the first non-tet code
@@ -186,15 +186,15 @@ struct icf_ctx {
     bool blocking = true;          // icf_set_blocking
     bool capturing = false;        // between icf_graph_begin and icf_graph_end
     bool overlap = true;           // icf_set_overlap: independent kernels on more than one stream
-    // key limit (workgroup size) of the staged BCSR-value reductions, per role
+    // chunk shape (workgroup size, staging) of the BCSR-value reductions, per role
     // [accumulate map][FP64 staging]: the candidate in use and, while tuning, the timings
     struct KeyTune {
-        int sel = KC_DEFAULT;      // index into KEY_CANDS
+        int sel = CAND_DEFAULT_FP32;   // candidate (cand_keys, cand_stage); FP64 roles: see icf_create
         bool active = false;       // timing the candidates in turn
         int turn = 0;              // launches started while tuning
         int samples = 0;           // launches timed
         int cand = -1;             // candidate of the launch being timed, -1: none
-        float best[NKC] = {};      // fastest launch per candidate (ms), 0: not timed
+        float best[NCAND] = {};    // fastest launch per candidate (ms), 0: not timed
         hipEvent_t ev[2] = {nullptr, nullptr};
     } tune[2][2];
     int tune_rounds = 0;           // timed launches per candidate (icf_set_autotune)
@@ -262,7 +262,7 @@ void free_map(SlotMap &m)
     m.node_row = nullptr;
     m.merge_ok = false;
     if (m.chunks[0][0]) (void)hipFree(m.chunks[0][0]);   // the start of the one allocation
-    for (int c = 0; c < NKC; c++) m.chunks[c][0] = m.chunks[c][1] = nullptr;
+    for (int c = 0; c < NCAND; c++) m.chunks[c][0] = m.chunks[c][1] = nullptr;
     if (m.snap) (void)hipFree(m.snap);
     m.snap = nullptr;
     m.off = nullptr;
@@ -490,10 +490,10 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         // chunk's first list position and first code position; the last entry closes the table
         auto key_of = [&](int p) { return list ? h_list[p] : p; };
         std::vector<int2> ch;          // every table, one after the other, in one allocation
-        size_t at[2 * NKC];
-        for (int kc = 0; kc < 2 * NKC; kc++) {
-            const int kind = kc & 1, keys = KEY_CANDS[kc >> 1];
-            const int cap = flat_cap_of(kind == 0, keys);
+        size_t at[2 * NCAND];
+        for (int kc = 0; kc < 2 * NCAND; kc++) {
+            const int kind = kc & 1, keys = cand_keys(kc >> 1);
+            const int cap = flat_cap_of(kind == 0, keys, cand_stage(kc >> 1));
             at[kc] = ch.size();
             ch.push_back(make_int2(0, h_off[key_of(0)]));
             int cnt = 0, nk = 0;
@@ -513,7 +513,7 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
         }
         int2 *d = nullptr;
         HIPCHK(ctx, hipMalloc(&d, sizeof(int2) * ch.size()));
-        for (int kc = 0; kc < 2 * NKC; kc++) m.chunks[kc >> 1][kc & 1] = d + at[kc];
+        for (int kc = 0; kc < 2 * NCAND; kc++) m.chunks[kc >> 1][kc & 1] = d + at[kc];
         HIPCHK(ctx, hipMemcpyAsync(d, ch.data(), sizeof(int2) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
@@ -823,9 +823,9 @@ void tune_fold(icf_ctx *ctx, icf_ctx::KeyTune &t)
         }
         t.cand = -1;
     }
-    if (t.active && (t.samples >= ctx->tune_rounds * NKC || t.turn >= 4 * ctx->tune_rounds * NKC)) {
+    if (t.active && (t.samples >= ctx->tune_rounds * NCAND || t.turn >= 4 * ctx->tune_rounds * NCAND)) {
         t.active = false;
-        for (int c = 0; c < NKC; c++)
+        for (int c = 0; c < NCAND; c++)
             if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
     }
 }
@@ -837,7 +837,7 @@ int tune_begin(icf_ctx *ctx, bool acc, bool fp64)
     if (!t.active || ctx->capturing) return t.sel;
     tune_fold(ctx, t);
     if (!t.active) return t.sel;
-    t.cand = t.turn++ % NKC;
+    t.cand = t.turn++ % NCAND;
     (void)hipEventRecord(t.ev[0], ctx->cur);
     return t.cand;
 }
@@ -848,17 +848,21 @@ void tune_end(icf_ctx *ctx, bool acc, bool fp64)
     if (t.cand >= 0 && !ctx->capturing) (void)hipEventRecord(t.ev[1], ctx->cur);
 }
 
-// Calls f(std::integral_constant<int, KEY_CANDS[c]>()).
+// Calls f(integral_constant<cand_keys(c)>, integral_constant<cand_stage(c)>).
 template <typename F>
-void with_keys(int c, F &&f)
+void with_cand(int c, F &&f)
 {
-    static_assert(NKC == 4 && KEY_CANDS[0] == 64 && KEY_CANDS[1] == 128 && KEY_CANDS[2] == 256 &&
-                      KEY_CANDS[3] == 512, "one case per candidate");
+    static_assert(NCAND == 8, "one case per candidate");
+    using std::integral_constant;
     switch (c) {
-    case 0: f(std::integral_constant<int, 64>()); break;
-    case 1: f(std::integral_constant<int, 128>()); break;
-    case 2: f(std::integral_constant<int, 256>()); break;
-    default: f(std::integral_constant<int, 512>()); break;
+    case 0: f(integral_constant<int, cand_keys(0)>(), integral_constant<int, cand_stage(0)>()); break;
+    case 1: f(integral_constant<int, cand_keys(1)>(), integral_constant<int, cand_stage(1)>()); break;
+    case 2: f(integral_constant<int, cand_keys(2)>(), integral_constant<int, cand_stage(2)>()); break;
+    case 3: f(integral_constant<int, cand_keys(3)>(), integral_constant<int, cand_stage(3)>()); break;
+    case 4: f(integral_constant<int, cand_keys(4)>(), integral_constant<int, cand_stage(4)>()); break;
+    case 5: f(integral_constant<int, cand_keys(5)>(), integral_constant<int, cand_stage(5)>()); break;
+    case 6: f(integral_constant<int, cand_keys(6)>(), integral_constant<int, cand_stage(6)>()); break;
+    default: f(integral_constant<int, cand_keys(7)>(), integral_constant<int, cand_stage(7)>()); break;
     }
 }
 
@@ -873,13 +877,13 @@ bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap 
     const int k = mixed ? 0 : 1;
     double *ts = (double *)ctx->tsum.p;
     const int c = tune_begin(ctx, false, !mixed);
-    with_keys(c, [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
+    with_cand(c, [&](auto kc, auto sc) {
+        constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
         const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
         const dim3 g(nch + nbf), b(K);   // neither is 0: at least one map has keys
 #define ICF_VF(MX, IS, L, NAME)                                                                                   \
-    LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, K>), g, b, nbf, mv->off, mv->codes, mv->chunks[c][k], nch, A, \
+    LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, K, S>), g, b, nbf, mv->off, mv->codes, mv->chunks[c][k], nch, A, \
            mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts)
         if (mixed && iso) ICF_VF(true, true, float, "k_vals_force<mixed,iso>");
         else if (mixed) ICF_VF(true, false, float, "k_vals_force<mixed,gen>");
@@ -903,14 +907,14 @@ bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const Slot
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const int k = mixed ? 0 : 1;
     const int c = tune_begin(ctx, false, !mixed);
-    with_keys(c, [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
+    with_cand(c, [&](auto kc, auto sc) {
+        constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
         const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
         const int nbc = nCZs ? round8(std::min<long long>(blocks_for(nCZs, K), 64)) : 0;
         const dim3 g(nbf + nbc + nch), bl(K);   // not empty: something has keys or CZs
 #define ICF_VFR(MX, IS, L, NAME)                                                                                   \
-    LAUNCH(ctx, NAME, (k_vals_force_rhs_fused<MX, IS, L, K>), g, bl, nbf, nbc, mv->off, mv->codes, mv->chunks[c][k], \
+    LAUNCH(ctx, NAME, (k_vals_force_rhs_fused<MX, IS, L, K, S>), g, bl, nbf, nbc, mv->off, mv->codes, mv->chunks[c][k], \
            nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, mr->node_row, mr->off, mr->codes, mr->cz_begin, \
            b, dmg, fld, nCZs, counts, st)
         if (mixed && iso) ICF_VFR(true, true, float, "k_vals_force_rhs<mixed,iso>");
@@ -933,14 +937,14 @@ bool contact_sums_fused(icf_ctx *ctx, const SlotMap *mf, double *F, int ns, cons
     const bool mixed = ctx->precision == ICF_MIXED;
     const int k = mixed ? 0 : 1;
     const int c = tune_begin(ctx, true, !mixed);
-    with_keys(c, [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
+    with_cand(c, [&](auto kc, auto sc) {
+        constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
         const int nb0 = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0, nb1 = round8(nch);
         const int nb2 = mr->nlaunch ? (int)blocks_for(mr->nlaunch, K) : 0;
         const dim3 g(nb0 + nb1 + nb2), bl(K);
 #define ICF_CS(MX, L, NAME)                                                                                       \
-    LAUNCH(ctx, NAME, (k_contact_sums<MX, L, K>), g, bl, nb0, nb1, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, \
+    LAUNCH(ctx, NAME, (k_contact_sums<MX, L, K, S>), g, bl, nb0, nb1, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, \
            mv->off, mv->codes, mv->keys, mv->chunks[c][k], nch, A, mr->off, mr->codes, mr->keys, mr->nlaunch, b, st)
         if (mixed) ICF_CS(true, float, "k_contact_sums<mixed>");
         else ICF_CS(false, double, "k_contact_sums<fp64>");
@@ -972,10 +976,10 @@ void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, co
     const int k = sizeof(L) == 4 ? 0 : 1;
     if (m->nlaunch == 0) return;   // then no chunks for any key limit
     const int c = tune_begin(ctx, ACC, k == 1);
-    with_keys(c, [&](auto kc) {
-        constexpr int K = decltype(kc)::value;
+    with_cand(c, [&](auto kc, auto sc) {
+        constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int n = m->nchunks[c][k];
-        LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L, K>), dim3(n), dim3(K), m->off, m->codes, m->keys,
+        LAUNCH(ctx, name, (k_reduce_vals<MX, IS, ACC, L, K, S>), dim3(n), dim3(K), m->off, m->codes, m->keys,
                m->chunks[c][k], n, A, sv);
     });
     tune_end(ctx, ACC, k == 1);
@@ -1043,6 +1047,7 @@ int icf_create(int device, icf_ctx **out)
     if (e != hipSuccess) return (int)e;
     if (device >= ndev) return ICF_E_INVALID;
     icf_ctx *ctx = new icf_ctx();
+    for (auto &row : ctx->tune) row[1].sel = CAND_DEFAULT_FP64;   // FP64 staging roles
     ctx->id = g_next_ctx_id++;
     ctx->device = device;
     memset(&ctx->hc, 0, sizeof(ctx->hc));
@@ -1403,7 +1408,7 @@ int icf_set_autotune(icf_ctx *ctx, int rounds)
             if (rounds == 0) {         // stop: fold what was timed, keep the fastest so far
                 if (t.active) {
                     tune_fold(ctx, t);
-                    for (int c = 0; c < NKC; c++)
+                    for (int c = 0; c < NCAND; c++)
                         if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
                 }
                 t.active = false;
@@ -1420,27 +1425,30 @@ int icf_set_autotune(icf_ctx *ctx, int rounds)
     return 0;
 }
 
-int icf_get_autotune(icf_ctx *ctx, int *keys, int *tuning, float *best_ms)
+int icf_get_autotune(icf_ctx *ctx, int *keys, int *stage, int *tuning, float *best_ms)
 {
     if (!ctx) return ICF_E_INVALID;
     for (int r = 0; r < 4; r++) {
         icf_ctx::KeyTune &t = ctx->tune[r >> 1][r & 1];
         if (t.active && !ctx->capturing) tune_fold(ctx, t);   // a finished last timing counts
-        if (keys) keys[r] = KEY_CANDS[t.sel];
+        if (keys) keys[r] = cand_keys(t.sel);
+        if (stage) stage[r] = cand_stage(t.sel);
         if (tuning) tuning[r] = t.active ? 1 : 0;
         if (best_ms)
-            for (int c = 0; c < NKC; c++) best_ms[r * NKC + c] = t.best[c];
+            for (int c = 0; c < NCAND; c++) best_ms[r * NCAND + c] = t.best[c];
     }
     return 0;
 }
 
-int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys)
+int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys, int stage)
 {
     if (!ctx) return ICF_E_INVALID;
-    int c = 0;
-    while (c < NKC && KEY_CANDS[c] != keys) c++;
-    if (c == NKC) return fail(ctx, ICF_E_INVALID, "icf_set_chunk_keys: keys must be 64, 128, 256 or 512");
     icf_ctx::KeyTune &t = ctx->tune[accumulate != 0][fp64 != 0];
+    if (stage == 0) stage = cand_stage(t.sel);
+    int c = 0;
+    while (c < NCAND && (cand_keys(c) != keys || cand_stage(c) != stage)) c++;
+    if (c == NCAND)
+        return fail(ctx, ICF_E_INVALID, "icf_set_chunk_keys: keys must be 64, 128, 256 or 512 and stage 2 or 3");
     if (t.active && !ctx->capturing) tune_fold(ctx, t);
     t.active = false;
     t.cand = -1;

@@ -119,23 +119,28 @@ __global__ __launch_bounds__(256) void k_map_compact(const int *__restrict__ fla
     if (k < nkeys && flag[k]) keys[pos[k]] = k;
 }
 
-// Chunk limits of the staged BCSR-value reduction: keys per workgroup (one thread each) and
-// contributions staged in LDS, 2 FP32 or 1.5 FP64 values x 9 per key (18 KB and 27 KB for
-// 256 keys).  FP32: the least that still holds the chunk's output blocks (9 doubles per key)
-// when they leave through the same LDS; chunks end on the contribution limit (keys average
-// about 5 contributions), and the small buffer lets more workgroups share a CU (3 per key
-// measured 4% slower, 4 and 5 slower still).  FP64: 1 per key measured 4% slower than 1.5.
-// The key limit is the workgroup size; a value map
-// has a chunk table for every candidate limit and staging type, and the library's autotuner
-// picks the limit per role (overwrite or accumulate map, FP32 or FP64 staging) by timing the
-// candidates.  Every limit gives the same values: each key's contributions are added in list
-// order whatever the chunking.
-constexpr int NKC = 4;                                   // candidate key limits
+// Chunk shape of the staged BCSR-value reduction: keys per workgroup (one thread each; the
+// workgroup size) and contributions staged in LDS, STAGE FP32 or STAGE/2 FP64 values x 9 per
+// key.  STAGE = 2 is the least that still holds the chunk's output blocks (9 doubles per key)
+// when they leave through the same LDS; a smaller buffer lets more workgroups share a CU, a
+// larger one ends fewer chunks on the contribution limit (keys average about 5
+// contributions).  A value map has a chunk table for every candidate shape and staging type,
+// and the library's autotuner picks the shape per role (overwrite or accumulate map, FP32 or
+// FP64 staging) by timing the candidates.  Every shape gives the same values: each key's
+// contributions are added in list order whatever the chunking.
+constexpr int NKC = 4, NSC = 2, NCAND = NKC * NSC;       // candidate key limits x stagings
 constexpr int KEY_CANDS[NKC] = {64, 128, 256, 512};
-constexpr int KC_DEFAULT = 2;                            // 256 keys until tuned
-template <typename L, int KEYS>
-__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? 2 * KEYS : 3 * KEYS / 2; }
-__host__ __device__ constexpr int flat_cap_of(bool fp32, int keys) { return fp32 ? 2 * keys : 3 * keys / 2; }
+constexpr int STAGE_CANDS[NSC] = {2, 3};
+__host__ __device__ constexpr int cand_keys(int c) { return KEY_CANDS[c / NSC]; }
+__host__ __device__ constexpr int cand_stage(int c) { return STAGE_CANDS[c % NSC]; }
+constexpr int CAND_DEFAULT_FP32 = 2 * NSC + 0;           // 256 keys, 2 per key, until tuned
+constexpr int CAND_DEFAULT_FP64 = 2 * NSC + 1;           // 256 keys, 1.5 per key
+template <typename L, int KEYS, int STAGE>
+__host__ __device__ constexpr int flat_cap() { return sizeof(L) == 4 ? STAGE * KEYS : STAGE * KEYS / 2; }
+__host__ __device__ constexpr int flat_cap_of(bool fp32, int keys, int stage)
+{
+    return fp32 ? stage * keys : stage * keys / 2;
+}
 
 // Staged per-unit records of the current call.
 struct Stage {
@@ -196,12 +201,12 @@ __device__ inline void contrib9(const Stage &sv, uint32_t code, L v[9])
 //   keys   null: chunk positions are keys; else keys[position] (accumulate mode)
 //   ACC    add to A (single-kernel calls); otherwise overwrite, zero where nothing contributes
 // L is the staging type: float for tet and CZ blocks in mixed mode, double otherwise.
-template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS>
+template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS, int STAGE>
 __device__ __forceinline__ void reduce_vals(int bid_, int nb_, const int *__restrict__ off,
         const uint32_t *__restrict__ codes, const int *__restrict__ keys, const int2 *__restrict__ chunks, int nchunks,
         double *__restrict__ A, Stage sv)
 {
-    constexpr int CAP = flat_cap<L, KEYS>();
+    constexpr int CAP = flat_cap<L, KEYS, STAGE>();
     static_assert(CAP * 9 * sizeof(L) >= KEYS * 9 * sizeof(double), "output staging fits the LDS");
     constexpr int PER = (CAP + KEYS - 1) / KEYS;
     __shared__ L lds[CAP * 9];
@@ -285,12 +290,12 @@ __device__ __forceinline__ void reduce_vals(int bid_, int nb_, const int *__rest
     for (int i = threadIdx.x; i < nout; i += KEYS) dst[i] = out[i];
 }
 
-template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS>
+template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS, int STAGE>
 __global__ __launch_bounds__(KEYS) void k_reduce_vals(const int *__restrict__ off, const uint32_t *__restrict__ codes,
                                                            const int *__restrict__ keys, const int2 *__restrict__ chunks,
                                                            int nchunks, double *__restrict__ A, Stage sv)
 {
-    reduce_vals<MIXED, ISO, ACC, L, KEYS>(blockIdx.x, gridDim.x, off, codes, keys, chunks, nchunks, A, sv);
+    reduce_vals<MIXED, ISO, ACC, L, KEYS, STAGE>(blockIdx.x, gridDim.x, off, codes, keys, chunks, nchunks, A, sv);
 }
 
 // Phase 2 for rhs rows (b[row*3+k]) or node forces (dn[node + ns*(F+k)]): one thread per key.

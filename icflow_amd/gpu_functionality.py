@@ -134,26 +134,29 @@ class GPUFunctionality:
     def set_precision(self, mode):
         check(self.lib.icf_set_precision(self.ctx, mode), self.ctx, "icf_set_precision")
 
-    # -- autotuner of the BCSR-value reductions' key limit (no reference counterpart) --
+    # -- autotuner of the BCSR-value reductions' chunk shape (no reference counterpart) --
     ROLES = ("overwrite,fp32", "overwrite,fp64", "accumulate,fp32", "accumulate,fp64")
     KEY_CANDIDATES = (64, 128, 256, 512)
+    STAGE_CANDIDATES = (2, 3)
+    CANDIDATES = ((64, 2), (64, 3), (128, 2), (128, 3), (256, 2), (256, 3), (512, 2), (512, 3))   # index c
 
     def set_autotune(self, rounds):
         check(self.lib.icf_set_autotune(self.ctx, rounds), self.ctx, "icf_set_autotune")
 
     def autotune_state(self):
-        """{role: (key limit in use, tuning?, {candidate: fastest ms})} for the four roles."""
-        nc = len(self.KEY_CANDIDATES)
-        keys, tuning = np.zeros(4, np.int32), np.zeros(4, np.int32)
+        """{role: ((keys, stage) in use, tuning?, {(keys, stage): fastest ms})} for the four
+        roles."""
+        nc = len(self.CANDIDATES)
+        keys, stage, tuning = np.zeros(4, np.int32), np.zeros(4, np.int32), np.zeros(4, np.int32)
         best = np.zeros(4 * nc, np.float32)
-        check(self.lib.icf_get_autotune(self.ctx, _lib.ptr(keys), _lib.ptr(tuning), _lib.ptr(best)), self.ctx,
-              "icf_get_autotune")
-        return {r: (int(keys[i]), bool(tuning[i]),
-                    {c: float(best[i * nc + j]) for j, c in enumerate(self.KEY_CANDIDATES)})
+        check(self.lib.icf_get_autotune(self.ctx, _lib.ptr(keys), _lib.ptr(stage), _lib.ptr(tuning), _lib.ptr(best)),
+              self.ctx, "icf_get_autotune")
+        return {r: ((int(keys[i]), int(stage[i])), bool(tuning[i]),
+                    {c: float(best[i * nc + j]) for j, c in enumerate(self.CANDIDATES)})
                 for i, r in enumerate(self.ROLES)}
 
-    def set_chunk_keys(self, accumulate, fp64, keys):
-        check(self.lib.icf_set_chunk_keys(self.ctx, int(accumulate), int(fp64), keys), self.ctx,
+    def set_chunk_keys(self, accumulate, fp64, keys, stage=0):
+        check(self.lib.icf_set_chunk_keys(self.ctx, int(accumulate), int(fp64), keys, stage), self.ctx,
               "icf_set_chunk_keys")
 
     # -- TransferStaticDataToDevice (:255-302) --

@@ -202,9 +202,11 @@ int icf_set_profiling(icf_ctx *ctx, int on);
 int icf_set_overlap(icf_ctx *ctx, int on);
 
 /* Autotuner of the BCSR-value reductions.  Their workgroups reduce chunks of consecutive keys
- * (BCSR blocks), one key per lane; the key limit (64, 128, 256 or 512 = ICF_KEY_CANDIDATES
- * sizes, 256 until tuned) trades staging-buffer size (LDS per workgroup, hence workgroups per
- * CU) against the tail of a chunk.  There are four roles, r = 2 * accumulate + fp64:
+ * (BCSR blocks), one key per lane, with the chunk's contributions staged in LDS.  A chunk
+ * shape is a key limit (the workgroup size: 64, 128, 256 or 512) and a staging size (stage =
+ * 2 or 3 FP32 contributions per key, half as many FP64 ones); the ICF_CHUNK_CANDIDATES = 8
+ * shapes are numbered c = 2 * (index of the key limit) + (stage - 2).  Until tuned: 256 keys,
+ * stage 2 for FP32 staging, 3 for FP64.  There are four roles, r = 2 * accumulate + fp64:
  * overwrite maps (icf_assemble_elems_and_czs) and accumulate maps (icf_collision_response and
  * the single-kernel calls), each with FP32 (mixed) or FP64 staging.
  * icf_set_autotune(ctx, rounds > 0) starts tuning every role: its next launches take the
@@ -212,16 +214,16 @@ int icf_set_overlap(icf_ctx *ctx, int on);
  * role), until every candidate has `rounds` timings; then the fastest (least time over its
  * timings) is kept.  rounds = 0 stops tuning and keeps the fastest timed so far.  Captured
  * calls are never timed.  Time it in blocking mode or with overlap off for clean timings.
- * icf_get_autotune: per role, keys[r] = the key limit in use, tuning[r] = 1 while tuning,
- * best_ms[r * ICF_KEY_CANDIDATES + c] = fastest timing of candidate c (0 = none); any of the
- * three may be null.  icf_set_chunk_keys fixes one role's key limit (ends its tuning).
- * Results do not depend on the key limit: each key's contributions are added in list order
- * whatever the chunking.  No reference counterpart (the reference launches every kernel with
- * a fixed block_size = 128, GPU_Functionality.cs:38,123-136). */
-#define ICF_KEY_CANDIDATES 4
+ * icf_get_autotune: per role, keys[r] and stage[r] = the shape in use, tuning[r] = 1 while
+ * tuning, best_ms[r * ICF_CHUNK_CANDIDATES + c] = fastest timing of candidate c (0 = none);
+ * any of the four may be null.  icf_set_chunk_keys fixes one role's shape (stage 0: keep the
+ * role's staging) and ends its tuning.  Results do not depend on the shape: each key's
+ * contributions are added in list order whatever the chunking.  No reference counterpart (the
+ * reference launches every kernel with a fixed block_size = 128, GPU_Functionality.cs:38,123-136). */
+#define ICF_CHUNK_CANDIDATES 8
 int icf_set_autotune(icf_ctx *ctx, int rounds);
-int icf_get_autotune(icf_ctx *ctx, int *keys, int *tuning, float *best_ms);
-int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys);
+int icf_get_autotune(icf_ctx *ctx, int *keys, int *stage, int *tuning, float *best_ms);
+int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys, int stage);
 int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);
 int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches);

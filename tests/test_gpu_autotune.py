@@ -1,8 +1,9 @@
-"""Autotuner of the BCSR-value reductions' key limit (icf_set_autotune, icf_get_autotune,
-icf_set_chunk_keys).  The key limit changes how the keys are chunked over workgroups, never the
-order in which a key's contributions are added: the fused iteration with contact gives the same
-bits (BCSR values, rhs rows, node forces, CZ flags) under every candidate of every role, and
-after tuning; so do the reference's separate launches (accumulate maps)."""
+"""Autotuner of the BCSR-value reductions' chunk shape (icf_set_autotune, icf_get_autotune,
+icf_set_chunk_keys): key limit and staging size.  The shape changes how the keys are chunked
+over workgroups, never the order in which a key's contributions are added: the fused iteration
+with contact gives the same bits (BCSR values, rhs rows, node forces, CZ flags) under every
+candidate of every role, and after tuning; so do the reference's separate launches
+(accumulate maps)."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ from workloads import DIST_EPS, H, PENALTY_K, Pipeline, make_state, small_specim
 
 pytestmark = pytest.mark.gpu
 
-CANDS = (64, 128, 256, 512)
+CANDS = tuple((k, s) for k in (64, 128, 256, 512) for s in (2, 3))
 
 
 def _iteration(p):
@@ -45,51 +46,55 @@ def _same(a, b, what):
 
 
 @pytest.mark.parametrize("precision", [_lib.ICF_FP64, _lib.ICF_MIXED], ids=["fp64", "mixed"])
-def test_every_key_limit_gives_the_same_bits(specimen_consts, precision):
+def test_every_chunk_shape_gives_the_same_bits(specimen_consts, precision):
     col = small_specimen(cells=(10, 8, 6), size=(0.1, 0.08, 0.06), grains=12, seed=31)
     st = make_state(col, specimen_consts, seed=32, platen_dz_frac=0.3, cells_z=6, size_z=0.06)
     p = Pipeline(col, specimen_consts, st, precision=precision, contact=True)
     gf = p.gf
     assert gf.nImpacts > 0
     fp64 = precision == _lib.ICF_FP64
+    stage = "fp64" if fp64 else "fp32"
     state = gf.autotune_state()
-    assert all(v[0] == 256 and not v[1] for v in state.values())      # default before tuning
+    for r, (shape, tuning, _) in state.items():          # defaults before tuning
+        assert shape == ((256, 3) if r.endswith("fp64") else (256, 2)) and not tuning, (r, shape)
     ref = _iteration(p)
-    for keys in CANDS:
+    for keys, stg in CANDS:
         for acc in (0, 1):
-            gf.set_chunk_keys(acc, fp64, keys)
-        assert gf.autotune_state()[f"overwrite,{'fp64' if fp64 else 'fp32'}"][0] == keys
-        _same(_iteration(p), ref, keys)
-    # the separate single-kernel launches (accumulate maps) under every key limit
-    for keys in CANDS:
-        gf.set_chunk_keys(1, fp64, keys)
-        _same(_separate(p), ref, ("separate", keys))
-    # mixed choices (overwrite and accumulate maps tuned apart)
-    gf.set_chunk_keys(0, fp64, 64)
-    gf.set_chunk_keys(1, fp64, 512)
+            gf.set_chunk_keys(acc, fp64, keys, stg)
+        assert gf.autotune_state()[f"overwrite,{stage}"][0] == (keys, stg)
+        _same(_iteration(p), ref, (keys, stg))
+    # the separate single-kernel launches (accumulate maps) under every shape
+    for keys, stg in CANDS:
+        gf.set_chunk_keys(1, fp64, keys, stg)
+        _same(_separate(p), ref, ("separate", keys, stg))
+    # mixed choices (overwrite and accumulate maps tuned apart); stage 0 keeps the staging
+    gf.set_chunk_keys(0, fp64, 64, 2)
+    gf.set_chunk_keys(1, fp64, 512, 0)
+    assert gf.autotune_state()[f"accumulate,{stage}"][0] == (512, 3)
     _same(_iteration(p), ref, "64/512")
     # tuning: every candidate of the two roles this iteration uses is timed, then one is kept
     gf.set_autotune(2)
-    used = [f"overwrite,{'fp64' if fp64 else 'fp32'}", f"accumulate,{'fp64' if fp64 else 'fp32'}"]
+    used = [f"overwrite,{stage}", f"accumulate,{stage}"]
     for it in range(2 * len(CANDS) + 1):
         _same(_iteration(p), ref, ("tuning", it))
     state = gf.autotune_state()
     for r in used:
-        keys, tuning, best = state[r]
+        shape, tuning, best = state[r]
         assert not tuning, (r, state[r])
-        assert keys in CANDS
+        assert shape in CANDS
         assert all(ms > 0 for ms in best.values()), (r, best)
-        assert best[keys] == min(best.values()), (r, best)
+        assert best[shape] == min(best.values()), (r, best)
     _same(_iteration(p), ref, "tuned")
-    # bad key limits are refused; rounds = 0 stops tuning and keeps a timed candidate
-    assert gf.lib.icf_set_chunk_keys(gf.ctx, 0, 0, 100) == _lib.ICF_E_INVALID
+    # bad shapes are refused; rounds = 0 stops tuning and keeps a timed candidate
+    assert gf.lib.icf_set_chunk_keys(gf.ctx, 0, 0, 100, 2) == _lib.ICF_E_INVALID
+    assert gf.lib.icf_set_chunk_keys(gf.ctx, 0, 0, 256, 4) == _lib.ICF_E_INVALID
     assert gf.lib.icf_set_autotune(gf.ctx, -1) == _lib.ICF_E_INVALID
     gf.set_autotune(1)
     _iteration(p)
     gf.set_autotune(0)
     state = gf.autotune_state()
     for r in used:
-        keys, tuning, best = state[r]
-        assert not tuning and best[keys] > 0, (r, state[r])
+        shape, tuning, best = state[r]
+        assert not tuning and best[shape] > 0, (r, state[r])
     _same(_iteration(p), ref, "stopped")
     gf.close()
