@@ -134,6 +134,27 @@ class UpperBcsrSolver:
         return solve_upper_bcsr(self.rows, self.cols, vals, rhs)
 
 
+def adjust_time_step(tsf, hold, explodes, diverges, converged, max_iter):
+    """_adjustTimeStep (ImplicitModel3.cs:258-288) and the acceptance rule of Step (:415-420)
+    for an attempt run at time-scale factor tsf with hold count hold.  Returns (accepted,
+    factor, hold) for what comes next: the discarded frame's (cf) retry values, or the accepted
+    frame's (tcf0) values.  Only cf is changed by an exploding, diverging or unconverged
+    attempt; only tcf0 by a converged one."""
+    cf_tsf, cf_hold, t_tsf, t_hold = tsf, hold, tsf, hold
+    if explodes:
+        cf_tsf, cf_hold = min(cf_tsf * 2, PARTS), HOLD_FACTOR_DELAY
+    elif (diverges or not converged) and t_tsf < PARTS and max_iter > 1:
+        cf_tsf, cf_hold = cf_tsf * 2, HOLD_FACTOR_DELAY
+    elif converged and cf_tsf > 1:
+        if t_hold > 0:
+            t_hold -= 1
+        else:
+            t_tsf, t_hold = max(1, t_tsf // 2), 2
+    accepted = (max_iter == 1 or t_tsf == PARTS or (not converged and t_tsf >= PARTS)
+                or (not explodes and not diverges and converged))
+    return (True, t_tsf, t_hold) if accepted else (False, cf_tsf, cf_hold)
+
+
 class ImplicitModel:
     """One specimen's implicit dynamics on the device.  `prescribed(t)` returns, for the
     anchored deformable nodes, the target displacement (n_anchored, 3) at time t (the role
@@ -391,25 +412,12 @@ class ImplicitModel:
                 break
         accepted = True
         if p.adapt:
-            # _adjustTimeStep: cf's factor for a retry, tcf0's for the next frame
-            cf_tsf, cf_hold, t_tsf, t_hold = self.tsf, self.hold, self.tsf, self.hold
-            if explodes:
-                cf_tsf, cf_hold = min(cf_tsf * 2, PARTS), HOLD_FACTOR_DELAY
-            elif (diverges or not converged) and t_tsf < PARTS and p.max_iter > 1:
-                cf_tsf, cf_hold = cf_tsf * 2, HOLD_FACTOR_DELAY
-            elif converged and cf_tsf > 1:
-                if t_hold > 0:
-                    t_hold -= 1
-                else:
-                    t_tsf, t_hold = max(1, t_tsf // 2), 2
-            accepted = (p.max_iter == 1 or t_tsf == PARTS or (not converged and t_tsf >= PARTS)
-                        or (not explodes and not diverges and converged))
+            accepted, self.tsf, self.hold = adjust_time_step(self.tsf, self.hold, explodes, diverges, converged,
+                                                             p.max_iter)
             if not accepted:
-                self.tsf, self.hold = cf_tsf, cf_hold
                 rec = StepRecord(it, converged, 0, dmg.value, norms, accepted=False, h=h, explodes=explodes)
                 self.records.append(rec)
                 return rec
-            self.tsf, self.hold = t_tsf, t_hold
             self.int_time += ticks
             self.frames += 1
             if self.frames == 1:

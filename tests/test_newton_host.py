@@ -68,3 +68,23 @@ def test_banded_solver_matches_dense_and_falls_back():
         b = rng.normal(size=3 * N)
         x = s.solve(vals, b)
         assert np.allclose(A @ x, b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
+
+
+def test_time_step_control_rules():
+    """adjust_time_step against the cases of ImplicitModel3._adjustTimeStep (:258-288) and the
+    acceptance rule of Step (:415-420), written out by hand."""
+    from icflow_amd.newton import HOLD_FACTOR_DELAY as HOLD, PARTS, adjust_time_step as adj
+    # converged at factor 1: accepted, nothing changes
+    assert adj(1, 0, False, False, True, 10) == (True, 1, 0)
+    # exploding: discarded, factor doubled, hold set
+    assert adj(4, 1, True, False, True, 10) == (False, 8, HOLD)
+    # diverging or not converged below the finest step: discarded, doubled
+    assert adj(4, 0, False, True, False, 10) == (False, 8, HOLD)
+    assert adj(4, 0, False, False, False, 10) == (False, 8, HOLD)
+    # converged with a raised factor: the hold counts down first, then the factor halves
+    assert adj(8, 3, False, False, True, 10) == (True, 8, 2)
+    assert adj(8, 0, False, False, True, 10) == (True, 4, 2)
+    # at the finest step a diverging attempt is accepted anyway, factor unchanged
+    assert adj(PARTS, 0, False, True, False, 10) == (True, PARTS, 0)
+    # with a single iteration per step every attempt is accepted
+    assert adj(2, 0, False, False, False, 1) == (True, 2, 0)
