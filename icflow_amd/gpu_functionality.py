@@ -159,6 +159,17 @@ class GPUFunctionality:
         check(self.lib.icf_set_chunk_keys(self.ctx, int(accumulate), int(fp64), keys, stage), self.ctx,
               "icf_set_chunk_keys")
 
+    def tuned_shapes(self):
+        """{role: [keys, stage]}: the shapes in use, to keep for a later run on the same specimen."""
+        return {r: list(v[0]) for r, v in self.autotune_state().items()}
+
+    def use_shapes(self, shapes):
+        """Fix the shapes of tuned_shapes() (for example read back from JSON) without tuning."""
+        for i, r in enumerate(self.ROLES):
+            if r in shapes:
+                keys, stage = shapes[r]
+                self.set_chunk_keys(i >> 1, i & 1, int(keys), int(stage))
+
     # -- TransferStaticDataToDevice (:255-302) --
     def TransferStaticDataToDevice(self, col):
         self.col = col

@@ -97,4 +97,12 @@ def test_every_chunk_shape_gives_the_same_bits(specimen_consts, precision):
         shape, tuning, best = state[r]
         assert not tuning and best[shape] > 0, (r, state[r])
     _same(_iteration(p), ref, "stopped")
+    # the tuned shapes carry over to a new context through JSON
+    import json
+    saved = json.loads(json.dumps(gf.tuned_shapes()))
     gf.close()
+    from icflow_amd.gpu_functionality import GPUFunctionality
+    g2 = GPUFunctionality(0)
+    g2.use_shapes(saved)
+    assert g2.tuned_shapes() == saved
+    g2.close()
