@@ -578,6 +578,16 @@ def main():
                 traffic_source="profiles/assembly_compression40_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                "copy-calibrated)" if traffic else None)
     it_bytes = iteration_bytes(r)
+    # every kernel of the step against HBM peak: measured traffic (committed counter summary,
+    # default workload only) and algorithmic bytes over its HIP-event duration
+    util = {}
+    for kname, (ms, n) in prof.items():
+        us = 1e3 * ms / max(n, 1)
+        tb = profiled_traffic(kname, key_limit(r.gf, kname)) if profiled else None
+        ab = kernel_bytes(alg, kname)
+        util[kname] = dict(avg_us=round(us, 2),
+                           traffic_frac=round(tb / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if tb else None,
+                           alg_frac=round(ab / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if ab else None)
 
     # device broad phase over the same surface tets (reported beside the step; the reference's
     # broad phase is a host BVH outside the assembly path)
@@ -654,6 +664,7 @@ def main():
                                "frac": round(it_bytes / sec / 1e9 / HBM_PEAK_GBS, 4)},
         "mixed_rel_err_vs_fp64": errs,
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
+        "kernels_hbm": util,
         "autotune": tuned or None,
         "roofline": roof,
         "hbm_copy_probe": bw,
