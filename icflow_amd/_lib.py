@@ -65,6 +65,8 @@ _SIGS = {
     "icf_narrow_phase": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "icf_find_closest_face": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_int, c_int, c_int]),
+    "icf_find_closest_face_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_int, c_int, c_int]),
     "icf_collision_response": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,
                                        c_int, c_int, c_double, c_double]),
     "icf_assemble_elems_and_czs": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int,

@@ -14,6 +14,7 @@
 #include <type_traits>
 #include <atomic>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/icflow_hip.h"
 
@@ -144,6 +145,31 @@ struct SlotMap {
 std::atomic<int> g_next_ctx_id{1};
 int g_const_owner[64];             // per device: id of the context whose constants are loaded
 
+// Scratch of the allocation-free entry point (icf_find_closest_face_noalloc): one array per
+// GPU, reserved when the code object loads on that GPU and held until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy (on the host the name is a
+// stand-in, not a device address): it uses the address that hipGetSymbolAddress returned for
+// the device, looked up once per device by icf_create and kept in g_scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64];           // per device: device address of `scratch`, null until looked up
+
+// With `device` current: the device address of its scratch array (looked up on first use).
+hipError_t scratch_for_device(int device, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct icf_ctx {
@@ -179,6 +205,7 @@ struct icf_ctx {
         bool valid = false;
     } adj;
     int *small = nullptr;          // 8 ints of scratch (max key, error flag, sums)
+    unsigned char *noalloc = nullptr;   // this device's `scratch` array (device address)
     std::list<SlotMap> maps;       // std::list: pointers handed out by get_map stay valid
     int next_map_id = 0;
     std::string err;
@@ -1064,6 +1091,7 @@ int icf_create(int device, icf_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) e = hipMalloc(&ctx->small, 8 * sizeof(int));
+    if (e == hipSuccess) e = scratch_for_device(device, (void **)&ctx->noalloc);
     if (e != hipSuccess) {
         delete ctx;
         return (int)e;
@@ -1614,6 +1642,53 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
            nPairs, tet_stride, nl2, dn, (const int *)ctx->adj.off.p, (const int *)ctx->adj.ids.p, faces, nd_stride,
            fc_stride);
     if (int r = ln.done()) return r;
+    return tm.finish();
+}
+
+int icf_find_closest_face_noalloc(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, const double *dn,
+                                  const int *ie, const int *faces, int nd_stride, int el_all_stride, int fc_stride)
+{
+    if (int r = check_ctx(ctx)) return r;
+    if (bad_units(nPairs, tet_stride) || nd_stride < 0 || el_all_stride < 0 || fc_stride < 0 ||
+        (nPairs && (!nl2 || !dn || !ie || !faces)))
+        return fail(ctx, ICF_E_INVALID, "icf_find_closest_face_noalloc: invalid argument");
+    if (nPairs == 0) return 0;
+    if (!ctx->noalloc) return fail(ctx, ICF_E_INVALID, "icf_find_closest_face_noalloc: no scratch on this device");
+    const int es = el_all_stride;
+    // adjacent-face rows of ie, from the extent of its allocation (as icf_find_closest_face)
+    void *base = nullptr;
+    size_t sz = 0;
+    HIPCHK(ctx, hipMemGetAddressRange(&base, &sz, const_cast<int *>(ie)));
+    const long long rows = ((long long)((uintptr_t)base + sz - (uintptr_t)ie)) / (4LL * std::max(es, 1));
+    if (rows < 5) return fail(ctx, ICF_E_INVALID, "icf_find_closest_face_noalloc: ie holds fewer than 5 rows");
+    const int max_rows = (int)std::min(rows - 5, 1LL << 20);
+    // scratch layout: list offsets [es + 1], the scan's temporary storage, the lists (at most
+    // max_rows faces per tet: k_adj_count clamps the counts)
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t tb = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (int *)nullptr, (int *)nullptr, es + 1, ctx->stream));
+    const size_t tmp_at = up(sizeof(int) * ((size_t)es + 1)), ids_at = tmp_at + up(tb);
+    const size_t need = ids_at + sizeof(int) * (size_t)es * (size_t)std::max(max_rows, 1);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, ICF_E_CAPACITY, "icf_find_closest_face_noalloc: el_all_stride " + std::to_string(es) +
+                                             " x " + std::to_string(max_rows) + " adjacent-face rows need " +
+                                             std::to_string(need) + " bytes of scratch, more than its " +
+                                             std::to_string(NOALLOC_SCRATCH_BYTES) + "; use icf_find_closest_face");
+    int *off = (int *)ctx->noalloc;
+    void *tmp = ctx->noalloc + tmp_at;
+    int *ids = (int *)(ctx->noalloc + ids_at);
+    // after every earlier call of the context, on its stream; later calls of either lane wait
+    if (int r = lane_barrier(ctx)) return r;
+    Timer tm(ctx);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, off + es, 1, 0);
+    hipLaunchKernelGGL(k_adj_count, dim3(blocks_for(es, 256)), dim3(256), 0, ctx->stream, ie, es, max_rows, off);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, off, off, es + 1, ctx->stream));
+    hipLaunchKernelGGL(k_adj_fill, dim3(blocks_for(es, 256)), dim3(256), 0, ctx->stream, ie, es, (const int *)off,
+                       ids);
+    hipLaunchKernelGGL(k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256), 0,
+                       ctx->stream, nPairs, tet_stride, nl2, dn, (const int *)off, (const int *)ids, faces, nd_stride,
+                       fc_stride);
+    if (int r = lane_exit(ctx, 0, everything)) return r;
     return tm.finish();
 }
 

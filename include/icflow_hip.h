@@ -103,6 +103,19 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
                           const int *ie, const int *faces, int nd_stride, int el_all_stride,
                           int fc_stride);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops): icf_find_closest_face with the same
+ * arguments and results, bit for bit, that calls no hipMalloc/hipFree or allocating library
+ * routine.  Its scratch (the compact adjacent-face lists of ie, rebuilt on every call) lives
+ * in one fixed 64 MiB device array per GPU, shared by every context and call on that GPU
+ * and held until the process ends.  It needs 4 (el_all_stride + 1) + 4 el_all_stride x
+ * (adjacent-face rows of ie) bytes plus a small scan workspace; larger sizes return
+ * ICF_E_CAPACITY before any work is issued.  Calls that overlap on one GPU (two contexts,
+ * or non-blocking calls on different streams) race on the array: order them. */
+int icf_find_closest_face_noalloc(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2,
+                                  const double *dn, const int *ie, const int *faces,
+                                  int nd_stride, int el_all_stride, int fc_stride);
+
 /* kCollisionResponseForce (collision_kernels9/collision_response.h:38-126; launch
  * GPU_Functionality.cs:668-672): point-triangle penalty force and tangent.
  * icr: int SoA 28 x cr_stride (0-3 node ids p,v0,v1,v2; 4-19 slots; 20-23 rows). */
