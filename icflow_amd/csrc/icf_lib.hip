@@ -143,7 +143,12 @@ struct SlotMap {
 };
 
 std::atomic<int> g_next_ctx_id{1};
-int g_const_owner[64];             // per device: id of the context whose constants are loaded
+// All contexts on a device share the one __constant__ g_c.  g_const_owner[d] is the id of the
+// context whose constants it holds; g_dev_mu[d] is held by every call that launches kernels
+// reading g_c, from the check of the owner to its last launch, so that another context
+// cannot replace the constants in between (ConstUse).
+int g_const_owner[64];
+std::mutex g_dev_mu[64];
 
 // Scratch of the allocation-free entry point (icf_find_closest_face_noalloc): one array per
 // GPU, reserved when the code object loads on that GPU and held until the process ends.
@@ -187,6 +192,11 @@ struct icf_ctx {
     std::vector<LaneRec> lane_log[2];
     bool lane0_unlogged = false;   // lane-0 work issued since the last lane-0 log entry
     bool has_consts = false;
+    bool consts_dirty = false;     // icf_set_constants changed hc since this context's last upload
+    // bumped whenever memory the context owns is freed or moved, or a slot map or the
+    // adjacency lists may no longer match what they were built from: a graph captured at an
+    // older epoch may point at stale memory and is refused by icf_graph_launch
+    unsigned long long epoch = 1;
     int precision = ICF_FP64;
     DevConsts hc;
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
@@ -231,7 +241,8 @@ struct icf_ctx {
         std::vector<int> off;      // 4 (n + 1): tets, CZs, rows, blocks
         int n = 0;
         DevBuf d;
-        int checked_v = 0, checked_r = 0;
+        // the units and epoch whose block-diagonal layout was last checked against `off`
+        std::vector<long long> checked;
     } batch;
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
@@ -273,6 +284,7 @@ int grow(icf_ctx *ctx, DevBuf &b, size_t bytes)
         b.bytes = 0;
     }
     size_t want = bytes + bytes / 4 + 256;
+    ctx->epoch++;
     HIPCHK(ctx, hipMalloc(&b.p, want));
     b.bytes = want;
     return 0;
@@ -300,16 +312,25 @@ void free_map(SlotMap &m)
 
 // Maps whose key fields overlap [lo, hi): dropped when the memory goes away (erase), else
 // marked suspect, to be compared against their snapshot before the next use.
+bool map_keys_overlap(const SlotMap &m, uintptr_t lo, uintptr_t hi)
+{
+    for (const SrcKey &s : m.srcs) {
+        uintptr_t a = (uintptr_t)(s.base + (size_t)s.stride * s.field0);
+        uintptr_t b = (uintptr_t)(s.base + (size_t)s.stride * (s.field0 + s.nq));
+        if (a < hi && lo < b) return true;
+    }
+    return false;
+}
+
 void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi, bool erase = false)
 {
-    if (ctx->adj.valid && ctx->adj.lo < hi && lo < ctx->adj.hi) ctx->adj.valid = false;
+    if (ctx->adj.valid && ctx->adj.lo < hi && lo < ctx->adj.hi) {
+        ctx->adj.valid = false;
+        ctx->epoch++;
+    }
     for (auto it = ctx->maps.begin(); it != ctx->maps.end();) {
-        bool hit = false;
-        for (const SrcKey &s : it->srcs) {
-            uintptr_t a = (uintptr_t)(s.base + (size_t)s.stride * s.field0);
-            uintptr_t b = (uintptr_t)(s.base + (size_t)s.stride * (s.field0 + s.nq));
-            if (a < hi && lo < b) hit = true;
-        }
+        const bool hit = map_keys_overlap(*it, lo, hi);
+        if (hit) ctx->epoch++;
         if (hit && erase) {
             (void)hipStreamSynchronize(ctx->stream);
             free_map(*it);
@@ -321,16 +342,58 @@ void invalidate_range(icf_ctx *ctx, uintptr_t lo, uintptr_t hi, bool erase = fal
     }
 }
 
-int upload_consts(icf_ctx *ctx)
+// A write by the library into caller memory [p, p + bytes) (copies, kernels that write int
+// fields in place).  Outside capture: slot maps and adjacency lists built from those bytes
+// are checked again before their next use.  During capture such a write is refused: every
+// replay would change what the maps were built from behind their back.
+int note_write(icf_ctx *ctx, const void *p, size_t bytes, const char *what)
 {
-    if (!ctx->has_consts) return fail(ctx, ICF_E_NOCONST, "icf_set_constants has not been called");
-    if (g_const_owner[ctx->device] != ctx->id) {
-        HIPCHK(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(g_c), &ctx->hc, sizeof(DevConsts), 0, hipMemcpyHostToDevice,
-                                           ctx->stream));
-        g_const_owner[ctx->device] = ctx->id;
+    const uintptr_t lo = (uintptr_t)p, hi = lo + bytes;
+    if (!bytes) return 0;
+    if (!ctx->capturing) {
+        invalidate_range(ctx, lo, hi);
+        return 0;
     }
+    bool hit = ctx->adj.valid && ctx->adj.lo < hi && lo < ctx->adj.hi;
+    for (const SlotMap &m : ctx->maps) hit = hit || map_keys_overlap(m, lo, hi);
+    if (hit)
+        return fail(ctx, ICF_E_CAPTURE, std::string(what) +
+                                            " would write fields that a slot map or the adjacency lists were built "
+                                            "from: it cannot be captured");
     return 0;
 }
+
+// The context's constants in g_c for the lifetime of this object: takes the device's lock,
+// and when another context's constants are loaded, waits for all work on the device (its
+// queued kernels read them) before loading ours.  During capture nothing is loaded: a replay
+// loads them in icf_graph_launch.
+struct ConstUse {
+    std::unique_lock<std::mutex> lk;
+    int err = 0;
+    explicit ConstUse(icf_ctx *ctx)
+    {
+        if (!ctx->has_consts) {
+            err = fail(ctx, ICF_E_NOCONST, "icf_set_constants has not been called");
+            return;
+        }
+        if (ctx->capturing) return;
+        lk = std::unique_lock<std::mutex>(g_dev_mu[ctx->device]);
+        int &owner = g_const_owner[ctx->device];
+        if (owner == ctx->id && !ctx->consts_dirty) return;
+        hipError_t e = hipSuccess;
+        if (owner != ctx->id && owner != 0) e = hipDeviceSynchronize();
+        if (e == hipSuccess)
+            e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_c), &ctx->hc, sizeof(DevConsts), 0, hipMemcpyHostToDevice,
+                                       ctx->stream);
+        if (e != hipSuccess) {
+            owner = 0;
+            err = fail(ctx, (int)e, std::string("loading the constants: ") + hipGetErrorString(e));
+            return;
+        }
+        owner = ctx->id;
+        ctx->consts_dirty = false;
+    }
+};
 
 unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -416,6 +479,7 @@ int get_map(icf_ctx *ctx, int target, int fixed_nkeys, const std::vector<SrcKey>
             if (diff) {
                 free_map(m);
                 ctx->maps.erase(it);
+                ctx->epoch++;
                 break;
             }
             m.suspect = false;
@@ -1129,7 +1193,10 @@ int icf_destroy(icf_ctx *ctx)
             if (ctx->lane_ev[l][k]) (void)hipEventDestroy(ctx->lane_ev[l][k]);
     if (ctx->stream_c) (void)hipStreamDestroy(ctx->stream_c);
     (void)hipStreamDestroy(ctx->stream);
-    if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_dev_mu[ctx->device]);
+        if (g_const_owner[ctx->device] == ctx->id) g_const_owner[ctx->device] = 0;
+    }
     delete ctx;
     return 0;
 }
@@ -1212,9 +1279,10 @@ int icf_set_constants(icf_ctx *ctx, const icf_consts *c)
     for (int i = 0; i < 36; i++) d.Ef[i] = (float)d.E[i];
     for (int i = 0; i < 144; i++) d.Mf[i] = (float)d.M[i];
     ctx->has_consts = true;
-    g_const_owner[ctx->device] = 0;  // force re-upload
+    ctx->consts_dirty = true;
     if (int r = check_ctx(ctx)) return r;
-    return upload_consts(ctx);
+    ConstUse cu(ctx);
+    return cu.err;
 }
 
 int icf_set_precision(icf_ctx *ctx, int mode)
@@ -1306,6 +1374,7 @@ int icf_pattern_changed(icf_ctx *ctx)
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_pattern_changed during capture");
     for (SlotMap &m : ctx->maps) m.suspect = true;
     ctx->adj.valid = false;
+    ctx->epoch++;
     return 0;
 }
 
@@ -1317,7 +1386,7 @@ int icf_d2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t src
     if (int r = check_ctx(ctx)) return r;
     if (!bytes) return 0;
     char *d = (char *)dst + dst_off;
-    if (!ctx->capturing) invalidate_range(ctx, (uintptr_t)d, (uintptr_t)d + bytes);
+    if (int r = note_write(ctx, d, bytes, "icf_d2d")) return r;
     OnLane ln(ctx, 0, [&] { return Access().rd((const char *)src + src_off, bytes).wr(d, bytes); });
     if (ln.err) return ln.err;
     Timer tm(ctx);
@@ -1351,6 +1420,8 @@ struct icf_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = 0;
+    int ctx_id = 0;                  // the context that captured it ...
+    unsigned long long epoch = 0;    // ... and its epoch at icf_graph_end
 };
 
 int icf_graph_begin(icf_ctx *ctx)
@@ -1382,6 +1453,8 @@ int icf_graph_end(icf_ctx *ctx, icf_graph **out)
     gr->graph = g;
     gr->exec = ex;
     gr->device = ctx->device;
+    gr->ctx_id = ctx->id;
+    gr->epoch = ctx->epoch;
     *out = gr;
     return 0;
 }
@@ -1391,7 +1464,14 @@ int icf_graph_launch(icf_ctx *ctx, icf_graph *g)
     if (int r = check_ctx(ctx)) return r;
     if (!g || g->device != ctx->device) return fail(ctx, ICF_E_INVALID, "icf_graph_launch: bad graph");
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_graph_launch during capture");
-    if (int r = upload_consts(ctx)) return r;
+    if (g->ctx_id != ctx->id)
+        return fail(ctx, ICF_E_CAPTURE, "icf_graph_launch: the graph was captured on another context");
+    if (g->epoch != ctx->epoch)
+        return fail(ctx, ICF_E_CAPTURE,
+                    "icf_graph_launch: the graph is stale (since its capture, memory it uses was freed or moved, or "
+                    "slot fields it was built from were written): capture it again");
+    ConstUse cu(ctx);
+    if (cu.err) return cu.err;
     OnLane ln(ctx, 0, everything);
     if (ln.err) return ln.err;
     Timer tm(ctx);
@@ -1513,7 +1593,8 @@ int icf_elem_force(icf_ctx *ctx, const int *ie, double *dn, double h, double *A,
     if (int r = check_ctx(ctx)) return r;
     if (bad_units(nElems, el_stride) || nd_stride < 0 || (nElems && (!ie || !dn || !A || !b)) || !(h > 0))
         return fail(ctx, ICF_E_INVALID, "icf_elem_force: invalid argument");
-    if (int r = upload_consts(ctx)) return r;
+    ConstUse cu(ctx);
+    if (cu.err) return cu.err;
     if (nElems == 0) return 0;
     OnLane ln(ctx, 0, everything);
     if (ln.err) return ln.err;
@@ -1540,7 +1621,8 @@ int icf_cz_force(icf_ctx *ctx, double *dcz, int *icz, double *dn, double *A, dou
     if (int r = check_ctx(ctx)) return r;
     if (bad_units(nCZs, cz_stride) || nd_stride < 0 || (nCZs && (!dcz || !icz || !dn || !A || !b)))
         return fail(ctx, ICF_E_INVALID, "icf_cz_force: invalid argument");
-    if (int r = upload_consts(ctx)) return r;
+    ConstUse cu(ctx);
+    if (cu.err) return cu.err;
     if (nCZs == 0) return 0;
     OnLane ln(ctx, 0, everything);
     if (ln.err) return ln.err;
@@ -1565,6 +1647,7 @@ int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, 
     if (nPairs < 0 || el_all_stride < 0 || nd_stride < 0 || (nPairs && (!dn || !ie || !narrowList)))
         return fail(ctx, ICF_E_INVALID, "icf_narrow_phase: invalid argument");
     if (nPairs == 0) return 0;
+    if (int r = note_write(ctx, narrowList, sizeof(int) * 2 * (size_t)nPairs, "icf_narrow_phase")) return r;
     // reads node positions and the tets' node ids, rewrites the list
     OnLane ln(ctx, contact_lane(ctx), [&] {
         return Access()
@@ -1626,6 +1709,8 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
         (nPairs && (!nl2 || !dn || !ie || !faces)))
         return fail(ctx, ICF_E_INVALID, "icf_find_closest_face: invalid argument");
     if (nPairs == 0) return 0;
+    if (int r = note_write(ctx, nl2 + tet_stride, sizeof(int) * 3 * (size_t)tet_stride, "icf_find_closest_face"))
+        return r;
     // the surface tets' adjacency lists, read from their compact copy
     if (int r = get_adjacency(ctx, ie, el_all_stride)) return r;
     OnLane ln(ctx, contact_lane(ctx), [&] {
@@ -1654,6 +1739,9 @@ int icf_find_closest_face_noalloc(icf_ctx *ctx, int nPairs, int tet_stride, int 
         return fail(ctx, ICF_E_INVALID, "icf_find_closest_face_noalloc: invalid argument");
     if (nPairs == 0) return 0;
     if (!ctx->noalloc) return fail(ctx, ICF_E_INVALID, "icf_find_closest_face_noalloc: no scratch on this device");
+    if (int r = note_write(ctx, nl2 + tet_stride, sizeof(int) * 3 * (size_t)tet_stride,
+                           "icf_find_closest_face_noalloc"))
+        return r;
     const int es = el_all_stride;
     // adjacent-face rows of ie, from the extent of its allocation (as icf_find_closest_face)
     void *base = nullptr;
@@ -1777,7 +1865,8 @@ static int assemble_impl(icf_ctx *ctx, const int *ie, int nElems, int el_stride,
     if (bad_units(nElems, el_stride) || bad_units(nCZs, cz_stride) || nd_stride < 0 || nnzb < 0 || N < 0 ||
         (nElems && !ie) || (nCZs && (!dcz || !icz)) || !dn || (nnzb && !A) || (N && !b) || !(h > 0))
         return fail(ctx, ICF_E_INVALID, "icf_assemble_elems_and_czs: invalid argument");
-    if (int r = upload_consts(ctx)) return r;
+    ConstUse cu(ctx);
+    if (cu.err) return cu.err;
     // node fields read (x0, u, v, a, x) and written (F); CZ records rewritten in place
     const size_t ns = (size_t)nd_stride;
     OnLane ln(ctx, 0, [&] {
@@ -1914,40 +2003,46 @@ int icf_assemble_batched(icf_ctx *ctx, const icf_worklist *wl, const int *ie, in
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         B.off = off;
         B.n = n;
-        B.checked_v = B.checked_r = 0;
+        B.checked.clear();
+        ctx->epoch++;              // a captured batch reads the old offsets
     }
-    int ids[2] = {0, 0};
-    if (int r = assemble_impl(ctx, ie, nE, el_stride, dcz, icz, nC, cz_stride, dn, nd_stride, h, A, nnzb, b, N,
-                              nullptr, nullptr, ids))
-        return r;
     const int *d = (const int *)B.d.p;
     const BatchOff bo{d, d + (n + 1), d + 2 * (n + 1), d + 3 * (n + 1), n};
     int *cnt = (int *)B.d.p + 4 * (n + 1);
+    // The block-diagonal layout, checked before anything is written whenever the units or the
+    // slot fields may have changed (a library write to them moves the epoch).
+    const std::vector<long long> sig{(long long)(uintptr_t)ie, (long long)(uintptr_t)icz, el_stride, cz_stride, nE,
+                                     nC, (long long)ctx->epoch};
+    if (sig != B.checked) {
+        if (ctx->capturing)
+            return fail(ctx, ICF_E_CAPTURE, "icf_assemble_batched: run the batch once uncaptured before capture");
+        if (int r = lane_barrier(ctx)) return r;
+        int *d_err = ctx->small + 1;
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+        if (nE + nC)
+            hipLaunchKernelGGL(k_batch_check, dim3(blocks_for((long long)nE + nC, 256)), dim3(256), 0, ctx->stream, ie,
+                               nE, el_stride, icz, nC, cz_stride, bo, d_err);
+        int e = 0;
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(&e, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (e)
+            return fail(ctx, ICF_E_INDEX,
+                        "icf_assemble_batched: a unit writes a BCSR block or rhs row outside its specimen's ranges");
+    }
+    if (int r = assemble_impl(ctx, ie, nE, el_stride, dcz, icz, nC, cz_stride, dn, nd_stride, h, A, nnzb, b, N,
+                              nullptr, nullptr, nullptr))
+        return r;
+    if (sig != B.checked) {
+        B.checked = sig;
+        B.checked.back() = (long long)ctx->epoch;   // map checks inside the call may have moved it
+    }
     OnLane ln(ctx, 0, [&] {
         return Access()
             .rd(nE ? ie : nullptr, (EL_ROWS + 4) * (size_t)el_stride)
             .rd(nC ? icz : nullptr, (CZI_ROWS + 6) * (size_t)cz_stride);
     });
     if (ln.err) return ln.err;
-    if (ids[0] != B.checked_v || ids[1] != B.checked_r) {
-        // the block-diagonal layout, checked once per pair of maps
-        if (ctx->capturing)
-            return fail(ctx, ICF_E_CAPTURE, "icf_assemble_batched: run the batch once uncaptured before capture");
-        int *d_err = ctx->small + 1;
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->cur, d_err, 1, 0);
-        if (nE + nC)
-            hipLaunchKernelGGL(k_batch_check, dim3(blocks_for((long long)nE + nC, 256)), dim3(256), 0, ctx->cur, ie, nE,
-                               el_stride, icz, nC, cz_stride, bo, d_err);
-        int e = 0;
-        HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(&e, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->cur));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->cur));
-        if (e)
-            return fail(ctx, ICF_E_INDEX,
-                        "icf_assemble_batched: a unit writes a BCSR block or rhs row outside its specimen's ranges");
-        B.checked_v = ids[0];
-        B.checked_r = ids[1];
-    }
     if (nC) {
         const int *dmg = icz + (size_t)cz_stride * CZI_TDAMAGED, *fld = icz + (size_t)cz_stride * CZI_TFAILED;
         LAUNCH(ctx, "k_batch_counts", k_batch_counts, dim3(n), dim3(256), dmg, fld, bo.cz, cnt);
@@ -2235,6 +2330,7 @@ int icf_nl2set(icf_ctx *ctx, const int *masks, const int *pairs, int nPairs, con
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_nl2set returns a host value");
     *nOut = 0;
     if (nPairs == 0) return 0;
+    if (int r = note_write(ctx, nl2, sizeof(int) * 2 * (size_t)tet_stride, "icf_nl2set")) return r;
     if (int r = grow(ctx, ctx->bp_i, sizeof(int) * 2 * ((size_t)nPairs + 1))) return r;
     int *cnt = (int *)ctx->bp_i.p, *off = cnt + nPairs + 1;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, cnt + nPairs, 1, 0);

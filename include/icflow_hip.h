@@ -18,7 +18,10 @@
  *
  * All functions return 0 on success, otherwise a positive hipError_t value or a negative
  * ICF_E_* code; icf_last_error_string() describes the last failure on that context.
- * A context is not re-entrant: one host thread at a time.
+ * A context is not re-entrant: one host thread at a time.  Contexts on one device may be used
+ * from different threads: the constants live in one per-device block, and a call that needs
+ * them holds a per-device lock from loading its context's constants (after all queued device
+ * work, when another context's were loaded) to its last launch.
  */
 #ifndef ICFLOW_HIP_H
 #define ICFLOW_HIP_H
@@ -190,7 +193,14 @@ double icf_last_device_ms(const icf_ctx *ctx);
  * capture again when any of them changes.  The slot maps of the captured calls must already
  * exist (run the iteration once uncaptured); calls that would build one, grow an internal
  * buffer, or copy to or from the host return ICF_E_CAPTURE during capture.  icf_d2d is
- * allowed and is recorded as a copy kernel. */
+ * allowed and is recorded as a copy kernel, except into the slot / row / node-id fields that
+ * a slot map was built from (ICF_E_CAPTURE: each replay would change them behind the maps).
+ * icf_graph_launch refuses (ICF_E_CAPTURE) a graph whose memory may be stale: one captured
+ * on another context, or one captured before the context freed or moved memory it owns
+ * (an internal buffer grew, a slot map was rebuilt or dropped, the adjacency lists were
+ * dropped, new batch offsets), or before a write to fields a slot map was built from
+ * (icf_h2d, icf_memset, icf_d2d, icf_pack_pcsr, the contact kernels' in-place outputs,
+ * icf_free, icf_pattern_changed).  Capture it again then. */
 typedef struct icf_graph icf_graph;
 int icf_set_blocking(icf_ctx *ctx, int blocking);
 int icf_cz_counts(icf_ctx *ctx, int *nCZDamaged, int *nCZFailed);

@@ -97,12 +97,17 @@ def test_every_chunk_shape_gives_the_same_bits(specimen_consts, precision):
         shape, tuning, best = state[r]
         assert not tuning and best[shape] > 0, (r, state[r])
     _same(_iteration(p), ref, "stopped")
-    # the tuned shapes carry over to a new context through JSON
+    # the tuned shapes carry over to a new context through JSON: a different shape per role,
+    # so that a role taken for another shows; the new context assembles the same bits with them
     import json
+    for i, (keys, stg) in enumerate(((64, 2), (512, 3), (128, 3), (256, 2))):
+        gf.set_chunk_keys(i >> 1, i & 1, keys, stg)
     saved = json.loads(json.dumps(gf.tuned_shapes()))
+    assert len({tuple(v) for v in saved.values()}) == 4
     gf.close()
-    from icflow_amd.gpu_functionality import GPUFunctionality
-    g2 = GPUFunctionality(0)
-    g2.use_shapes(saved)
-    assert g2.tuned_shapes() == saved
-    g2.close()
+    p2 = Pipeline(col, specimen_consts, st, precision=precision, contact=True)
+    p2.gf.use_shapes(saved)
+    for r, (shape, tuning, _) in p2.gf.autotune_state().items():
+        assert list(shape) == saved[r] and not tuning, (r, shape)
+    _same(_iteration(p2), ref, "restored shapes")
+    p2.gf.close()
