@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
 """Newton-iteration assembly throughput of libicflow_hip.so on one MI355X.
 
-Workload (default): uniaxial compression with contact -- a 0.1 m cube as a 40x40x40 hex grid
-split into 6 tets per hex (384,000 tets), 500 Voronoi grains (seed 1) with PPR cohesive zones
-on every inter-grain face, and two rigid one-cell-thick tet platens (32x32 cells) pressed
-0.3 cell into the cube.  Material E=9 GPa, nu=0.3, rho=916; PPR sigma_max=1 MPa,
-tau_max=2 MPa, phi=10 J/m^2, alpha=beta=4, lambda=0.02.  Synthetic seeded state.
+Headline workload (default, --workload batched): 256 independent cubes with lognormal tet
+counts (median 12k, truncated to 2k-40k, log-sd 0.6, seed 2), 8-64 Voronoi grains each, PPR
+cohesive zones on every inter-grain face, no contact: 3.5M tets and 0.8M CZs assembled per
+Newton iteration by icf_assemble_batched, one launch per phase over all specimens.
 
-One step = one Newton iteration's device work with every input already in HBM:
-narrow phase over the broad-phase candidate pairs, closest face for the (node, tet) tuples,
-the fused element + CZ assembly (AssembleElemsAndCZs) and the contact response.  The in-place
-inputs of the two narrow-phase kernels are restored by device-to-device copies before each
-step.  Throughput = (tets + active CZs + impacts) / step time.
+Further workloads (--extra, reported under "extra_workloads"; --workload selects one of them
+as the headline instead):
+  compression: a 0.1 m cube as a 40x40x40 hex grid split into 6 tets per hex (384,000 tets),
+    500 Voronoi grains (seed 1), two rigid one-cell-thick tet platens (32x32 cells) pressed 0.3
+    cell into the cube; contact: narrow phase over the broad-phase candidate pairs, closest
+    face, the fused assembly and the contact response.  The in-place inputs of the two
+    narrow-phase kernels are restored by device-to-device copies before each step.
+  large: the same with a 0.2 m cube, 60^3 hex x 6 tets (1,296,000 tets), 2,000 grains
+    (seed 3), 48x48 platens.
+Material E=9 GPa, nu=0.3, rho=916; PPR sigma_max=1 MPa, tau_max=2 MPa, phi=10 J/m^2,
+alpha=beta=4, lambda=0.02.  Synthetic seeded states.
+
+One step = one Newton iteration's device work with every input already in HBM.
+Throughput = (tets + active CZs + impacts) / step time.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -428,30 +436,108 @@ def broad_phase_timing(r, reps=5):
     return {"pairs": int(n.value), "same_as_host": same, "ms_per_call": round(ms, 3)}
 
 
-def cpu_baseline(runner, seconds):
-    """The CPU oracle (plain C, one core) over the same iteration's element, CZ and contact
-    passes, repeated for about `seconds`."""
+def host_threads():
+    """Threads for the all-core CPU baseline: OMP_NUM_THREADS when set (the GPU box sets it to
+    the CPU share of its one GPU), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(runner, seconds, units):
+    """The oracle's element, CZ and contact passes of the same iteration on the host
+    (orc_assemble_par: OpenMP over units, compare-and-swap scatter additions as the reference's
+    atomicAdd2; A and b zeroed inside, as the fused device call does), timed on all threads of
+    this process's CPU share and on one thread.  Buffers are allocated once, outside the timed
+    passes.  Units are the device value's: tets + active CZs + impacts."""
     from oracle import oracle as orc
     gf, col = runner.gf, runner.col
     orc.set_constants(gf.consts)
     T, C, P = len(col.elastic), len(col.czs), runner.nT
-    passes, t0 = 0, time.perf_counter()
-    while True:
-        dn = gf.c_dn.copy()
-        A = np.zeros(gf.csr.nnz * 9)
-        b = np.zeros(gf.csr.N * 3)
-        ie, icz, dcz = gf.c_ie_pcsr.copy(), gf.c_icz.copy(), gf.c_dcz.copy()
-        orc.elem_force(ie, dn, runner.h, A, b, T, gf.el_elastic_stride, gf.nd_stride)
-        orc.cz_force(dcz, icz, dn, A, b, runner.h, C, gf.cz_stride, gf.nd_stride)
-        orc.collision_response(gf.c_icr, dn, runner.h, A, b, P, gf.cr_stride, gf.nd_stride, runner.k, runner.eps)
-        passes += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or passes >= 50:
-            break
-    units = T + C + P
-    return dict(value=round(units * passes / el / 1e6, 4), unit="Munits/s", cores=1, kind="port",
-                sample=f"{passes} full passes of the same iteration (elements, CZs, contacts) through the "
-                       f"C oracle, single thread, {el:.1f} s")
+    dn = gf.c_dn.copy()
+    A = np.zeros(gf.csr.nnz * 9)
+    b = np.zeros(gf.csr.N * 3)
+    ie = gf.c_ie_pcsr
+    icz0 = gf.c_icz.copy() if C else None
+    dcz0 = gf.c_dcz.copy() if C else None
+    icz = icz0.copy() if C else None
+    dcz = dcz0.copy() if C else None
+    icr = getattr(gf, "c_icr", None) if P else None
+    cr = getattr(gf, "cr_stride", 0) if P else 0
+    k = getattr(runner, "k", 0.0)
+    eps = getattr(runner, "eps", 0.0)
+
+    def one_pass(threads):
+        orc.assemble_par(ie, T, gf.el_elastic_stride, dcz, icz, C, gf.cz_stride, icr, P, cr, dn, gf.nd_stride,
+                         runner.h, k, eps, A, b, threads)
+
+    def rate(threads, budget, warm):
+        if warm:
+            one_pass(threads)                   # caches and the thread pool
+        passes, t0 = 0, time.perf_counter()
+        while True:
+            one_pass(threads)
+            passes += 1
+            el = time.perf_counter() - t0
+            if el >= budget or passes >= 50:
+                return units * passes / el / 1e6, passes, el
+
+    n = host_threads()
+    v_all, p_all, t_all = rate(n, seconds / 3, True)
+    v_one, p_one, t_one = rate(1, 2 * seconds / 3, False)
+    return dict(value=round(v_all, 4), unit="Munits/s", cores=n, kind="port",
+                value_1core=round(v_one, 4), nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
+                cpu_model=cpu_model(),
+                sample=(f"the whole iteration (elements, CZs, contacts: {units} units) through the C oracle's "
+                        f"OpenMP pass with CAS scatter: {p_all} passes on {n} threads in {t_all:.1f} s, "
+                        f"{p_one} on 1 thread in {t_one:.1f} s; the reference's own C# + MKL host cannot run here"))
+
+
+def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
+    """A further workload beside the headline: autotuned, both modes timed (graph replays and
+    eager non-blocking calls, the faster kept), the dominant kernel's algorithmic HBM rate from
+    a HIP-event pass."""
+    from icflow_amd import _lib
+    r = Runner(consts, col, st, _lib.ICF_MIXED, contact=contact)
+    res = {}
+    for mname, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
+        r.set_mode(mode)
+        autotune(r, max(1, args.tune_rounds - 1), mname)
+        t = {how: timed(r, args.steps, args.warmup, how) for how in ("async", "graph", "profile")}
+        issue = min(("async", "graph"), key=lambda h: t[h][0])
+        res[mname] = (issue, t[issue][0], t["profile"][1])
+    T, C, P = len(col.elastic), len(col.czs), r.nT
+    units = T + C - r.failed_czs() + P
+    alg = algorithmic_bytes(r)
+    issue, sec, prof = res["mixed"]
+    dname, (dms, dn) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_s = dms / dn / 1e3
+    dbytes = kernel_bytes(alg, dname) or 0
+    dkeys = key_limit(r.gf, dname)
+    traffic = profiled_traffic(dname, dkeys, traffic_md) if traffic_md else None
+    out = {"workload": name, "tets": T, "czs": C, "active_czs": C - r.failed_czs(), "impacts": P,
+           "candidate_pairs": r.Q, "nnzb": r.gf.csr.nnz, "rows": r.gf.csr.N,
+           "value": round(units / sec / 1e6, 3), "ms_per_step": round(sec * 1e3, 4), "issue": issue,
+           "fp64": {"value": round(units / res["fp64"][1] / 1e6, 3), "ms_per_step": round(res["fp64"][1] * 1e3, 4),
+                    "issue": res["fp64"][0]},
+           "mixed_over_fp64": round(res["fp64"][1] / sec, 3),
+           "iteration_roofline_frac": round(iteration_bytes(r) / sec / 1e9 / HBM_PEAK_GBS, 4),
+           "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
+           "roofline": {"kernel": dname, "alg_bytes": int(dbytes), "avg_us": round(avg_s * 1e6, 2),
+                        "achieved": round(dbytes / avg_s / 1e9, 1),
+                        "frac": round(dbytes / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": int(traffic) if traffic else None}}
+    r.gf.close()
+    return out
 
 
 def main():
@@ -468,9 +554,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="precision modes to time (fp64, mixed)")
-    ap.add_argument("--workload", default="compression", choices=["compression", "batched"],
-                    help="compression: the 40^3 cube with platens and contact; batched: --specimens cubes, "
-                         "lognormal sizes (seed 2), no contact")
+    ap.add_argument("--workload", default="batched", choices=["compression", "batched"],
+                    help="batched: --specimens cubes, lognormal sizes (seed 2), no contact; compression: the "
+                         "--cells^3 cube with platens and contact")
+    ap.add_argument("--extra", default="compression",
+                    help="comma list of further workloads timed after the headline: compression (40^3 cube with "
+                         "contact), large (60^3 cube, 2000 grains, 48x48 platens); empty: none")
     ap.add_argument("--specimens", type=int, default=256)
     ap.add_argument("--newton-damage-steps", type=int, default=26,
                     help="simulated time (in 1e-4 s steps) of the FP64-vs-mixed three-point-bend run into fracture "
@@ -568,22 +657,27 @@ def main():
     avg_s = dms / dn / 1e3
     achieved = dbytes / avg_s / 1e9
     dkeys = key_limit(r.gf, dname)
-    # the committed counter summary covers the default compression workload only
-    profiled = contact and (args.cells, args.grains, args.seed, args.size, args.platen_cells) == (40, 500, 1, 0.1, 32)
-    traffic = profiled_traffic(dname, dkeys) if profiled else None
+    # the committed counter summaries cover the default workloads only
+    if contact:
+        profiled = (args.cells, args.grains, args.seed, args.size, args.platen_cells) == (40, 500, 1, 0.1, 32)
+        md = os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")
+    else:
+        profiled = args.specimens == 256
+        md = os.path.join(ROOT, "profiles", "assembly_batched256_kernels.md")
+    traffic = profiled_traffic(dname, dkeys, md) if profiled else None
     roof = dict(kernel=dname + (f" ({dkeys[0]}-lane workgroups, {dkeys[1]} staged per key)" if dkeys else ""),
                 bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=int(traffic) if traffic else None,
                 alg_bytes=int(dbytes), avg_us=round(avg_s * 1e6, 2),
-                traffic_source="profiles/assembly_compression40_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                               "copy-calibrated)" if traffic else None)
+                traffic_source=(os.path.relpath(md, ROOT) + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                                if traffic else None))
     it_bytes = iteration_bytes(r)
     # every kernel of the step against HBM peak: measured traffic (committed counter summary,
     # default workload only) and algorithmic bytes over its HIP-event duration
     util = {}
     for kname, (ms, n) in prof.items():
         us = 1e3 * ms / max(n, 1)
-        tb = profiled_traffic(kname, key_limit(r.gf, kname)) if profiled else None
+        tb = profiled_traffic(kname, key_limit(r.gf, kname), md) if profiled else None
         ab = kernel_bytes(alg, kname)
         util[kname] = dict(avg_us=round(us, 2),
                            traffic_frac=round(tb / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if tb else None,
@@ -626,7 +720,7 @@ def main():
                                                        max_attempts=args.newton_contact_attempts)
         study_contact["wall_s"] = round(time.time() - t0, 1)
         log(f"precision study with contact: {study_contact}")
-    cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds)
+    cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds, units)
     if cpu:
         log(f"cpu baseline: {cpu}")
     line = {
@@ -674,8 +768,30 @@ def main():
         "precision_study_contact": study_contact,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
     r.gf.close()
+    extra = {}
+    for ex in [e for e in args.extra.split(",") if e]:
+        if ex == ("compression" if contact else "batched"):
+            continue
+        t0 = time.time()
+        if ex == "compression":
+            c2, col2, st2 = build_workload(40, 500, 1, 0.3, 0.1, 32)
+            extra[ex] = measure_extra("compression with contact: 0.1 m cube, 40^3 hex x 6 tets, 500 grains (seed 1), "
+                                      "2 rigid 32x32 platens", c2, col2, st2, True, args,
+                                      os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md"))
+        elif ex == "large":
+            c2, col2, st2 = build_workload(60, 2000, 3, 0.3, 0.2, 48)
+            extra[ex] = measure_extra("compression with contact: 0.2 m cube, 60^3 hex x 6 tets, 2000 grains (seed 3), "
+                                      "2 rigid 48x48 platens", c2, col2, st2, True, args)
+        elif ex == "batched":
+            c2, col2, st2 = build_batched(args.specimens, 2)
+            extra[ex] = measure_extra(f"batched: {args.specimens} cubes", c2, col2, st2, False, args)
+        else:
+            raise SystemExit(f"unknown workload {ex}")
+        extra[ex]["wall_s"] = round(time.time() - t0, 1)
+        log(f"extra workload {ex}: {extra[ex]}")
+    line["extra_workloads"] = extra or None
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -22,6 +22,28 @@ static orc_consts C;
 
 void orc_set_constants(const orc_consts *c) { C = *c; }
 
+/* Scatter addition.  Sequential calls add in unit order.  Inside orc_assemble_par the units
+ * run on several threads and every addition into A, b and the node forces is a
+ * compare-and-swap loop on the 64-bit pattern of the double, as the reference's device
+ * scatter does (atomicAdd2, elem_kernels9/atomic_add.h:1-13). */
+static int g_par = 0;
+
+static void acc(double *p, double v)
+{
+    if (!g_par) {
+        *p += v;
+        return;
+    }
+    unsigned long long *q = (unsigned long long *)p;
+    unsigned long long old = __atomic_load_n(q, __ATOMIC_RELAXED), nw;
+    do {
+        double d;
+        memcpy(&d, &old, sizeof d);
+        d += v;
+        memcpy(&nw, &d, sizeof nw);
+    } while (!__atomic_compare_exchange_n(q, &old, nw, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+}
+
 /* node-array field offsets: elem_kernels9/element_kernels2.h:8-12, cz_kernels.h:4-10 */
 enum { X0 = 0, UN = 3, VN = 6, AN = 9, FF = 12, XC = 15 };
 
@@ -159,20 +181,20 @@ static void scatter4(const int *pcsr, int stride, double *A, double *b, const do
             int slot = pcsr[stride * (i * 4 + j)];
             if (slot >= 0)
                 for (int k = 0; k < 3; k++)
-                    for (int l = 0; l < 3; l++) A[slot * 9 + 3 * k + l] += LHS[(i * 3 + k) * 12 + j * 3 + l];
+                    for (int l = 0; l < 3; l++) acc(&A[slot * 9 + 3 * k + l], LHS[(i * 3 + k) * 12 + j * 3 + l]);
         }
         int row = pcsr[stride * (16 + i)];
         if (row >= 0)
-            for (int k = 0; k < 3; k++) b[row * 3 + k] += rhs[i * 3 + k];
+            for (int k = 0; k < 3; k++) acc(&b[row * 3 + k], rhs[i * 3 + k]);
     }
 }
 
 /* Newmark effective tangent and residual per tet, scattered.
  * Restates kelElementElasticityForce, elem_kernels9/element_kernels2.h:195-263. */
-void orc_elem_force(const int *ie, double *dn, double h, double *A, double *b,
-                    int nElems, int el_stride, int nd_stride)
+static void elem_unit(int u, const int *ie, double *dn, double h, double *A, double *b, int el_stride,
+                      int nd_stride)
 {
-    for (int u = 0; u < nElems; u++) {
+    {
         double x0[12], un[12], vn[12], an[12];
         int nid[4];
         for (int i = 0; i < 4; i++) {
@@ -204,9 +226,15 @@ void orc_elem_force(const int *ie, double *dn, double h, double *A, double *b,
         }
         /* node force (element_kernels2.h:256-260), then BCSR (:262) */
         for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 3; j++) dn[nid[i] + nd_stride * (FF + j)] += rhs[i * 3 + j];
+            for (int j = 0; j < 3; j++) acc(&dn[nid[i] + nd_stride * (FF + j)], rhs[i * 3 + j]);
         scatter4(&ie[u + el_stride * 4], el_stride, A, b, LHS, rhs);
     }
+}
+
+void orc_elem_force(const int *ie, double *dn, double h, double *A, double *b,
+                    int nElems, int el_stride, int nd_stride)
+{
+    for (int u = 0; u < nElems; u++) elem_unit(u, ie, dn, h, A, b, el_stride, nd_stride);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -410,12 +438,10 @@ enum { CZ_FAILED = 0, CZ_TCONTACT = 1, CZ_TDAMAGED = 2, CZ_TFAILED = 3, CZ_VRTS 
 
 /* 6-node zero-thickness CZ, 3 Gauss points.  Restates kczCZForce, cz_kernels.h:210-434
  * (with AssembleBCSR18, :177-203). */
-void orc_cz_force(double *dcz, int *icz, double *dn, double *A, double *b, double h,
-                  int nCZs, int cz_stride, int nd_stride)
+static void cz_unit(int u, double *dcz, int *icz, double *dn, double *A, double *b, int cz_stride, int nd_stride)
 {
-    (void)h;
-    for (int u = 0; u < nCZs; u++) {
-        if (icz[u + cz_stride * CZ_FAILED] != 0) continue;    /* :219 */
+    {
+        if (icz[u + cz_stride * CZ_FAILED] != 0) return;      /* :219 */
         double xc[18], xr[18], pmax[3], tmax[3];
         for (int i = 0; i < 6; i++) {
             int v = icz[u + cz_stride * (CZ_VRTS + i)];
@@ -530,7 +556,7 @@ void orc_cz_force(double *dcz, int *icz, double *dn, double *A, double *b, doubl
         icz[u + cz_stride * CZ_TCONTACT] = contact ? 1 : 0;
         icz[u + cz_stride * CZ_TFAILED] = failed ? 1 : 0;
         icz[u + cz_stride * CZ_TDAMAGED] = failed ? 0 : damaged;
-        if (failed) continue;                              /* :424 */
+        if (failed) return;                                /* :424 */
 
         const int *pcsr = &icz[u + cz_stride * CZ_PCSR];   /* AssembleBCSR18, :177-203 */
         for (int i = 0; i < 6; i++) {
@@ -538,17 +564,24 @@ void orc_cz_force(double *dcz, int *icz, double *dn, double *A, double *b, doubl
                 int slot = pcsr[cz_stride * (i * 6 + j)];
                 if (slot >= 0)
                     for (int k = 0; k < 3; k++)
-                        for (int l = 0; l < 3; l++) A[slot * 9 + 3 * k + l] += Keff[(i * 3 + k) * 18 + j * 3 + l];
+                        for (int l = 0; l < 3; l++) acc(&A[slot * 9 + 3 * k + l], Keff[(i * 3 + k) * 18 + j * 3 + l]);
             }
             int row = pcsr[cz_stride * (36 + i)];
             if (row >= 0)
-                for (int k = 0; k < 3; k++) b[row * 3 + k] += rhs[i * 3 + k];
+                for (int k = 0; k < 3; k++) acc(&b[row * 3 + k], rhs[i * 3 + k]);
         }
         dcz[u + cz_stride * CZ_DN] = avgDn;                /* :430-433 */
         dcz[u + cz_stride * CZ_DT] = avgDt;
         dcz[u + cz_stride * CZ_TN] = avgTn;
         dcz[u + cz_stride * CZ_TT] = avgTt;
     }
+}
+
+void orc_cz_force(double *dcz, int *icz, double *dn, double *A, double *b, double h,
+                  int nCZs, int cz_stride, int nd_stride)
+{
+    (void)h;
+    for (int u = 0; u < nCZs; u++) cz_unit(u, dcz, icz, dn, A, b, cz_stride, nd_stride);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -950,15 +983,14 @@ double orc_pt(const double x[12], double fd[12], double sd[144], double *so, dou
 
 /* Penalty response of one impact (point + triangle).  Restates kCollisionResponseForce,
  * collision_kernels9/collision_response.h:38-126. */
-void orc_collision_response(const int *icr, double *dn, double h, double *A, double *b,
-                            int nImpacts, int cr_stride, int nd_stride, double k, double distEps)
+static void contact_unit(int u, const int *icr, double *dn, double *A, double *b, int cr_stride, int nd_stride,
+                         double k, double distEps)
 {
-    (void)h;
-    for (int u = 0; u < nImpacts; u++) {
+    {
         const int *pcsr = &icr[u + cr_stride * 4];
         int anyActive = 0;
         for (int i = 0; i < 4; i++) if (pcsr[cr_stride * (16 + i)] >= 0) anyActive = 1;
-        if (!anyActive) continue;                          /* :48-51 */
+        if (!anyActive) return;                            /* :48-51 */
         double x[12];
         int nid[4];
         for (int i = 0; i < 4; i++) {
@@ -970,7 +1002,7 @@ void orc_collision_response(const int *icr, double *dn, double h, double *A, dou
         int br;
         double dsq = orc_pt(x, fd, sd, &s, &t, &br);
         double w[3] = {1 - (s + t), s, t};
-        if (!(dsq > distEps * distEps)) continue;
+        if (!(dsq > distEps * distEps)) return;
         double fx = k * 0.5 * fd[0], fy = k * 0.5 * fd[1], fz = k * 0.5 * fd[2];
         double fi[12] = {-fx, -fy, -fz,
                          w[0] * fx, w[0] * fy, w[0] * fz,
@@ -980,9 +1012,41 @@ void orc_collision_response(const int *icr, double *dn, double h, double *A, dou
         for (int i = 0; i < 12; i++)
             for (int j = i; j < 12; j++) H[i * 12 + j] = H[j * 12 + i] = k * sd[i * 12 + j] / 2;
         for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 3; j++) dn[nid[i] + nd_stride * (FF + j)] += fi[i * 3 + j];
+            for (int j = 0; j < 3; j++) acc(&dn[nid[i] + nd_stride * (FF + j)], fi[i * 3 + j]);
         scatter4(pcsr, cr_stride, A, b, H, fi);
     }
+}
+
+void orc_collision_response(const int *icr, double *dn, double h, double *A, double *b,
+                            int nImpacts, int cr_stride, int nd_stride, double k, double distEps)
+{
+    (void)h;
+    for (int u = 0; u < nImpacts; u++) contact_unit(u, icr, dn, A, b, cr_stride, nd_stride, k, distEps);
+}
+
+/* One Newton iteration's assembly on `threads` host threads (OpenMP): A[0..nA) and b[0..nb)
+ * zeroed, then the element, CZ and contact passes with their units spread over the threads
+ * and every scatter addition a compare-and-swap (see acc).  The order of the additions is
+ * arbitrary, as on the reference's device; the CPU baseline times this. */
+void orc_assemble_par(const int *ie, int nElems, int el_stride, double *dcz, int *icz, int nCZs, int cz_stride,
+                      const int *icr, int nImpacts, int cr_stride, double *dn, int nd_stride, double h, double k,
+                      double distEps, double *A, long long nA, double *b, long long nb, int threads)
+{
+    g_par = 1;
+#pragma omp parallel num_threads(threads)
+    {
+#pragma omp for schedule(static)
+        for (long long i = 0; i < nA; i++) A[i] = 0;
+#pragma omp for schedule(static)
+        for (long long i = 0; i < nb; i++) b[i] = 0;
+#pragma omp for schedule(dynamic, 256)
+        for (int u = 0; u < nElems; u++) elem_unit(u, ie, dn, h, A, b, el_stride, nd_stride);
+#pragma omp for schedule(dynamic, 64) nowait
+        for (int u = 0; u < nCZs; u++) cz_unit(u, dcz, icz, dn, A, b, cz_stride, nd_stride);
+#pragma omp for schedule(dynamic, 64)
+        for (int u = 0; u < nImpacts; u++) contact_unit(u, icr, dn, A, b, cr_stride, nd_stride, k, distEps);
+    }
+    g_par = 0;
 }
 
 /* ---------------------------------------------------------------------------------------

@@ -68,6 +68,13 @@ void   orc_collision_response(const int *icr, double *dn, double h, double *A, d
 /* Node.InferTentativeValues / InferTentativeUVA (Node.cs:79-97) for every node, stored the way
  * GPU_Functionality.TransferNodesToGPU packs them (GPU_Functionality.cs:307-340): un, vn, an,
  * x_current = x0 + un, F = 0 (x0 is read from dn). */
+/* The element, CZ and contact passes of one iteration on `threads` OpenMP threads, with
+ * compare-and-swap scatter additions (the CPU baseline; results equal the sequential passes up
+ * to the order of the additions). */
+void orc_assemble_par(const int *ie, int nElems, int el_stride, double *dcz, int *icz, int nCZs, int cz_stride,
+                      const int *icr, int nImpacts, int cr_stride, double *dn, int nd_stride, double h, double k,
+                      double distEps, double *A, long long nA, double *b, long long nb, int threads);
+
 void orc_nodes_infer(const double *st, int st_stride, double *dn, int nd_stride, int n, double h,
                      double beta, double gamma);
 /* ImplicitModel3._XtoDU (ImplicitModel3.cs:247-256): du += dx[3 altId + k] for altId >= 0. */

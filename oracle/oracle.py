@@ -50,6 +50,9 @@ _SIGS = {
     "orc_pt": (c_double, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "orc_collision_response": (None, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int, c_int, c_int,
                                       c_double, c_double]),
+    "orc_assemble_par": (None, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                c_void_p, c_int, c_double, c_double, c_double, c_void_p, ctypes.c_longlong, c_void_p,
+                                ctypes.c_longlong, c_int]),
     "orc_nodes_infer": (None, [c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double]),
     "orc_nodes_add_dx": (None, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "orc_nodes_accept": (None, [c_void_p, c_int, c_void_p, c_int, c_int, c_double]),
@@ -187,6 +190,14 @@ def pt(x12):
     br = np.zeros(1, dtype=np.int32)
     d = load().orc_pt(_p(x12), _p(fd), _p(sd), _p(s), _p(t), _p(br))
     return d, fd, sd, s[0], t[0], int(br[0])
+
+
+def assemble_par(ie, nE, es, dcz, icz, nC, cs, icr, nP, crs, dn, ns, h, k, eps, A, b, threads):
+    """The iteration's element, CZ and contact passes on `threads` OpenMP threads with
+    compare-and-swap scatter (A and b are zeroed first); dcz/icz/icr may be None when their
+    counts are 0."""
+    load().orc_assemble_par(_p(ie), nE, es, _p(dcz), _p(icz), nC, cs, _p(icr), nP, crs, _p(dn), ns, h, k, eps,
+                            _p(A), A.size, _p(b), b.size, threads)
 
 
 def nodes_infer(st, ss, dn, ns, n, h, beta, gamma):
