@@ -108,6 +108,8 @@ _SIGS = {
     "icfh_nl2set": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int, P_int]),
     "icfh_kdop_pairs": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_longlong,
                                 ctypes.POINTER(ctypes.c_longlong)]),
+    "SolveDouble3": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                             c_int]),
     "icf_nodes_infer": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double]),
     "icf_nodes_add_dx": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "icf_nodes_accept": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double]),

@@ -28,7 +28,7 @@ def _hipcc():
 
 
 def build_library(verbose=False):
-    srcs = [os.path.join(CSRC, "icf_lib.hip"), os.path.join(CSRC, "icf_host.cpp")]
+    srcs = [os.path.join(CSRC, "icf_lib.hip"), os.path.join(CSRC, "icf_host.cpp"), os.path.join(CSRC, "icf_solve.cpp")]
     tmp = OUT + ".tmp"
     cmd = [_hipcc(), *HIP_FLAGS, "-o", tmp, *srcs]
     if verbose:

@@ -134,6 +134,32 @@ class UpperBcsrSolver:
         return solve_upper_bcsr(self.rows, self.cols, vals, rhs)
 
 
+class SolveDouble3Solver:
+    """LinearSystem.Solve (LinearSystem.cs:45-64) through the library's SolveDouble3 with the
+    reference's arguments (ja = block columns, ia = block row starts, mtype -2, iparam4 0,
+    dim 3, msglvl 0, check 0).  SolveDouble3 is a sparse LDL^T, not MKL PARDISO; it raises, as
+    the reference does on a nonzero PARDISO error."""
+
+    def __init__(self, rows, cols):
+        self.lib = _lib.load()
+        self.rows = np.ascontiguousarray(rows, np.int32)
+        self.cols = np.ascontiguousarray(cols, np.int32)
+        self.N = len(self.rows) - 1
+
+    def solve(self, vals, rhs):
+        a = np.ascontiguousarray(vals, np.float64)
+        b = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros(3 * self.N)
+        r = self.lib.SolveDouble3(_lib.ptr(self.cols), _lib.ptr(self.rows), _lib.ptr(a), self.N, _lib.ptr(b),
+                                  _lib.ptr(x), -2, 0, 3, 0, 0)
+        if r != 0:
+            raise RuntimeError(f"SolveDouble3 error {r}")
+        return x
+
+
+SOLVERS = {"SolveDouble3": SolveDouble3Solver, "banded": UpperBcsrSolver}
+
+
 def adjust_time_step(tsf, hold, explodes, diverges, converged, max_iter):
     """_adjustTimeStep (ImplicitModel3.cs:258-288) and the acceptance rule of Step (:415-420)
     for an attempt run at time-scale factor tsf with hold count hold.  Returns (accepted,
@@ -161,8 +187,9 @@ class ImplicitModel:
     of the anchored SurfaceFragments' dx, dy, dz with their application time)."""
 
     def __init__(self, col, consts, precision=_lib.ICF_FP64, step=None, prescribed=None, prescribed_nodes=None,
-                 contact=False, penalty=None, dist_eps=1e-15):
+                 contact=False, penalty=None, dist_eps=1e-15, solver="SolveDouble3"):
         self.col, self.consts = col, consts
+        self.solver_cls = SOLVERS[solver]
         self.p = step or StepParams()
         self.prescribed = prescribed
         gf = self.gf = GPUFunctionality(0)
@@ -241,7 +268,7 @@ class ImplicitModel:
             self.g_icz.p if C else None, C, self.cs, None, 0, 0), self.ctx, "icf_pack_pcsr")
         self.rows = self.g_rows.download(self.N + 1)
         self.cols = self.g_cols.download(self.nnz)
-        self.solver = UpperBcsrSolver(self.rows, self.cols)
+        self.solver = self.solver_cls(self.rows, self.cols)
         self.g_vals = gf.buf("vals", 9 * self.nnz, np.float64)
         self.g_rhs = gf.buf("rhs", 3 * self.N, np.float64)
         self.g_dx = gf.buf("dx", 3 * self.N, np.float64)
@@ -314,7 +341,7 @@ class ImplicitModel:
         cols = self.g_cols.download(nnz.value)
         if nnz.value != self.nnz or not np.array_equal(rows, self.rows) or not np.array_equal(cols, self.cols):
             self.nnz, self.rows, self.cols = nnz.value, rows, cols
-            self.solver = UpperBcsrSolver(rows, cols)
+            self.solver = self.solver_cls(rows, cols)
             self.g_vals = gf.buf("vals", 9 * self.nnz, np.float64)
 
     def _upload_history(self):

@@ -11,7 +11,7 @@ def declared_functions():
     for h in ("icflow_hip.h", "icflow_host.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        names += re.findall(r"^\s*(?:const\s+)?(?:int|double|char\s*\*|const\s+char\s*\*)\s*\*?\s*(ic\w+)\s*\(",
+        names += re.findall(r"^\s*(?:const\s+)?(?:int|double|char\s*\*|const\s+char\s*\*)\s*\*?\s*([A-Za-z_]\w*)\s*\(",
                             txt, flags=re.M)
     return sorted(set(names))
 
