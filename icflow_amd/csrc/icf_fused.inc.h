@@ -10,25 +10,50 @@
 
 namespace icf {
 
-// Phase 1 of the fused call: tets (workgroups [0, nbt)), then CZs.  Workgroup 0 also clears the
-// two CZ counters that the last launch of the call adds up.
+// Two phases interleaved in one launch: rounds of 8 consecutive workgroups (one per XCD) go to
+// phase A (na workgroups, a multiple of 8) or phase B (nb workgroups), B's rounds spread evenly
+// among A's, so that work that waits on memory and work that keeps the VALUs busy share the
+// CUs for the whole launch instead of one after the other.  The grid has 8 (na/8 + ceil(nb/8))
+// workgroups; `local` is the workgroup's index within its phase (round * 8 + XCD, so each phase
+// keeps its own XCD mapping); B's padding workgroups get local >= nb.
+__host__ __device__ inline int interleave2_grid(int na, int nb) { return na + (nb + 7) / 8 * 8; }
+__device__ inline bool interleave2(int w, int na, int nb, int &local)
+{
+    const long long ra = na / 8, rb = (nb + 7) / 8, R = w / 8;
+    const int x = w % 8;
+    if (ra + rb == 0) {
+        local = w;
+        return false;
+    }
+    const long long b_before = R * rb / (ra + rb), b_upto = (R + 1) * rb / (ra + rb);
+    if (b_upto > b_before) {
+        local = (int)(b_before * 8 + x);
+        return true;
+    }
+    local = (int)((R - b_before) * 8 + x);
+    return false;
+}
+
+// Phase 1 of the fused call: tets (nbt workgroups) interleaved with CZs (nbc).  Workgroup 0 also
+// clears the two CZ counters that the last launch of the call adds up.
 template <bool MIXED, bool ISO>
-__global__ __launch_bounds__(256) void k_phase1_fused(int nbt, const int *__restrict__ ie, const double *__restrict__ dn,
-                                                      int nt, int tp, int es, int ns, double *__restrict__ tet_rhs,
-                                                      void *__restrict__ tet_tan, double *__restrict__ dcz,
-                                                      int *__restrict__ icz, int nc, int cs,
+__global__ __launch_bounds__(256) void k_phase1_fused(int nbt, int nbc, const int *__restrict__ ie,
+                                                      const double *__restrict__ dn, int nt, int tp, int es, int ns,
+                                                      double *__restrict__ tet_rhs, void *__restrict__ tet_tan,
+                                                      double *__restrict__ dcz, int *__restrict__ icz, int nc, int cs,
                                                       double *__restrict__ cz_rhs, void *__restrict__ cz_tan,
                                                       int *__restrict__ counts)
 {
     if (blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = 0;
-    if ((int)blockIdx.x < nbt)
-        tet_phase1<MIXED, ISO>(blockIdx.x, nbt, ie, dn, nt, tp, es, ns, tet_rhs, tet_tan);
-    else
-        cz_phase1<MIXED>(blockIdx.x - nbt, gridDim.x - nbt, dcz, icz, dn, nc, cs, ns, cz_rhs, cz_tan);
+    int local;
+    if (!interleave2(blockIdx.x, nbt, nbc, local))
+        tet_phase1<MIXED, ISO>(local, nbt, ie, dn, nt, tp, es, ns, tet_rhs, tet_tan);
+    else if (local < nbc)
+        cz_phase1<MIXED>(local, nbc, dcz, icz, dn, nc, cs, ns, cz_rhs, cz_tan);
 }
 
-// Node forces with each node's tet sum (k_reduce_vec3<true, true>; workgroups [0, nbf), the
-// short phase first), then the BCSR values of the fused call (overwrite).  Workgroups of KEYS
+// Node forces with each node's tet sum (k_reduce_vec3<true, true>; nbf workgroups) interleaved
+// with the BCSR values of the fused call (overwrite; nchunks workgroups).  Workgroups of KEYS
 // lanes: one key per lane in both phases.
 template <bool MIXED, bool ISO, typename L, int KEYS, int STAGE>
 __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *__restrict__ off,
@@ -40,16 +65,16 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *_
                                                            double *__restrict__ F, int ns, Stage sv,
                                                            double *__restrict__ tsum)
 {
-    if ((int)blockIdx.x < nbf)
-        reduce_vec3<true, true>(blockIdx.x, nbf, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, tsum);
-    else
-        reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(blockIdx.x - nbf, gridDim.x - nbf, off, codes, nullptr, chunks,
-                                                nchunks, A, sv);
+    int local;
+    if (!interleave2(blockIdx.x, nbf, nchunks, local))
+        reduce_vec3<true, true>(local, nbf, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, tsum);
+    else if (local < nchunks)
+        reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(local, nchunks, off, codes, nullptr, chunks, nchunks, A, sv);
 }
 
-// Node forces with the rhs rows (reduce_force_rhs; workgroups [0, nbf)), the CZ counters (the
-// next nbc), then the BCSR values of the fused call (overwrite): the whole second half of the
-// fused call as one launch.  Workgroups of KEYS lanes.
+// The CZ counters (workgroups [0, nbc)), then node forces with the rhs rows (reduce_force_rhs;
+// nbf workgroups) interleaved with the BCSR values of the fused call (overwrite; nchunks
+// workgroups): the whole second half of the fused call as one launch.  Workgroups of KEYS lanes.
 template <bool MIXED, bool ISO, typename L, int KEYS, int STAGE>
 __global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc, const int *__restrict__ off,
         const uint32_t *__restrict__ codes, const int2 *__restrict__ chunks, int nchunks, double *__restrict__ A,
@@ -59,13 +84,13 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc,
         const int *__restrict__ damaged, const int *__restrict__ failed, int nc, int *__restrict__ counts, Stage sv)
 {
     const int w = blockIdx.x;
-    if (w < nbf)
-        reduce_force_rhs(w, nbf, foff, fcodes, fkeys, nfkeys, F, ns, node_row, roff, rcodes, cz_begin, b, sv);
-    else if (w < nbf + nbc)
-        count_flags_n<KEYS>(w - nbf, nbc, damaged, failed, nc, counts);
-    else
-        reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(w - nbf - nbc, gridDim.x - nbf - nbc, off, codes, nullptr, chunks,
-                                                nchunks, A, sv);
+    int local;
+    if (w < nbc)
+        count_flags_n<KEYS>(w, nbc, damaged, failed, nc, counts);
+    else if (!interleave2(w - nbc, nbf, nchunks, local))
+        reduce_force_rhs(local, nbf, foff, fcodes, fkeys, nfkeys, F, ns, node_row, roff, rcodes, cz_begin, b, sv);
+    else if (local < nchunks)
+        reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(local, nchunks, off, codes, nullptr, chunks, nchunks, A, sv);
 }
 
 // rhs rows from the tet sums plus their CZ terms (workgroups [0, nbr)), then the CZ counters.

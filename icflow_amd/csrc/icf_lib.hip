@@ -883,13 +883,13 @@ void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es,
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const int nbt = nt ? round8(blocks_for(nt, 256)) : 0;
     const int nbc = nc ? (int)blocks_for((long long)(nc + CZ_PER_WAVE - 1) / CZ_PER_WAVE * 64, 256) : 0;
-    const dim3 g(std::max(nbt + nbc, 1)), b(256);
+    const dim3 g(std::max(interleave2_grid(nbt, nbc), 1)), b(256);
     double *tr = (double *)ctx->tet_rhs.p, *cr = (double *)ctx->cz_rhs.p;
     void *tt = ctx->tet_tan.p, *ct = ctx->cz_tan.p;
     const int tp = ctx->tplane = tet_plane(nt);
 #define ICF_P1(MX, IS, NAME)                                                                                      \
-    LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS>), g, b, nbt, ie, dn, nt, tp, es, ns, tr, tt, dcz, icz, nc, cs, cr, ct, \
-           counts)
+    LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS>), g, b, nbt, nbc, ie, dn, nt, tp, es, ns, tr, tt, dcz, icz, nc, cs, cr, \
+           ct, counts)
     if (mixed && iso) ICF_P1(true, true, "k_phase1<mixed,iso>");
     else if (mixed) ICF_P1(true, false, "k_phase1<mixed,gen>");
     else if (iso) ICF_P1(false, true, "k_phase1<fp64,iso>");
@@ -972,7 +972,7 @@ bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap 
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
         const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
-        const dim3 g(nch + nbf), b(K);   // neither is 0: at least one map has keys
+        const dim3 g(interleave2_grid(nbf, nch)), b(K);   // not empty: at least one map has keys
 #define ICF_VF(MX, IS, L, NAME)                                                                                   \
     LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, K, S>), g, b, nbf, mv->off, mv->codes, mv->chunks[c][k], nch, A, \
            mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts)
@@ -1003,7 +1003,7 @@ bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const Slot
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
         const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
         const int nbc = nCZs ? round8(std::min<long long>(blocks_for(nCZs, K), 64)) : 0;
-        const dim3 g(nbf + nbc + nch), bl(K);   // not empty: something has keys or CZs
+        const dim3 g(nbc + interleave2_grid(nbf, nch)), bl(K);   // not empty: something has keys or CZs
 #define ICF_VFR(MX, IS, L, NAME)                                                                                   \
     LAUNCH(ctx, NAME, (k_vals_force_rhs_fused<MX, IS, L, K, S>), g, bl, nbf, nbc, mv->off, mv->codes, mv->chunks[c][k], \
            nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, mr->node_row, mr->off, mr->codes, mr->cz_begin, \
