@@ -36,6 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of MI355X in wave64 instructions per second: 256 CUs x 4 SIMDs, one wave64
+# FMA (FP32 unpacked or FP64) per SIMD every 4 cycles at the 2.4 GHz peak engine clock
+SIMDS, CLOCK_HZ = 1024, 2.4e9
+VALU_ISSUE_PEAK = SIMDS * CLOCK_HZ / 4
 
 
 def log(*a):
@@ -264,6 +268,9 @@ def autotune(r, rounds, name):
     got = {k: dict(keys=state[k][0][0], stage=state[k][0][1],
                    best_us={f"{c[0]}/{c[1]}": round(1e3 * ms, 1) for c, ms in state[k][2].items()})
            for k in roles}
+    shapes = gf.shape_state()
+    got.update({f"shape,{k}": dict(value=v, best_us={str(c): round(1e3 * ms, 1) for c, ms in best.items()})
+                for k, (v, _, best) in shapes.items() if any(ms > 0 for ms in best.values())})
     log(f"autotune {name} ({n} iterations): {got}")
     return got
 
@@ -396,6 +403,44 @@ def profiled_traffic(name, shape=None, path=os.path.join(ROOT, "profiles", "asse
     return None
 
 
+def profiled_valu(name, shape=None, path=None):
+    """VALU wave-instructions per dispatch of a kernel from the committed rocprofv3 summary
+    (SQ_INSTS_VALU of the wave-counter pass), or None."""
+    sym = _rocprof_symbol(name) + (f", {shape[0]}, {shape[1]}>" if shape else "")
+    try:
+        for line in open(path):
+            head = f"| `{sym}"
+            if line.startswith(head) and line[len(head)] in ",>`" and line.count("|") == 8:
+                return float(line.split("|")[6]) * 1e6
+    except (OSError, TypeError, ValueError):
+        return None
+    return None
+
+
+def valu_probe(r, workgroups=4096, iters=4096, reps=5):
+    """icf_probe_valu in FP64 and FP32: the achievable FMA rate (TFLOP/s) and the wave64
+    VALU instructions per second it issues, best of `reps` blocking launches (HIP events)."""
+    gf, lib, ctx = r.gf, r.lib, r.gf.ctx
+    buf = gf.buf("valu_probe", workgroups * 256, np.float64)
+    lib.icf_set_blocking(ctx, 1)
+    out = {}
+    for name, fp64 in (("fp64", 1), ("fp32", 0)):
+        best = None
+        for _ in range(reps):
+            r.check(lib.icf_probe_valu(ctx, fp64, workgroups, iters, buf.p), ctx, "icf_probe_valu")
+            ms = lib.icf_last_device_ms(ctx)
+            best = ms if best is None else min(best, ms)
+        instr = workgroups * 4 * iters * 16             # wave64 FMA instructions
+        out[name] = dict(tflops=round(2 * 64 * instr / (best * 1e-3) / 1e12, 2),
+                         wave_instr_per_s=round(instr / (best * 1e-3), 0),
+                         frac_of_issue_peak=round(instr / (best * 1e-3) / VALU_ISSUE_PEAK, 4), ms=round(best, 4))
+    buf.free()
+    gf._bufs.pop("valu_probe", None)
+    out["note"] = ("16 independent FMA chains per thread, 4096 x 256 threads; issue peak = 1024 SIMDs x 2.4 GHz / 4 "
+                   "cycles per wave64 instruction (78.6 TFLOP/s FP64 or unpacked FP32)")
+    return out
+
+
 def copy_bandwidth(r, mib=1024, reps=10):
     """A STREAM-like copy through icf_d2d (the library's 16-byte-per-thread copy kernel): GB/s
     of bytes read plus bytes written, best of `reps` blocking copies of `mib` MiB."""
@@ -524,6 +569,14 @@ def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
     dbytes = kernel_bytes(alg, dname) or 0
     dkeys = key_limit(r.gf, dname)
     traffic = profiled_traffic(dname, dkeys, traffic_md) if traffic_md else None
+    util = {}
+    for kname, (ms, n) in prof.items():
+        us = 1e3 * ms / max(n, 1)
+        vi = profiled_valu(kname, key_limit(r.gf, kname), traffic_md) if traffic_md else None
+        tb = profiled_traffic(kname, key_limit(r.gf, kname), traffic_md) if traffic_md else None
+        util[kname] = dict(avg_us=round(us, 2),
+                           traffic_frac=round(tb / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if tb else None,
+                           valu_frac=round(vi / (us * 1e-6) / VALU_ISSUE_PEAK, 4) if vi else None)
     out = {"workload": name, "tets": T, "czs": C, "active_czs": C - r.failed_czs(), "impacts": P,
            "candidate_pairs": r.Q, "nnzb": r.gf.csr.nnz, "rows": r.gf.csr.N,
            "value": round(units / sec / 1e6, 3), "ms_per_step": round(sec * 1e3, 4), "issue": issue,
@@ -532,6 +585,7 @@ def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
            "mixed_over_fp64": round(res["fp64"][1] / sec, 3),
            "iteration_roofline_frac": round(iteration_bytes(r) / sec / 1e9 / HBM_PEAK_GBS, 4),
            "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
+           "kernels_util": util,
            "roofline": {"kernel": dname, "alg_bytes": int(dbytes), "avg_us": round(avg_s * 1e6, 2),
                         "achieved": round(dbytes / avg_s / 1e9, 1),
                         "frac": round(dbytes / avg_s / 1e9 / HBM_PEAK_GBS, 4),
@@ -564,9 +618,22 @@ def main():
     ap.add_argument("--newton-damage-steps", type=int, default=26,
                     help="simulated time (in 1e-4 s steps) of the FP64-vs-mixed three-point-bend run into fracture "
                          "(2 mm deflection ramped over 30 steps, the reference's time-step control; 0: skip)")
-    ap.add_argument("--newton-contact-attempts", type=int, default=200,
-                    help="attempts of the FP64-vs-mixed Newton run with contact (a cube between rigid platens, "
-                         "the reference's time-step control; 0: skip)")
+    ap.add_argument("--newton-converged", type=int, default=1,
+                    help="also run the fracture study with every Newton loop converged (relative tolerance 1e-9, "
+                         "no absolute cutoff, at most 40 iterations): mixed against FP64 without the reference's "
+                         "early stop (0: skip)")
+    ap.add_argument("--newton-contact-attempts", type=int, default=0,
+                    help="attempts of an FP64-vs-mixed Newton run with contact of a cube between rigid platens "
+                         "(the reference's time-step control; 0: skip)")
+    ap.add_argument("--newton-self-contact-steps", type=int, default=30,
+                    help="simulated time (in 1e-4 s steps) of the FP64-vs-mixed three-point-bend run with "
+                         "self-contact (every exterior and CZ face in the contact surface; 2 mm ramped over 20 "
+                         "steps, the reference's time-step control; 0: skip)")
+    ap.add_argument("--newton-batched-specimens", type=int, default=32,
+                    help="specimens (the first of the headline's 256) of the FP64-vs-mixed batched Newton run: "
+                         "each cube pulled to 3e-4 strain over 10 steps, per-specimen solves on host threads "
+                         "(0: skip)")
+    ap.add_argument("--newton-batched-steps", type=int, default=4, help="fixed steps of the batched Newton run")
     ap.add_argument("--newton-steps", type=int, default=10,
                     help="steps of the FP64-vs-mixed three-point-bend Newton run (0: skip)")
     ap.add_argument("--no-overlap", action="store_true",
@@ -674,20 +741,27 @@ def main():
     it_bytes = iteration_bytes(r)
     # every kernel of the step against HBM peak: measured traffic (committed counter summary,
     # default workload only) and algorithmic bytes over its HIP-event duration
+    vp = valu_probe(r)
+    log(f"VALU probe: {vp}")
     util = {}
     for kname, (ms, n) in prof.items():
         us = 1e3 * ms / max(n, 1)
         tb = profiled_traffic(kname, key_limit(r.gf, kname), md) if profiled else None
+        vi = profiled_valu(kname, key_limit(r.gf, kname), md) if profiled else None
         ab = kernel_bytes(alg, kname)
         util[kname] = dict(avg_us=round(us, 2),
                            traffic_frac=round(tb / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if tb else None,
-                           alg_frac=round(ab / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if ab else None)
+                           alg_frac=round(ab / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if ab else None,
+                           valu_frac=round(vi / (us * 1e-6) / VALU_ISSUE_PEAK, 4) if vi else None,
+                           valu_frac_of_probe=(round(vi / (us * 1e-6) / vp["fp64"]["wave_instr_per_s"], 4)
+                                               if vi else None))
 
     # device broad phase over the same surface tets (reported beside the step; the reference's
     # broad phase is a host BVH outside the assembly path)
     bw = copy_bandwidth(r)
     log(f"copy probe: {bw}")
     roof["frac_of_copy_probe"] = round(achieved / bw["GBs"], 4)
+    roof["valu_frac"] = util.get(dname, {}).get("valu_frac")
     bp = broad_phase_timing(r) if contact else None
     if bp:
         log(f"device broad phase: {bp}")
@@ -698,20 +772,29 @@ def main():
         study = newton.precision_study(consts, steps=args.newton_steps, log=log)
         study["wall_s"] = round(time.time() - t0, 1)
         log(f"precision study: {study}")
+    last = [time.time()]
+
+    def every_15s(msg):                     # progress of a long run, a line at a time
+        if time.time() - last[0] > 15:
+            last[0] = time.time()
+            log(msg)
+
     study_damage = None
     if args.newton_damage_steps > 0:
         from icflow_amd import newton
         t0 = time.time()
-        last = [t0]
-
-        def every_15s(msg):                 # progress of a two-minute run, a line at a time
-            if time.time() - last[0] > 15:
-                last[0] = time.time()
-                log(msg)
         study_damage = newton.precision_study(consts, steps=args.newton_damage_steps, ramp_steps=30, adapt=True,
                                               log=every_15s)
         study_damage["wall_s"] = round(time.time() - t0, 1)
         log(f"precision study into fracture: {study_damage}")
+    study_conv = None
+    if args.newton_damage_steps > 0 and args.newton_converged:
+        from icflow_amd import newton
+        t0 = time.time()
+        study_conv = newton.precision_study(consts, steps=args.newton_damage_steps, ramp_steps=30, adapt=True,
+                                            conv_eps=1e-9, conv_cutoff=1e-30, max_iter=40, log=every_15s)
+        study_conv["wall_s"] = round(time.time() - t0, 1)
+        log(f"precision study into fracture, Newton converged: {study_conv}")
     study_contact = None
     if args.newton_contact_attempts > 0:
         from icflow_amd import newton
@@ -720,6 +803,22 @@ def main():
                                                        max_attempts=args.newton_contact_attempts)
         study_contact["wall_s"] = round(time.time() - t0, 1)
         log(f"precision study with contact: {study_contact}")
+    study_self = None
+    if args.newton_self_contact_steps > 0:
+        from icflow_amd import newton
+        t0 = time.time()
+        study_self = newton.precision_study(consts, steps=args.newton_self_contact_steps, ramp_steps=20, adapt=True,
+                                            contact=True, log=every_15s)
+        study_self["wall_s"] = round(time.time() - t0, 1)
+        log(f"precision study with self-contact: {study_self}")
+    study_batched = None
+    if args.newton_batched_specimens > 0:
+        from icflow_amd import newton
+        t0 = time.time()
+        study_batched = newton.batched_precision_study(consts, n_specimens=args.newton_batched_specimens,
+                                                       steps=args.newton_batched_steps, log=log)
+        study_batched["wall_s"] = round(time.time() - t0, 1)
+        log(f"batched precision study: {study_batched}")
     cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds, units)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -759,13 +858,22 @@ def main():
         "mixed_rel_err_vs_fp64": errs,
         "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
         "kernels_hbm": util,
+        "kernels_util_note": ("per kernel: traffic_frac = copy-calibrated FETCH_SIZE + WRITE_SIZE bytes per dispatch "
+                              "(committed rocprofv3 summary) / live HIP-event duration / 8 TB/s; alg_frac = "
+                              "algorithmic bytes / duration / 8 TB/s; valu_frac = SQ_INSTS_VALU per dispatch (same "
+                              "summary) / duration / VALU issue peak; valu_frac_of_probe = the same against the "
+                              "FP64 FMA probe's measured issue rate"),
+        "valu_probe": vp,
         "autotune": tuned or None,
         "roofline": roof,
         "hbm_copy_probe": bw,
         "device_broad_phase": bp,
         "precision_study": study,
         "precision_study_fracture": study_damage,
+        "precision_study_fracture_converged": study_conv,
         "precision_study_contact": study_contact,
+        "precision_study_self_contact": study_self,
+        "precision_study_batched": study_batched,
         "cpu_baseline": cpu,
     }
     r.gf.close()

@@ -79,6 +79,9 @@ _SIGS = {
     "icf_pattern_changed": (c_int, [c_void_p]),
     "icf_last_device_ms": (c_double, [c_void_p]),
     "icf_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t]),
+    "icf_probe_valu": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "icf_set_shape": (c_int, [c_void_p, c_int, c_int]),
+    "icf_get_shape": (c_int, [c_void_p, c_int, P_int, P_int, ctypes.POINTER(ctypes.c_float)]),
     "icf_set_blocking": (c_int, [c_void_p, c_int]),
     "icf_cz_counts": (c_int, [c_void_p, P_int, P_int]),
     "icf_graph_begin": (c_int, [c_void_p]),
@@ -110,6 +113,7 @@ _SIGS = {
                                 ctypes.POINTER(ctypes.c_longlong)]),
     "SolveDouble3": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                              c_int]),
+    "icfh_solve_threads": (c_int, [c_int]),
     "icf_nodes_infer": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double]),
     "icf_nodes_add_dx": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "icf_nodes_accept": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double]),

@@ -27,16 +27,25 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build_library(verbose=False):
-    srcs = [os.path.join(CSRC, "icf_lib.hip"), os.path.join(CSRC, "icf_host.cpp"), os.path.join(CSRC, "icf_solve.cpp")]
-    tmp = OUT + ".tmp"
-    cmd = [_hipcc(), *HIP_FLAGS, "-o", tmp, *srcs]
+def _run(cmd, verbose, what):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libicflow_hip.so")
+        raise RuntimeError(f"{what} failed building libicflow_hip.so")
+
+
+def build_library(verbose=False):
+    # the host solve (OpenMP, GNU runtime: the one torch and the oracle load) as an object of
+    # its own, linked into the HIP library
+    solve_o = os.path.join(HERE, "icf_solve.o")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-fopenmp", "-c", "-o", solve_o, os.path.join(CSRC, "icf_solve.cpp")],
+         verbose, "g++")
+    srcs = [os.path.join(CSRC, "icf_lib.hip"), os.path.join(CSRC, "icf_host.cpp")]
+    tmp = OUT + ".tmp"
+    _run([_hipcc(), *HIP_FLAGS, "-o", tmp, *srcs, "-x", "none", solve_o, "-lgomp"], verbose, "hipcc")
+    os.remove(solve_o)
     os.replace(tmp, OUT)
     return OUT
 

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, 
 }
 
 // The k-DOPs and node ids in cell order: sorted position j holds tet sidx[j], so that the tets
-// of one cell, and of neighbouring cells, are adjacent in memory for k_bp_pairs.  Per position:
+// of one cell, and of neighbouring cells, are adjacent in memory for k_bp_pairs9.  Per position:
 // the k-DOP as (lo, hi) pairs per direction (skd, 24 doubles), the node ids (snode), and the
 // axis-aligned box in FP32 rounded outward, lo down and hi up (box): a prefilter that never
 // rejects a pair whose FP64 boxes overlap.
@@ -109,95 +109,81 @@ __global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ lo
     box[2 * (size_t)j + 1] = make_float4(__double2float_ru(h[0]), __double2float_ru(h[1]), __double2float_ru(h[2]), 0.f);
 }
 
-// Visit each overlapping pair once, from the partner with the lower sorted position (one
-// thread per sorted position, over the cell-ordered copies); count the pairs, and write
-// (min << 32 | max) keys of the original indices: at off[ja] (WRITE), or, counting, into the
-// thread's own slots out[ja * cap .. ja * cap + cap) when out is given (pairs beyond cap are
-// only counted; the caller then runs the WRITE pass).  The 27 neighbouring cells are 9 rows of
-// three: a row (x - 1 .. x + 1; x is the key's low field) is one key range, so it takes one
-// binary search, and only positions after the thread's own are visited.  A candidate passes
-// the outward-rounded FP32 box prefilter, then (when asked) the shared-node exclusion, which
-// removes most neighbours of a conforming mesh before the FP64 12-direction test (the
-// predicate of the reference's k-DOP overlap).
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_bp_pairs(const double2 *__restrict__ skd, int nT,
-                                                  const unsigned long long *__restrict__ skey,
-                                                  const int *__restrict__ sidx, const int4 *__restrict__ snode,
-                                                  const float4 *__restrict__ box, int exclude_shared,
-                                                  const double *__restrict__ red, long long *__restrict__ count,
-                                                  const long long *__restrict__ off,
-                                                  unsigned long long *__restrict__ out, int cap)
+// Visit each overlapping pair once, from the partner with the lower sorted position, over the
+// cell-ordered copies, with nine lanes per sorted position, one per neighbouring row of cells
+// (item i: position i / 9, row i % 9), so that a tet's rows are scanned side by side.  The 27
+// neighbouring cells are 9 rows of three: a row (x - 1 .. x + 1; x is the key's low field) is
+// one key range, so it takes one binary search, and only positions after the lane's own are
+// visited.  A candidate passes the outward-rounded FP32 box prefilter, then (when asked) the
+// shared-node exclusion, which removes most neighbours of a conforming mesh before the FP64
+// 12-direction test (the predicate of the reference's k-DOP overlap).  Keys are
+// (min << 32 | max) of the original indices.  Pairs are appended in any order (the caller sorts the keys): the
+// lanes of a wave that find a pair in the same iteration take consecutive places from one
+// atomicAdd on *total, and keys beyond cap are only counted (the caller then grows the
+// buffer and searches again).
+__global__ __launch_bounds__(256) void k_bp_pairs9(const double2 *__restrict__ skd, int nT,
+                                                   const unsigned long long *__restrict__ skey,
+                                                   const int *__restrict__ sidx, const int4 *__restrict__ snode,
+                                                   const float4 *__restrict__ box, int exclude_shared,
+                                                   const double *__restrict__ red,
+                                                   unsigned long long *__restrict__ total,
+                                                   unsigned long long *__restrict__ out, long long cap)
 {
-    const int ja = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ja >= nT) return;
-    const int a = sidx[ja];
+    const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= 9LL * nT) return;
+    const int ja = (int)(item / 9), r = (int)(item % 9);
+    const int lane = __lane_id();
     const double cs = bp_cell_size(red);
-    double la[BP_DIRS], ha[BP_DIRS];
-#pragma unroll
-    for (int d = 0; d < BP_DIRS; d++) {
-        const double2 v = skd[(size_t)ja * BP_DIRS + d];
-        la[d] = v.x;
-        ha[d] = v.y;
-    }
-    const float4 bl = box[2 * (size_t)ja], bh = box[2 * (size_t)ja + 1];
     int ca[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) ca[k] = (int)floor((la[k] - red[1 + k]) / cs);
+    for (int k = 0; k < 3; k++) ca[k] = (int)floor((skd[(size_t)ja * BP_DIRS + k].x - red[1 + k]) / cs);
+    const int cy = ca[1] + (r % 3) - 1, cz = ca[2] + (r / 3) - 1;
+    if (cy < 0 || cz < 0) return;
+    const unsigned long long row = ((unsigned long long)cy << BP_CELL_BITS) | ((unsigned long long)cz << (2 * BP_CELL_BITS));
+    const unsigned long long k0 = row | (unsigned long long)(ca[0] > 0 ? ca[0] - 1 : 0);
+    const unsigned long long k1 = row | (unsigned long long)(ca[0] + 1);
+    int lo_i = 0, hi_i = nT;
+    while (lo_i < hi_i) {
+        const int mid = (lo_i + hi_i) >> 1;
+        if (skey[mid] < k0) lo_i = mid + 1;
+        else hi_i = mid;
+    }
+    const float4 bl = box[2 * (size_t)ja], bh = box[2 * (size_t)ja + 1];
     const int4 na = snode[ja];
-    long long pos = WRITE ? off[ja] : 0;
-    int cnt = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            const int cy = ca[1] + dy, cz = ca[2] + dz;
-            if (cy < 0 || cz < 0) continue;
-            const unsigned long long row = ((unsigned long long)cy << BP_CELL_BITS) |
-                                           ((unsigned long long)cz << (2 * BP_CELL_BITS));
-            const unsigned long long k0 = row | (unsigned long long)(ca[0] > 0 ? ca[0] - 1 : 0);
-            const unsigned long long k1 = row | (unsigned long long)(ca[0] + 1);
-            int lo_i = 0, hi_i = nT;
-            while (lo_i < hi_i) {
-                const int mid = (lo_i + hi_i) >> 1;
-                if (skey[mid] < k0) lo_i = mid + 1;
-                else hi_i = mid;
-            }
-            for (int j = lo_i > ja ? lo_i : ja + 1; j < nT && skey[j] <= k1; j++) {
-                const float4 ol = box[2 * (size_t)j], oh = box[2 * (size_t)j + 1];
-                if (bl.x > oh.x || bl.y > oh.y || bl.z > oh.z || bh.x < ol.x || bh.y < ol.y || bh.z < ol.z) continue;
-                if (exclude_shared) {
-                    const int4 nb = snode[j];
-                    const int m[4] = {nb.x, nb.y, nb.z, nb.w};
-                    bool sh = false;
+    const unsigned a = (unsigned)sidx[ja];
+    for (int j = lo_i > ja ? lo_i : ja + 1; j < nT && skey[j] <= k1; j++) {
+        bool hit = true;
+        const float4 ol = box[2 * (size_t)j], oh = box[2 * (size_t)j + 1];
+        if (bl.x > oh.x || bl.y > oh.y || bl.z > oh.z || bh.x < ol.x || bh.y < ol.y || bh.z < ol.z) hit = false;
+        if (hit && exclude_shared) {
+            const int4 nb = snode[j];
+            const int m[4] = {nb.x, nb.y, nb.z, nb.w};
 #pragma unroll
-                    for (int j2 = 0; j2 < 4; j2++) sh = sh || m[j2] == na.x || m[j2] == na.y || m[j2] == na.z || m[j2] == na.w;
-                    if (sh) continue;
-                }
-                bool ov = true;
+            for (int j2 = 0; j2 < 4; j2++) hit = hit && m[j2] != na.x && m[j2] != na.y && m[j2] != na.z && m[j2] != na.w;
+        }
+        if (hit) {
 #pragma unroll
-                for (int d = 0; d < BP_DIRS; d++) {
-                    const double2 v = skd[(size_t)j * BP_DIRS + d];
-                    ov = ov && !(la[d] > v.y) && !(ha[d] < v.x);
-                }
-                if (!ov) continue;
-                if (WRITE || (out && cnt < cap)) {
-                    const unsigned b = (unsigned)sidx[j];
-                    const unsigned p = (unsigned)a < b ? (unsigned)a : b, q = (unsigned)a < b ? b : (unsigned)a;
-                    out[WRITE ? pos + cnt : (long long)ja * cap + cnt] = ((unsigned long long)p << 32) | q;
-                }
-                cnt++;
+            for (int d = 0; d < BP_DIRS; d++) {
+                const double2 u = skd[(size_t)ja * BP_DIRS + d], v = skd[(size_t)j * BP_DIRS + d];
+                hit = hit && !(u.x > v.y) && !(u.y < v.x);
             }
         }
-    if (!WRITE) count[ja] = cnt;
-}
-
-// The counting pass's slot pairs, packed at their offsets (threads whose pairs all fit).
-__global__ __launch_bounds__(256) void k_bp_pack(const unsigned long long *__restrict__ slots, int cap,
-                                                 const long long *__restrict__ count, const long long *__restrict__ off,
-                                                 int nT, unsigned long long *__restrict__ out)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ja = (int)(i / cap), k = (int)(i % cap);
-    if (ja >= nT || k >= count[ja]) return;
-    out[off[ja] + k] = slots[i];
+        const unsigned long long m = __ballot(hit);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(total, (unsigned long long)__popcll(m));
+            const unsigned lo32 = __shfl((unsigned)(base & 0xffffffffULL), leader, 64);
+            const unsigned hi32 = __shfl((unsigned)(base >> 32), leader, 64);
+            if (hit) {
+                const unsigned long long at = (((unsigned long long)hi32 << 32) | lo32) +
+                                              (unsigned long long)__popcll(m & ((1ULL << lane) - 1));
+                const unsigned b = (unsigned)sidx[j];
+                const unsigned p = a < b ? a : b, q = a < b ? b : a;
+                if (at < (unsigned long long)cap) out[at] = ((unsigned long long)p << 32) | q;
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bp_unpack(const unsigned long long *__restrict__ keys, long long n,

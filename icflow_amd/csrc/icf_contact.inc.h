@@ -146,14 +146,14 @@ __global__ __launch_bounds__(256) void k_adj_fill(const int *__restrict__ ie, in
     for (int k = 0; k < n; k++) ids[o + k] = ie[t + (size_t)es * (5 + k)];
 }
 
-// (node, tet) tuples (narrowphase.h:159-201), CF_GROUP lanes per tuple: lane l evaluates the
+// (node, tet) tuples (narrowphase.h:159-201), CF_GROUP lanes per tuple (4, 8 or 16: the
+// autotuned shape; at least 3, the lanes that write the face's ids): lane l evaluates the
 // adjacent faces k = l, l + CF_GROUP, ...; the group then takes the lexicographic minimum of
 // (distance, k).  That is the reference's sequential "first strict minimum" rule: a NaN
 // distance at k > 0 never compares smaller (it is ranked +inf), and a NaN at k = 0 keeps
 // face 0 (ranked -inf).  A tet without adjacent faces (not produced by the reference's
 // surface detection) yields -1 ids instead of reading faces[-1].
-constexpr int CF_GROUP = 8;
-
+template <int CF_GROUP>
 __global__ __launch_bounds__(256) void k_closest_face(int nPairs, int ts, int *__restrict__ nl2,
                                                       const double *__restrict__ dn,
                                                       const int *__restrict__ adj_off,

@@ -177,6 +177,15 @@ hipError_t scratch_for_device(int device, void **out)
 
 }  // namespace
 
+// Launch-shape roles besides the value reductions (icf_set_shape / icf_get_shape), three
+// candidates each: the workgroup size (64, 128 or 256 lanes) of phase 1 of the fused call, of
+// the narrow phase and of the impacts' phase 1, and the lanes per (node, tet) tuple of the
+// closest face (4, 8 or 16).  Every candidate gives the same results (each unit's work does
+// not depend on the shape).
+constexpr int NSHAPE = 4, NSHAPE_CAND = 3;
+constexpr int shape_value(int role, int c) { return role == ICF_SHAPE_CLOSEST ? 4 << c : 64 << c; }
+constexpr int SHAPE_DEFAULT[NSHAPE] = {2, 2, 1, 2};   // 256, 256, 8 lanes, 256
+
 struct icf_ctx {
     int id = 0;
     int device = 0;
@@ -202,6 +211,7 @@ struct icf_ctx {
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
     DevBuf pat_a, pat_b;           // pattern build / broad phase / NL2set: 64-bit keys
     DevBuf bp_f, bp_k, bp_i, bp_l;   // broad phase and NL2set scratch (doubles, keys, ints, int64)
+    long long bp_cap = 0;            // pair keys the broad phase's buffer holds (grows with use)
     DevBuf tsum;                     // per-node tet sums of the fused call
     DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
     int tplane = 64;                 // plane stride of the staged tet records (tet_plane)
@@ -223,17 +233,22 @@ struct icf_ctx {
     bool blocking = true;          // icf_set_blocking
     bool capturing = false;        // between icf_graph_begin and icf_graph_end
     bool overlap = true;           // icf_set_overlap: independent kernels on more than one stream
-    // chunk shape (workgroup size, staging) of the BCSR-value reductions, per role
-    // [accumulate map][FP64 staging]: the candidate in use and, while tuning, the timings
-    struct KeyTune {
-        int sel = CAND_DEFAULT_FP32;   // candidate (cand_keys, cand_stage); FP64 roles: see icf_create
+    // autotuned launch shapes.  tune: the chunk shape (workgroup size, staging) of the BCSR-value
+    // reductions, per role [accumulate map][FP64 staging]; shape: the workgroup size of phase 1,
+    // the narrow phase and the impacts' phase 1, and the lanes per tuple of the closest face.
+    // Per role: the candidate in use and, while tuning, the timings
+    struct Tune {
+        int ncand = NCAND;
+        int sel = CAND_DEFAULT_FP32;   // candidate in use
         bool active = false;       // timing the candidates in turn
+        bool fixed = false;        // chosen by the caller (icf_set_chunk_keys / icf_set_shape)
         int turn = 0;              // launches started while tuning
         int samples = 0;           // launches timed
         int cand = -1;             // candidate of the launch being timed, -1: none
+        int bucket = -1;           // size class of the launches timed (the shape cache's key)
         float best[NCAND] = {};    // fastest launch per candidate (ms), 0: not timed
         hipEvent_t ev[2] = {nullptr, nullptr};
-    } tune[2][2];
+    } tune[2][2], shape[NSHAPE];
     int tune_rounds = 0;           // timed launches per candidate (icf_set_autotune)
     // icf_assemble_batched: the last offsets (host copy; device copy followed by 2 counts per
     // specimen) and the value and rhs maps whose units were last checked against them
@@ -872,6 +887,92 @@ int stage_czs(icf_ctx *ctx, double *dcz, int *icz, const double *dn, int n, int 
     return 0;
 }
 
+// ---- autotuner of the launch shapes -----------------------------------------------------------
+// While a role is being tuned, its launches cycle through the role's candidates, each timed
+// with a pair of events around the launch; the next launch of the role reads the previous
+// timing (waiting for it if it has not finished), and after tune_rounds timings per candidate
+// the fastest launch decides.  Captured calls are not timed; they use the current choice.
+// Roles: the value reductions' chunk shape per (accumulate map, FP64 staging) (eight
+// candidates), and the shape roles (three candidates each).  A finished tuning is also kept in
+// a cache held by the library, per device, role, precision and size class (the launch's units,
+// to the next power of two): contexts whose role is neither being tuned nor set by the caller
+// take the cached choice for launches of that size class.
+
+struct ShapeCache {
+    std::mutex mu;
+    std::map<std::array<int, 4>, int> choice;   // (device, role id, precision, size class) -> candidate
+};
+ShapeCache g_shape_cache;
+
+int size_class(long long n)
+{
+    int b = 0;
+    while (b < 62 && (1LL << b) < n) b++;
+    return b;
+}
+
+void tune_fold(icf_ctx *ctx, icf_ctx::Tune &t, int role_id)
+{
+    if (t.cand >= 0) {
+        float ms = 0;
+        if (hipEventSynchronize(t.ev[1]) == hipSuccess && hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) == hipSuccess &&
+            ms > 0) {
+            if (t.best[t.cand] == 0 || ms < t.best[t.cand]) t.best[t.cand] = ms;
+            t.samples++;
+        }
+        t.cand = -1;
+    }
+    if (t.active && (t.samples >= ctx->tune_rounds * t.ncand || t.turn >= 4 * ctx->tune_rounds * t.ncand)) {
+        t.active = false;
+        for (int c = 0; c < t.ncand; c++)
+            if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
+        if (t.bucket >= 0) {
+            std::lock_guard<std::mutex> lk(g_shape_cache.mu);
+            g_shape_cache.choice[{ctx->device, role_id, ctx->precision, t.bucket}] = t.sel;
+        }
+    }
+}
+
+// The candidate for the next launch of a role with `units` units; starts its timing if tuning
+// (not when `peek`).
+int tune_begin(icf_ctx *ctx, icf_ctx::Tune &t, int role_id, long long units, bool peek = false)
+{
+    const int bucket = size_class(units);
+    if (!t.active || ctx->capturing || peek) {
+        if (!t.active && !t.fixed) {
+            std::lock_guard<std::mutex> lk(g_shape_cache.mu);
+            auto it = g_shape_cache.choice.find({ctx->device, role_id, ctx->precision, bucket});
+            if (it != g_shape_cache.choice.end()) return it->second;
+        }
+        return t.sel;
+    }
+    tune_fold(ctx, t, role_id);
+    if (!t.active) return t.sel;
+    t.bucket = bucket;
+    t.cand = t.turn++ % t.ncand;
+    (void)hipEventRecord(t.ev[0], ctx->cur);
+    return t.cand;
+}
+
+void tune_end(icf_ctx *ctx, icf_ctx::Tune &t)
+{
+    if (t.cand >= 0 && !ctx->capturing) (void)hipEventRecord(t.ev[1], ctx->cur);
+}
+
+// value-reduction roles: ids 0-3
+int tune_begin(icf_ctx *ctx, bool acc, bool fp64, long long keys)
+{
+    return tune_begin(ctx, ctx->tune[acc][fp64], 2 * acc + fp64, keys);
+}
+void tune_end(icf_ctx *ctx, bool acc, bool fp64) { tune_end(ctx, ctx->tune[acc][fp64]); }
+
+// shape roles: ids 4-7; returns the role's value (workgroup size or lanes per tuple)
+int shape_begin(icf_ctx *ctx, int role, long long units, bool peek = false)
+{
+    return shape_value(role, tune_begin(ctx, ctx->shape[role], 4 + role, units, peek));
+}
+void shape_end(icf_ctx *ctx, int role) { tune_end(ctx, ctx->shape[role]); }
+
 // ---- fused launches (icf_fused.inc.h) ----------------------------------------------------------
 
 int round8(long long n) { return (int)((n + 7) / 8 * 8); }
@@ -881,9 +982,10 @@ void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es,
                   int cs, int *counts)
 {
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
-    const int nbt = nt ? round8(blocks_for(nt, 256)) : 0;
-    const int nbc = nc ? (int)blocks_for((long long)(nc + CZ_PER_WAVE - 1) / CZ_PER_WAVE * 64, 256) : 0;
-    const dim3 g(std::max(interleave2_grid(nbt, nbc), 1)), b(256);
+    const int B = shape_begin(ctx, ICF_SHAPE_PHASE1, (long long)nt + nc);
+    const int nbt = nt ? round8(blocks_for(nt, B)) : 0;
+    const int nbc = nc ? (int)blocks_for((long long)(nc + CZ_PER_WAVE - 1) / CZ_PER_WAVE * 64, B) : 0;
+    const dim3 g(std::max(interleave2_grid(nbt, nbc), 1)), b(B);
     double *tr = (double *)ctx->tet_rhs.p, *cr = (double *)ctx->cz_rhs.p;
     void *tt = ctx->tet_tan.p, *ct = ctx->cz_tan.p;
     const int tp = ctx->tplane = tet_plane(nt);
@@ -895,48 +997,7 @@ void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es,
     else if (iso) ICF_P1(false, true, "k_phase1<fp64,iso>");
     else ICF_P1(false, false, "k_phase1<fp64,gen>");
 #undef ICF_P1
-}
-
-// ---- autotuner of the value reductions' key limit --------------------------------------------
-// While a role is being tuned, its launches cycle through the candidate key limits, each timed
-// with a pair of events around the launch; the next launch of the role reads the previous
-// timing (waiting for it if it has not finished), and after tune_rounds timings per candidate
-// the fastest launch decides.  Captured calls are not timed; they use the current choice.
-
-void tune_fold(icf_ctx *ctx, icf_ctx::KeyTune &t)
-{
-    if (t.cand >= 0) {
-        float ms = 0;
-        if (hipEventSynchronize(t.ev[1]) == hipSuccess && hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) == hipSuccess &&
-            ms > 0) {
-            if (t.best[t.cand] == 0 || ms < t.best[t.cand]) t.best[t.cand] = ms;
-            t.samples++;
-        }
-        t.cand = -1;
-    }
-    if (t.active && (t.samples >= ctx->tune_rounds * NCAND || t.turn >= 4 * ctx->tune_rounds * NCAND)) {
-        t.active = false;
-        for (int c = 0; c < NCAND; c++)
-            if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
-    }
-}
-
-// The candidate for the next value reduction of role (acc, fp64); starts its timing if tuning.
-int tune_begin(icf_ctx *ctx, bool acc, bool fp64)
-{
-    icf_ctx::KeyTune &t = ctx->tune[acc][fp64];
-    if (!t.active || ctx->capturing) return t.sel;
-    tune_fold(ctx, t);
-    if (!t.active) return t.sel;
-    t.cand = t.turn++ % NCAND;
-    (void)hipEventRecord(t.ev[0], ctx->cur);
-    return t.cand;
-}
-
-void tune_end(icf_ctx *ctx, bool acc, bool fp64)
-{
-    icf_ctx::KeyTune &t = ctx->tune[acc][fp64];
-    if (t.cand >= 0 && !ctx->capturing) (void)hipEventRecord(t.ev[1], ctx->cur);
+    shape_end(ctx, ICF_SHAPE_PHASE1);
 }
 
 // Calls f(integral_constant<cand_keys(c)>, integral_constant<cand_stage(c)>).
@@ -967,7 +1028,7 @@ bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap 
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const int k = mixed ? 0 : 1;
     double *ts = (double *)ctx->tsum.p;
-    const int c = tune_begin(ctx, false, !mixed);
+    const int c = tune_begin(ctx, false, !mixed, mv->nlaunch);
     with_cand(c, [&](auto kc, auto sc) {
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
@@ -997,7 +1058,7 @@ bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const Slot
     if (mv->nlaunch == 0 && mf->nlaunch == 0 && nCZs == 0) return true;
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const int k = mixed ? 0 : 1;
-    const int c = tune_begin(ctx, false, !mixed);
+    const int c = tune_begin(ctx, false, !mixed, mv->nlaunch);
     with_cand(c, [&](auto kc, auto sc) {
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
@@ -1027,7 +1088,7 @@ bool contact_sums_fused(icf_ctx *ctx, const SlotMap *mf, double *F, int ns, cons
     if (mf->nlaunch == 0 && mv->nlaunch == 0 && mr->nlaunch == 0) return true;
     const bool mixed = ctx->precision == ICF_MIXED;
     const int k = mixed ? 0 : 1;
-    const int c = tune_begin(ctx, true, !mixed);
+    const int c = tune_begin(ctx, true, !mixed, mv->nlaunch);
     with_cand(c, [&](auto kc, auto sc) {
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
@@ -1066,7 +1127,7 @@ void vals_kernel(icf_ctx *ctx, const char *name, const SlotMap *m, double *A, co
 {
     const int k = sizeof(L) == 4 ? 0 : 1;
     if (m->nlaunch == 0) return;   // then no chunks for any key limit
-    const int c = tune_begin(ctx, ACC, k == 1);
+    const int c = tune_begin(ctx, ACC, k == 1, m->nlaunch);
     with_cand(c, [&](auto kc, auto sc) {
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int n = m->nchunks[c][k];
@@ -1139,6 +1200,10 @@ int icf_create(int device, icf_ctx **out)
     if (device >= ndev) return ICF_E_INVALID;
     icf_ctx *ctx = new icf_ctx();
     for (auto &row : ctx->tune) row[1].sel = CAND_DEFAULT_FP64;   // FP64 staging roles
+    for (int r = 0; r < NSHAPE; r++) {
+        ctx->shape[r].ncand = NSHAPE_CAND;
+        ctx->shape[r].sel = SHAPE_DEFAULT[r];
+    }
     ctx->id = g_next_ctx_id++;
     ctx->device = device;
     memset(&ctx->hc, 0, sizeof(ctx->hc));
@@ -1188,6 +1253,9 @@ int icf_destroy(icf_ctx *ctx)
         for (auto &t : row)
             for (hipEvent_t e : t.ev)
                 if (e) (void)hipEventDestroy(e);
+    for (auto &t : ctx->shape)
+        for (hipEvent_t e : t.ev)
+            if (e) (void)hipEventDestroy(e);
     for (int l = 0; l < 2; l++)
         for (int k = 0; k < LANE_EVENTS; k++)
             if (ctx->lane_ev[l][k]) (void)hipEventDestroy(ctx->lane_ev[l][k]);
@@ -1396,6 +1464,26 @@ int icf_d2d(icf_ctx *ctx, void *dst, size_t dst_off, const void *src, size_t src
     return tm.finish();
 }
 
+int icf_probe_valu(icf_ctx *ctx, int fp64, int workgroups, int iters, void *out)
+{
+    if (!ctx || !out || workgroups <= 0 || iters <= 0) return ICF_E_INVALID;
+    if (int r = check_ctx(ctx)) return r;
+    const size_t bytes = (size_t)workgroups * 256 * (fp64 ? 8 : 4);
+    if (int r = note_write(ctx, out, bytes, "icf_probe_valu")) return r;
+    OnLane ln(ctx, 0, [&] { return Access().wr((const char *)out, bytes); });
+    if (ln.err) return ln.err;
+    Timer tm(ctx);
+    // m slightly below one and c small keep every chain finite for any iteration count
+    if (fp64)
+        LAUNCH(ctx, "k_valu_probe<double>", k_valu_probe<double>, dim3(workgroups), dim3(256), (double *)out, iters,
+               0.999999, 1e-6);
+    else
+        LAUNCH(ctx, "k_valu_probe<float>", k_valu_probe<float>, dim3(workgroups), dim3(256), (float *)out, iters,
+               0.9999f, 1e-4f);
+    if (int r = ln.done()) return r;
+    return tm.finish();
+}
+
 int icf_set_blocking(icf_ctx *ctx, int blocking)
 {
     if (!ctx) return ICF_E_INVALID;
@@ -1511,24 +1599,30 @@ int icf_set_autotune(icf_ctx *ctx, int rounds)
     if (!ctx || rounds < 0 || rounds > 1000) return ICF_E_INVALID;
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_set_autotune during capture");
     if (int r = check_ctx(ctx)) return r;
-    for (auto &row : ctx->tune)
-        for (icf_ctx::KeyTune &t : row) {
-            if (rounds == 0) {         // stop: fold what was timed, keep the fastest so far
-                if (t.active) {
-                    tune_fold(ctx, t);
-                    for (int c = 0; c < NCAND; c++)
-                        if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
-                }
-                t.active = false;
-                continue;
+    std::vector<std::pair<icf_ctx::Tune *, int>> all;
+    for (int a = 0; a < 2; a++)
+        for (int f = 0; f < 2; f++) all.push_back({&ctx->tune[a][f], 2 * a + f});
+    for (int r = 0; r < NSHAPE; r++) all.push_back({&ctx->shape[r], 4 + r});
+    for (auto &tr : all) {
+        icf_ctx::Tune &t = *tr.first;
+        if (rounds == 0) {             // stop: fold what was timed, keep the fastest so far
+            if (t.active) {
+                tune_fold(ctx, t, tr.second);
+                for (int c = 0; c < t.ncand; c++)
+                    if (t.best[c] > 0 && (t.best[t.sel] == 0 || t.best[c] < t.best[t.sel])) t.sel = c;
             }
-            for (hipEvent_t &e : t.ev)
-                if (!e) HIPCHK(ctx, hipEventCreate(&e));
-            t.active = true;
-            t.turn = t.samples = 0;
-            t.cand = -1;
-            for (float &b : t.best) b = 0;
+            t.active = false;
+            continue;
         }
+        for (hipEvent_t &e : t.ev)
+            if (!e) HIPCHK(ctx, hipEventCreate(&e));
+        t.active = true;
+        t.fixed = false;
+        t.turn = t.samples = 0;
+        t.cand = -1;
+        t.bucket = -1;
+        for (float &b : t.best) b = 0;
+    }
     ctx->tune_rounds = rounds;
     return 0;
 }
@@ -1537,8 +1631,8 @@ int icf_get_autotune(icf_ctx *ctx, int *keys, int *stage, int *tuning, float *be
 {
     if (!ctx) return ICF_E_INVALID;
     for (int r = 0; r < 4; r++) {
-        icf_ctx::KeyTune &t = ctx->tune[r >> 1][r & 1];
-        if (t.active && !ctx->capturing) tune_fold(ctx, t);   // a finished last timing counts
+        icf_ctx::Tune &t = ctx->tune[r >> 1][r & 1];
+        if (t.active && !ctx->capturing) tune_fold(ctx, t, r);   // a finished last timing counts
         if (keys) keys[r] = cand_keys(t.sel);
         if (stage) stage[r] = cand_stage(t.sel);
         if (tuning) tuning[r] = t.active ? 1 : 0;
@@ -1551,16 +1645,48 @@ int icf_get_autotune(icf_ctx *ctx, int *keys, int *stage, int *tuning, float *be
 int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys, int stage)
 {
     if (!ctx) return ICF_E_INVALID;
-    icf_ctx::KeyTune &t = ctx->tune[accumulate != 0][fp64 != 0];
+    icf_ctx::Tune &t = ctx->tune[accumulate != 0][fp64 != 0];
     if (stage == 0) stage = cand_stage(t.sel);
     int c = 0;
     while (c < NCAND && (cand_keys(c) != keys || cand_stage(c) != stage)) c++;
     if (c == NCAND)
         return fail(ctx, ICF_E_INVALID, "icf_set_chunk_keys: keys must be 64, 128, 256 or 512 and stage 2 or 3");
-    if (t.active && !ctx->capturing) tune_fold(ctx, t);
+    if (t.active && !ctx->capturing) tune_fold(ctx, t, 2 * (accumulate != 0) + (fp64 != 0));
     t.active = false;
+    t.fixed = true;
     t.cand = -1;
     t.sel = c;
+    return 0;
+}
+
+int icf_set_shape(icf_ctx *ctx, int role, int value)
+{
+    if (!ctx) return ICF_E_INVALID;
+    if (role < 0 || role >= NSHAPE) return fail(ctx, ICF_E_INVALID, "icf_set_shape: unknown role");
+    int c = 0;
+    while (c < NSHAPE_CAND && shape_value(role, c) != value) c++;
+    if (c == NSHAPE_CAND)
+        return fail(ctx, ICF_E_INVALID, role == ICF_SHAPE_CLOSEST ? "icf_set_shape: lanes per tuple must be 4, 8 or 16"
+                                                                   : "icf_set_shape: workgroup size must be 64, 128 or 256");
+    icf_ctx::Tune &t = ctx->shape[role];
+    if (t.active && !ctx->capturing) tune_fold(ctx, t, 4 + role);
+    t.active = false;
+    t.fixed = true;
+    t.cand = -1;
+    t.sel = c;
+    return 0;
+}
+
+int icf_get_shape(icf_ctx *ctx, int role, int *value, int *tuning, float *best_ms)
+{
+    if (!ctx) return ICF_E_INVALID;
+    if (role < 0 || role >= NSHAPE) return fail(ctx, ICF_E_INVALID, "icf_get_shape: unknown role");
+    icf_ctx::Tune &t = ctx->shape[role];
+    if (t.active && !ctx->capturing) tune_fold(ctx, t, 4 + role);
+    if (value) *value = shape_value(role, t.sel);
+    if (tuning) *tuning = t.active ? 1 : 0;
+    if (best_ms)
+        for (int c = 0; c < NSHAPE_CAND; c++) best_ms[c] = t.best[c];
     return 0;
 }
 
@@ -1657,8 +1783,10 @@ int icf_narrow_phase(icf_ctx *ctx, int nPairs, const double *dn, const int *ie, 
     });
     if (ln.err) return ln.err;
     Timer tm(ctx);
-    LAUNCH(ctx, "k_narrow_phase", k_narrow_phase, dim3(blocks_for(nPairs, 256)), dim3(256), nPairs, dn, ie,
+    const int B = shape_begin(ctx, ICF_SHAPE_NARROW, nPairs);
+    LAUNCH(ctx, "k_narrow_phase", k_narrow_phase, dim3(blocks_for(nPairs, B)), dim3(B), nPairs, dn, ie,
            narrowList, el_all_stride, nd_stride);
+    shape_end(ctx, ICF_SHAPE_NARROW);
     if (int r = ln.done()) return r;
     return tm.finish();
 }
@@ -1701,6 +1829,24 @@ int get_adjacency(icf_ctx *ctx, const int *ie, int es)
     return 0;
 }
 
+// k_closest_face with the tuned lanes per tuple, on `stream` (the profiled launch when it is
+// the context's current stream)
+void launch_closest_face(icf_ctx *ctx, int nPairs, int ts, int *nl2, const double *dn, const int *adj_off,
+                         const int *adj_ids, const int *faces, int ns, int fs, hipStream_t stream)
+{
+    const bool timed = stream == ctx->cur;   // tuning times launches on the current stream only
+    const int G = shape_begin(ctx, ICF_SHAPE_CLOSEST, nPairs, !timed);
+    const dim3 g(blocks_for((long long)nPairs * G, 256)), b(256);
+    auto go = [&](auto kern) {
+        if (stream == ctx->cur) LAUNCH(ctx, "k_closest_face", kern, g, b, nPairs, ts, nl2, dn, adj_off, adj_ids, faces, ns, fs);
+        else hipLaunchKernelGGL(kern, g, b, 0, stream, nPairs, ts, nl2, dn, adj_off, adj_ids, faces, ns, fs);
+    };
+    if (G == 4) go(k_closest_face<4>);
+    else if (G == 16) go(k_closest_face<16>);
+    else go(k_closest_face<8>);
+    if (timed) shape_end(ctx, ICF_SHAPE_CLOSEST);
+}
+
 int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, const double *dn, const int *ie,
                           const int *faces, int nd_stride, int el_all_stride, int fc_stride)
 {
@@ -1723,9 +1869,8 @@ int icf_find_closest_face(icf_ctx *ctx, int nPairs, int tet_stride, int *nl2, co
     });
     if (ln.err) return ln.err;
     Timer tm(ctx);
-    LAUNCH(ctx, "k_closest_face", k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256),
-           nPairs, tet_stride, nl2, dn, (const int *)ctx->adj.off.p, (const int *)ctx->adj.ids.p, faces, nd_stride,
-           fc_stride);
+    launch_closest_face(ctx, nPairs, tet_stride, nl2, dn, (const int *)ctx->adj.off.p, (const int *)ctx->adj.ids.p,
+                        faces, nd_stride, fc_stride, ctx->cur);
     if (int r = ln.done()) return r;
     return tm.finish();
 }
@@ -1773,9 +1918,8 @@ int icf_find_closest_face_noalloc(icf_ctx *ctx, int nPairs, int tet_stride, int 
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, off, off, es + 1, ctx->stream));
     hipLaunchKernelGGL(k_adj_fill, dim3(blocks_for(es, 256)), dim3(256), 0, ctx->stream, ie, es, (const int *)off,
                        ids);
-    hipLaunchKernelGGL(k_closest_face, dim3(blocks_for((long long)nPairs * CF_GROUP, 256)), dim3(256), 0,
-                       ctx->stream, nPairs, tet_stride, nl2, dn, (const int *)off, (const int *)ids, faces, nd_stride,
-                       fc_stride);
+    launch_closest_face(ctx, nPairs, tet_stride, nl2, dn, (const int *)off, (const int *)ids, faces, nd_stride,
+                        fc_stride, ctx->stream);
     if (int r = lane_exit(ctx, 0, everything)) return r;
     return tm.finish();
 }
@@ -1810,13 +1954,15 @@ int icf_collision_response(icf_ctx *ctx, const int *icr, double *dn, double h, d
             .wr((const char *)ctx->ct_f.p, f_bytes);
     });
     if (l1.err) return l1.err;
-    const dim3 cg(blocks_for(nImpacts, 256)), cb(256);
+    const int B = shape_begin(ctx, ICF_SHAPE_IMPACT, nImpacts);
+    const dim3 cg(blocks_for(nImpacts, B)), cb(B);
     if (mixed)
         LAUNCH(ctx, "k_contact_impact<mixed>", k_contact_impact<float>, cg, cb, icr, dn, nImpacts, cr_stride, nd_stride,
                k, distanceEpsilon, (float *)ctx->ct.p, (double *)ctx->ct_f.p);
     else
         LAUNCH(ctx, "k_contact_impact<fp64>", k_contact_impact<double>, cg, cb, icr, dn, nImpacts, cr_stride,
                nd_stride, k, distanceEpsilon, (double *)ctx->ct.p, (double *)ctx->ct_f.p);
+    shape_end(ctx, ICF_SHAPE_IMPACT);
     if (int r = l1.done()) return r;
     // sums into node forces, BCSR blocks (at slots the call does not bound) and rhs rows
     OnLane l0(ctx, 0, [&] {
@@ -2242,7 +2388,7 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)60 * nT + 4))) return r;
     if (int r = grow(ctx, ctx->bp_k, sizeof(unsigned long long) * 2 * (size_t)nT)) return r;
     if (int r = grow(ctx, ctx->bp_i, sizeof(int) * (6 * (size_t)nT + 4))) return r;
-    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * (2 * ((size_t)nT + 1) + 1))) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2)) return r;
     double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT, *ext = hi + (size_t)12 * nT,
            *mn = ext + nT, *red = mn + (size_t)3 * nT;
     double2 *skd = (double2 *)(red + 4);                  // 12 double2 = 24 doubles per tet
@@ -2250,20 +2396,16 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     auto *key = (unsigned long long *)ctx->bp_k.p, *skey = key + nT;
     int *idx = (int *)ctx->bp_i.p, *sidx = idx + nT;
     int4 *snode = (int4 *)(sidx + nT + ((4 - ((2 * nT) & 3)) & 3));   // 16-byte aligned, 4 ints per tet
-    long long *cnt = (long long *)ctx->bp_l.p, *off = cnt + nT + 1;
     int *d_err = ctx->small + 1;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
     hipLaunchKernelGGL(k_bp_kdop, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, dn, nd_stride, ie,
                        el_all_stride, nT, lo, hi, ext, mn, d_err, nd_stride);
-    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    size_t b1 = 0, b2 = 0, b3 = 0;
     HIPCHK(ctx, hipcub::DeviceReduce::Max(nullptr, b1, ext, red, nT, ctx->stream));
     HIPCHK(ctx, hipcub::DeviceReduce::Min(nullptr, b2, mn, red + 1, nT, ctx->stream));
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b3, key, skey, idx, sidx, nT, 0, 3 * BP_CELL_BITS,
                                                    ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, cnt, off, nT + 1, ctx->stream));
-    size_t b6 = 0;
-    HIPCHK(ctx, hipcub::DeviceReduce::Max(nullptr, b6, cnt, off + nT + 1, nT, ctx->stream));
-    size_t tb = std::max(std::max(std::max(b1, b2), std::max(b3, b4)), b6);
+    size_t tb = std::max(std::max(b1, b2), b3);
     if (int r = grow(ctx, ctx->scan_tmp, tb)) return r;
     HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b1, ext, red, nT, ctx->stream));
     for (int k = 0; k < 3; k++)
@@ -2271,43 +2413,36 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     hipLaunchKernelGGL(k_bp_cell, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, nT, red, key, idx, d_err);
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, b3, key, skey, idx, sidx, nT, 0,
                                                    3 * BP_CELL_BITS, ctx->stream));
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)(cnt + nT), 2, 0);
     hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, sidx, ie,
                        el_all_stride, skd, snode, box);
-    // counting pass; each thread also keeps its first BP_SLOTS pairs, so that one pass suffices
-    // when no thread has more
-    constexpr int BP_SLOTS = 64;
-    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * BP_SLOTS * (size_t)nT)) return r;
-    auto *slots = (unsigned long long *)ctx->pat_a.p;
-    hipLaunchKernelGGL((k_bp_pairs<false>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skd, nT, skey, sidx,
-                       snode, box, exclude_shared, red, cnt, nullptr, slots, BP_SLOTS);
-    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b4, cnt, off, nT + 1, ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b6, cnt, off + nT + 1, nT, ctx->stream));
-    long long total = 0, most = 0;
-    int err = 0;
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(&total, off + nT, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&most, off + nT + 1, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
+    // pairs appended by nine lanes per tet; a second search when they outgrew the buffer
+    auto *tot = (unsigned long long *)ctx->bp_l.p;
+    long long cap = std::max<long long>(ctx->bp_cap, 4LL * nT);
+    long long total = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)cap)) return r;
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)tot, 2, 0);
+        hipLaunchKernelGGL(k_bp_pairs9, dim3(blocks_for(9LL * nT, 256)), dim3(256), 0, ctx->stream, skd, nT, skey,
+                           sidx, snode, box, exclude_shared, red, tot, (unsigned long long *)ctx->pat_a.p, cap);
+        unsigned long long got = 0;
+        int err = 0;
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(&got, tot, sizeof(got), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
+        total = (long long)got;
+        if (total <= cap) break;
+        if (pass == 1) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: pair count changed between two searches");
+        cap = total + total / 4;
+    }
+    ctx->bp_cap = std::max(ctx->bp_cap, total + total / 4);
     *nOut = total;
     if (total > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
     if (total == 0) return 0;
     if (total >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 pairs");
     if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)total)) return r;
     auto *pa = (unsigned long long *)ctx->pat_a.p, *pb = (unsigned long long *)ctx->pat_b.p;
-    if (most <= BP_SLOTS) {
-        // every pair is in a slot: pack them at their offsets (into pb), then sort into pa
-        hipLaunchKernelGGL(k_bp_pack, dim3(blocks_for((long long)nT * BP_SLOTS, 256)), dim3(256), 0, ctx->stream, pa,
-                           BP_SLOTS, cnt, off, nT, pb);
-        std::swap(pa, pb);
-    } else {
-        if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)total)) return r;
-        pa = (unsigned long long *)ctx->pat_a.p;
-        hipLaunchKernelGGL((k_bp_pairs<true>), dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skd, nT, skey,
-                           sidx, snode, box, exclude_shared, red, nullptr, off, pa, 0);
-    }
     size_t b5 = 0;
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, b5, pa, pb, (int)total, 0, 64, ctx->stream));
     if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;

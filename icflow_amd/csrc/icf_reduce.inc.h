@@ -497,6 +497,27 @@ __global__ __launch_bounds__(256) void k_copy(char *__restrict__ dst, const char
     }
 }
 
+// VALU issue-rate probe (icf_probe_valu): every thread runs `iters` rounds of 16 independent
+// fused multiply-add chains x = x * m + c (one VALU instruction each, no memory traffic in the
+// loop) and stores the sum of its chains, so that no chain can be dropped.  With enough waves
+// per SIMD the loop is bound by VALU issue alone: its rate is the achievable FMA peak.
+template <typename T>
+__global__ __launch_bounds__(256) void k_valu_probe(T *__restrict__ out, int iters, T m, T c)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    T x[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = (T)(tid & 7) * (T)0.125 + (T)k;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) x[k] = fma(x[k], m, c);
+    }
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += x[k];
+    out[tid] = s;
+}
+
 __global__ void k_fill_int(int *__restrict__ p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

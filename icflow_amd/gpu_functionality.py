@@ -159,9 +159,31 @@ class GPUFunctionality:
         check(self.lib.icf_set_chunk_keys(self.ctx, int(accumulate), int(fp64), keys, stage), self.ctx,
               "icf_set_chunk_keys")
 
+    # launch-shape roles besides the value reductions (icf_set_shape / icf_get_shape)
+    SHAPE_ROLES = ("phase1", "narrow", "closest", "impact")
+    SHAPE_CANDIDATES = {"phase1": (64, 128, 256), "narrow": (64, 128, 256), "closest": (4, 8, 16),
+                        "impact": (64, 128, 256)}
+
+    def set_shape(self, role, value):
+        check(self.lib.icf_set_shape(self.ctx, self.SHAPE_ROLES.index(role), int(value)), self.ctx, "icf_set_shape")
+
+    def shape_state(self):
+        """{role: (value in use, tuning?, {candidate: fastest ms})} for the four shape roles."""
+        out = {}
+        for i, r in enumerate(self.SHAPE_ROLES):
+            v, t = ctypes.c_int(), ctypes.c_int()
+            best = np.zeros(3, np.float32)
+            check(self.lib.icf_get_shape(self.ctx, i, ctypes.byref(v), ctypes.byref(t), _lib.ptr(best)), self.ctx,
+                  "icf_get_shape")
+            out[r] = (int(v.value), bool(t.value), {c: float(best[j]) for j, c in enumerate(self.SHAPE_CANDIDATES[r])})
+        return out
+
     def tuned_shapes(self):
-        """{role: [keys, stage]}: the shapes in use, to keep for a later run on the same specimen."""
-        return {r: list(v[0]) for r, v in self.autotune_state().items()}
+        """{role: [keys, stage]} for the value reductions and {role: value} for the other launch
+        shapes in use, to keep for a later run on the same specimen."""
+        out = {r: list(v[0]) for r, v in self.autotune_state().items()}
+        out.update({r: v[0] for r, v in self.shape_state().items()})
+        return out
 
     def use_shapes(self, shapes):
         """Fix the shapes of tuned_shapes() (for example read back from JSON) without tuning."""
@@ -169,6 +191,9 @@ class GPUFunctionality:
             if r in shapes:
                 keys, stage = shapes[r]
                 self.set_chunk_keys(i >> 1, i & 1, int(keys), int(stage))
+        for r in self.SHAPE_ROLES:
+            if r in shapes:
+                self.set_shape(r, shapes[r])
 
     # -- TransferStaticDataToDevice (:255-302) --
     def TransferStaticDataToDevice(self, col):

@@ -12,9 +12,9 @@ Per step:
                                  pattern is rebuilt (_acceptFrame, :290-300)
 
 The linear solve stands in for MKL PARDISO (LinearSystem.Solve, LinearSystem.cs:45-64,
-mtype -2 on the upper-triangular BCSR): the symmetric matrix is expanded from the upper
-blocks and factored with SuperLU (scipy.sparse.linalg.splu).  It is a host substitute and not
-PARDISO.
+mtype -2 on the upper-triangular BCSR): the library's SolveDouble3, a supernodal sparse LDL^T
+behind PARDISO's signature (SolveDouble3Solver; per specimen on host threads for batched
+specimens, BatchedSolver).  It is a host substitute and not PARDISO.
 
 Time-step control (StepParams.adapt): the reference's _checkDamage (:198-211), _adjustTimeStep
 (:258-288), the acceptance rule of Step (:415-431) and FrameInfo.IncrementTime
@@ -24,6 +24,9 @@ initial CZs is discarded and retried at twice the factor, and after converged st
 is halved again.  Without it every attempt is accepted at h0.
 """
 import ctypes
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -160,6 +163,67 @@ class SolveDouble3Solver:
 SOLVERS = {"SolveDouble3": SolveDouble3Solver, "banded": UpperBcsrSolver}
 
 
+class BatchedSolver:
+    """SolveDouble3 per specimen over the diagonal blocks of a block-diagonal upper BCSR matrix
+    (specimen s owns the rows [row_off[s], row_off[s+1]) and the blocks they start), the
+    specimens spread over a pool of host threads (largest first), each call single-threaded
+    inside (icfh_solve_threads(1)).  The reference solves each specimen in its own process
+    (LinearSystem.Solve, LinearSystem.cs:45-64)."""
+
+    def __init__(self, rows, cols, row_off, threads=None):
+        self.lib = _lib.load()
+        rows = np.ascontiguousarray(rows, np.int64)
+        cols = np.ascontiguousarray(cols, np.int64)
+        self.row_off = np.asarray(row_off, np.int64)
+        self.nz_off = rows[self.row_off]
+        self.parts = []
+        for s in range(len(self.row_off) - 1):
+            r0, r1 = int(self.row_off[s]), int(self.row_off[s + 1])
+            z0, z1 = int(self.nz_off[s]), int(self.nz_off[s + 1])
+            ja = cols[z0:z1] - r0
+            if len(ja) and (ja.min() < 0 or ja.max() >= r1 - r0):
+                raise ValueError(f"specimen {s}: a block outside its own rows (the matrix is not block-diagonal)")
+            self.parts.append((r0, r1, z0, z1, np.ascontiguousarray(rows[r0:r1 + 1] - z0, np.int32),
+                               np.ascontiguousarray(ja, np.int32)))
+        self.order = sorted(range(len(self.parts)), key=lambda s: -(self.parts[s][3] - self.parts[s][2]))
+        self.threads = threads or min(16, len(os.sched_getaffinity(0)))
+        self.pool = ThreadPoolExecutor(self.threads)
+        self.solve_s = 0.0
+
+    def solve(self, vals, rhs, active=None):
+        """dx for the specimens with active[s] (all by default); zero rows for the others."""
+        vals = np.ascontiguousarray(vals, np.float64)
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        dx = np.zeros_like(rhs)
+        lib = self.lib
+
+        def one(s):
+            r0, r1, z0, z1, ia, ja = self.parts[s]
+            if r1 == r0:
+                return 0
+            x = np.zeros(3 * (r1 - r0))
+            r = lib.SolveDouble3(_lib.ptr(ja), _lib.ptr(ia), _lib.ptr(vals[9 * z0:9 * z1]), r1 - r0,
+                                 _lib.ptr(rhs[3 * r0:3 * r1]), _lib.ptr(x), -2, 0, 3, 0, 0)
+            dx[3 * r0:3 * r1] = x
+            return r
+
+        todo = [s for s in self.order if active is None or active[s]]
+        t0 = time.perf_counter()
+        lib.icfh_solve_threads(1)
+        try:
+            codes = list(self.pool.map(one, todo))
+        finally:
+            lib.icfh_solve_threads(0)
+        self.solve_s += time.perf_counter() - t0
+        bad = [(s, c) for s, c in zip(todo, codes) if c]
+        if bad:
+            raise RuntimeError(f"SolveDouble3 failed for specimens {bad[:5]}")
+        return dx
+
+    def close(self):
+        self.pool.shutdown()
+
+
 def adjust_time_step(tsf, hold, explodes, diverges, converged, max_iter):
     """_adjustTimeStep (ImplicitModel3.cs:258-288) and the acceptance rule of Step (:415-420)
     for an attempt run at time-scale factor tsf with hold count hold.  Returns (accepted,
@@ -187,8 +251,11 @@ class ImplicitModel:
     of the anchored SurfaceFragments' dx, dy, dz with their application time)."""
 
     def __init__(self, col, consts, precision=_lib.ICF_FP64, step=None, prescribed=None, prescribed_nodes=None,
-                 contact=False, penalty=None, dist_eps=1e-15, solver="SolveDouble3"):
+                 contact=False, penalty=None, dist_eps=1e-15, solver="SolveDouble3", perturb=0.0, perturb_seed=0):
         self.col, self.consts = col, consts
+        # sensitivity studies: every assembled BCSR value is multiplied by (1 + perturb * N(0,1))
+        # before the solve
+        self.perturb, self._rng = perturb, np.random.default_rng(perturb_seed)
         self.solver_cls = SOLVERS[solver]
         self.p = step or StepParams()
         self.prescribed = prescribed
@@ -419,6 +486,8 @@ class ImplicitModel:
                 explodes = self._check_damage(dmg.value, fld.value)
             vals = self.g_vals.download(9 * self.nnz)
             rhs = self.g_rhs.download(3 * self.N)
+            if self.perturb:
+                vals = vals * (1.0 + self.perturb * self._rng.standard_normal(vals.size))
             dx = self.solver.solve(vals, rhs)
             norm = float(dx @ dx)                       # LinearSystem.NormOfDx (:68-74)
             norms.append(norm)
@@ -481,6 +550,238 @@ class ImplicitModel:
 
     def close(self):
         self.gf.close()
+
+
+class BatchedImplicitModel(ImplicitModel):
+    """Independent specimens of a batched collection (specimens.batch) advanced together with
+    one fixed time step: each Newton iteration is one icf_assemble_batched over every specimen,
+    each specimen's diagonal block is solved on the host (BatchedSolver), and each specimen
+    iterates to its own convergence with the reference's criterion and iteration limits
+    (ImplicitModel3.Step's loop, ImplicitModel3.cs:369-435, run per specimen); a specimen that
+    has finished keeps its state (dx = 0) while the others iterate."""
+
+    def __init__(self, col, consts, precision=_lib.ICF_FP64, step=None, prescribed=None, prescribed_nodes=None,
+                 threads=None):
+        from . import specimens
+        self.nspec = len(col.meshes)
+        tet, cz, row = specimens.specimen_offsets(col)
+        self.tet_off, self.row_off = tet, row
+        self.cz_spec = col.node_mesh[np.asarray(col.czs)[:, 0]] if len(col.czs) else np.zeros(0, np.int64)
+        nm = np.asarray(col.node_mesh)
+        if np.any(np.diff(nm) < 0):
+            raise ValueError("nodes are not ordered by specimen")
+        self.node_off = np.r_[0, np.cumsum(np.bincount(nm, minlength=self.nspec))]
+        self.threads = threads
+        self.timing = dict(assemble_ms=0.0, transfer_s=0.0, solve_s=0.0, iterations=0)
+        super().__init__(col, consts, precision=precision, step=step or StepParams(adapt=False), prescribed=prescribed,
+                         prescribed_nodes=prescribed_nodes)
+        if self.p.adapt:
+            raise ValueError("batched specimens share one fixed time step (adapt=False)")
+
+    def _setup_czs(self):
+        super()._setup_czs()
+        cz = np.bincount(self.cz_spec[self.cz_alive], minlength=self.nspec) if self.C else np.zeros(self.nspec, int)
+        self.cz_off = np.r_[0, np.cumsum(cz)].astype(np.int32)
+        self.nnzb_off = self.rows[self.row_off].astype(np.int32)
+        self.wl_arrays = [np.ascontiguousarray(a, np.int32) for a in (self.tet_off, self.cz_off, self.row_off,
+                                                                        self.nnzb_off)]
+        self.worklist = _lib.IcfWorklist(self.nspec, *[_lib.ptr(a) for a in self.wl_arrays])
+        if getattr(self, "bsolver", None) is not None:
+            self.bsolver.close()
+        self.bsolver = BatchedSolver(self.rows, self.cols, self.row_off, self.threads)
+
+    def step(self):
+        """One fixed step of every specimen; the record carries the largest iteration count, and
+        per-specimen iterations and convergence are kept in self.spec_records."""
+        p, lib, ctx = self.p, self.lib, self.ctx
+        h, t_new = p.h, self.t + p.h
+        self._begin_step(h, t_new)
+        S = self.nspec
+        it = np.zeros(S, np.int64)
+        err0 = np.zeros(S)
+        conv = np.zeros(S, bool)
+        div = np.zeros(S, bool)
+        done = np.zeros(S, bool)
+        dmg = (ctypes.c_int * S)()
+        fld = (ctypes.c_int * S)()
+        held = np.zeros(S, bool)
+        dmg_last = np.zeros(S, np.int64)
+        spec_of_row = np.repeat(np.arange(S), np.diff(self.row_off))
+        while True:
+            _lib.check(lib.icf_nodes_infer(ctx, self.g_st.p, self.st_stride, self.g_dn.p, self.ns, self.n, h, p.beta,
+                                           p.gamma), ctx, "icf_nodes_infer")
+            if self.C:
+                self._upload_history()
+                _lib.check(lib.icf_pack_pcsr(ctx, self.g_alt.p, self.n, self.g_rows.p, self.g_cols.p, self.N,
+                                             None, 0, 0, self.g_icz.p, self.C, self.cs, None, 0, 0), ctx, "pack")
+            t0 = time.perf_counter()
+            _lib.check(lib.icf_assemble_batched(
+                ctx, ctypes.byref(self.worklist), self.g_ie.p, self.es, self.g_dcz.p if self.C else None,
+                self.g_icz.p if self.C else None, self.cs, self.g_dn.p, self.ns, h, self.g_vals.p, self.g_rhs.p,
+                dmg, fld), ctx, "icf_assemble_batched")
+            t1 = time.perf_counter()
+            vals = self.g_vals.download(9 * self.nnz)
+            rhs = self.g_rhs.download(3 * self.N)
+            t2 = time.perf_counter()
+            active = ~done
+            dx = self.bsolver.solve(vals, rhs, active)
+            sq = np.bincount(spec_of_row, weights=(dx * dx).reshape(-1, 3).sum(1), minlength=S)
+            for s in np.flatnonzero(active):        # _checkDivergence per specimen (ImplicitModel3.cs:214-245)
+                norm = sq[s]
+                if it[s] == 0:
+                    err0[s] = norm if norm != 0 else p.conv_eps
+                elif norm < p.conv_cutoff:
+                    conv[s] = True
+                else:
+                    rel = np.sqrt(norm / err0[s])
+                    conv[s] = rel <= p.conv_eps
+                    div[s] = rel > 1.01
+            self.g_dx.upload(np.ascontiguousarray(dx))
+            _lib.check(lib.icf_nodes_add_dx(ctx, self.g_st.p, self.st_stride, self.g_alt.p, self.n, self.g_dx.p),
+                       ctx, "icf_nodes_add_dx")
+            self.timing["assemble_ms"] += 1e3 * (t1 - t0)
+            self.timing["transfer_s"] += t2 - t1
+            self.timing["iterations"] += 1
+            it[active] += 1
+            done |= active & ~((it < p.min_iter) | (~div & ~conv & (it < p.max_iter)))
+            newly = np.flatnonzero(active & done)
+            if done.all():
+                break
+            # a specimen that has finished keeps the CZ state of its last assembly (the one its
+            # own loop would have accepted), not that of later assemblies at its final iterate
+            self._held_fields(newly, save=True)
+            held[newly] = True
+            dmg_last[newly] = np.asarray(dmg)[newly]
+        self._held_fields(np.flatnonzero(held), save=False)
+        dmg_last[~held] = np.asarray(dmg)[~held]
+        self.timing["solve_s"] = self.bsolver.solve_s
+        _lib.check(lib.icf_nodes_accept(ctx, self.g_st.p, self.st_stride, self.g_dn.p, self.ns, self.n, h), ctx,
+                   "icf_nodes_accept")
+        failed_now = 0
+        if self.C:
+            d = self.g_dcz.download(16 * self.cs)
+            icz = self.g_icz.download(4 * self.cs)
+            for j in range(3):
+                self.pmax[self.cz_alive, j] = d[(6 + j) * self.cs:(6 + j) * self.cs + self.C]
+                self.tmax[self.cz_alive, j] = d[(9 + j) * self.cs:(9 + j) * self.cs + self.C]
+            failed = icz[3 * self.cs:3 * self.cs + self.C] != 0
+            failed_now = int(failed.sum())
+            if failed_now:
+                self.cz_alive = self.cz_alive[~failed]
+                self._setup_czs()
+        self.t = t_new
+        self.spec_records = getattr(self, "spec_records", [])
+        self.spec_records.append((it.copy(), conv.copy()))
+        rec = StepRecord(int(it.max()), bool(conv.all()), failed_now, int(dmg_last.sum()), [], accepted=True, h=h)
+        self.records.append(rec)
+        return rec
+
+    # what the accept reads of a finished specimen: the tentative displacement un as its last
+    # inference left it (dn 3-5; Node.AcceptTentativeValues takes un as last inferred, before the
+    # loop's final dx), and the CZ fields of its last assembly: tentative pmax, tmax and averages
+    # (dcz 6-15), tentative contact, damaged and failed flags (icz 1-3)
+    _DN_FIELDS, _DCZ_FIELDS, _ICZ_FIELDS = range(3, 6), range(6, 16), range(1, 4)
+
+    def _held_fields(self, specs, save):
+        """Copy those fields of the specimens `specs` into (save) or back from (restore) a
+        device copy, so that later iterations of the other specimens do not change them."""
+        if len(specs) == 0:
+            return
+        gf, lib, ctx, cs, ns = self.gf, self.lib, self.ctx, self.cs, self.ns
+        groups = [(self._DN_FIELDS, self.g_dn, gf.buf("held_dn", 6 * ns, np.float64), 8, ns, self.node_off)]
+        if self.C:
+            groups += [(self._DCZ_FIELDS, self.g_dcz, gf.buf("held_dcz", 16 * cs, np.float64), 8, cs, self.cz_off),
+                       (self._ICZ_FIELDS, self.g_icz, gf.buf("held_icz", 4 * cs, np.int32), 4, cs, self.cz_off)]
+        for fields, live, keep, w, stride, offs in groups:
+            for s in specs:
+                u0, u1 = int(offs[s]), int(offs[s + 1])
+                if u1 == u0:
+                    continue
+                for f in fields:
+                    off = w * (f * stride + u0)
+                    src, dst = (live, keep) if save else (keep, live)
+                    _lib.check(lib.icf_d2d(ctx, dst.p, off, src.p, off, w * (u1 - u0)), ctx, "icf_d2d")
+
+    def close(self):
+        if getattr(self, "bsolver", None) is not None:
+            self.bsolver.close()
+        super().close()
+
+
+def tension(col, strain=3e-4, ramp_steps=10, h=1e-4, size=0.02):
+    """Specimens of edge `size` standing on z = 0, each pulled in z: the nodes of its bottom face
+    fixed, those of its top face moved up by strain * size, ramped linearly over ramp_steps
+    steps and then held.  Finalizes `col`; returns it, the prescribed displacement of the
+    anchored nodes at time t, and those nodes."""
+    if getattr(col, "x0", None) is None:
+        col.finalize()
+    z = col.x0[:, 2]
+    bottom, top = np.isclose(z, 0.0), np.isclose(z, size)
+    col.anchored = bottom | top
+    col.finalize()
+    ids = np.flatnonzero(col.anchored)
+    is_top = np.isclose(col.x0[ids, 2], size)
+    T_ramp = ramp_steps * h
+
+    def prescribed(t):
+        out = np.zeros((len(ids), 3))
+        out[is_top, 2] = strain * size * min(1.0, t / T_ramp)
+        return out
+
+    return col, prescribed, ids
+
+
+def batched_tension(consts, n_specimens=256, seed=2, strain=3e-4, ramp_steps=10, h=1e-4, size=0.02, **cubes):
+    """tension of the batched specimens (specimens.lognormal_cubes, seed 2 for the benchmark's
+    256)."""
+    from . import specimens
+    return tension(specimens.lognormal_cubes(n_specimens, seed, size=size, **cubes), strain, ramp_steps, h, size)
+
+
+def batched_precision_study(consts, n_specimens=256, steps=4, seed=2, strain=3e-4, ramp_steps=10, h=1e-4,
+                            threads=None, log=None):
+    """batched_tension in FP64 and in mixed precision for `steps` fixed steps: the relative
+    displacement error of the mixed run (over all nodes, and the largest over specimens), the
+    failed-CZ sets, Newton iterations, and where the time of an iteration goes (the batched
+    assembly, the values' and residual's transfer to the host, the host solves)."""
+    out = {}
+    runs = {}
+    for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
+        col, presc, ids = batched_tension(consts, n_specimens, seed, strain, ramp_steps, h)
+        t0 = time.time()
+        m = BatchedImplicitModel(col, consts, precision=mode, step=StepParams(h=h), prescribed=presc,
+                                 prescribed_nodes=ids, threads=threads)
+        for k in range(steps):
+            rec = m.step()
+            if log:
+                log(f"batched newton {name}: step {k + 1}, {rec.iterations} iterations at most, "
+                    f"{len(m.failed_set())} CZs failed, {m.timing}")
+        runs[name] = (m.displacements(), m.failed_set(), [r.iterations for r in m.records],
+                      int(sum(int(i.sum()) for i, _ in m.spec_records)), col.node_mesh.copy())
+        tm = dict(m.timing)
+        it = max(1, tm["iterations"])
+        out[f"timing_{name}"] = dict(iterations=tm["iterations"], wall_s=round(time.time() - t0, 1),
+                                     assemble_ms_per_iter=round(tm["assemble_ms"] / it, 3),
+                                     transfer_ms_per_iter=round(1e3 * tm["transfer_s"] / it, 1),
+                                     solve_ms_per_iter=round(1e3 * tm["solve_s"] / it, 1),
+                                     solve_threads=m.bsolver.threads)
+        out.update(specimens=n_specimens, tets=int(m.T), czs=int(len(m.cz_nodes)), rows=int(m.N), nnzb=int(m.nnz))
+        m.close()
+    u64, f64, it64, sit64, nm = runs["fp64"]
+    umx, fmx, itmx, sitmx, _ = runs["mixed"]
+    scale = float(np.linalg.norm(u64))
+    per = []
+    for s in range(n_specimens):
+        sel = nm == s
+        d = float(np.linalg.norm(u64[sel]))
+        per.append(float(np.linalg.norm(umx[sel] - u64[sel]) / d) if d > 0 else 0.0)
+    out.update(steps=steps, dt=h, strain=strain, ramp_steps=ramp_steps,
+               rel_err_displacement=float(np.linalg.norm(umx - u64) / scale) if scale > 0 else 0.0,
+               max_specimen_rel_err=float(max(per)), max_abs_displacement=float(np.abs(u64).max()),
+               failed_czs_fp64=int(len(f64)), failed_czs_mixed=int(len(fmx)),
+               failed_set_identical=bool(np.array_equal(f64, fmx)),
+               newton_iterations_fp64=sit64, newton_iterations_mixed=sitmx)
+    return out
 
 
 def three_point_bend(consts, cells=(40, 8, 8), size=(0.40, 0.05, 0.05), grains=16, seed=0, deflection=2e-3,
@@ -573,8 +874,59 @@ def contact_precision_study(consts, steps=20, h=1e-4, adapt=True, max_attempts=4
     return out
 
 
+def bend_run(consts, mode, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-4, ramp_steps=200, adapt=False,
+             max_attempts=4000, conv_eps=0.005, max_iter=10, conv_cutoff=1e-8, log=None, name=None, history=False,
+             perturb=0.0, contact=False):
+    """One three-point-bend run up to time steps * h in precision `mode` with the Newton
+    tolerance `conv_eps` (relative) and `conv_cutoff` (absolute, on the squared norm of dx) and
+    at most `max_iter` iterations per step.  Returns a dict with the
+    displacements `u`, the failed CZs, the per-frame iterations, convergence and newly failed
+    CZs, the end time and the attempts; with `history` also each frame's time and
+    displacements (`frame_t`, `frame_u`); `perturb` as ImplicitModel's.  With `contact`, every
+    exterior and CZ face of the beam is in the contact surface (self-contact, penalty 1e11 N/m^3
+    times the square of the cell edge along the beam)."""
+    col, presc = three_point_bend(consts, cells=cells, grains=grains, seed=seed, ramp_steps=ramp_steps, h=h)
+    kw = dict(contact=True, penalty=1e11 * (0.40 / cells[0]) ** 2) if contact else {}   # the 0.4 m beam's cell
+    m = ImplicitModel(col, consts, precision=mode, prescribed=presc, perturb=perturb, **kw,
+                      step=StepParams(h=h, adapt=adapt, conv_eps=conv_eps, max_iter=max_iter, conv_cutoff=conv_cutoff))
+    attempts, frames, max_tsf = 0, 0, 1
+    frame_t, frame_u, impacts = [], [], []
+    name = name or ("fp64" if mode == _lib.ICF_FP64 else "mixed")
+    while attempts < max_attempts and (m.int_time < steps * PARTS if adapt else frames < steps):
+        max_tsf = max(max_tsf, m.tsf)
+        rec = m.step()
+        attempts += 1
+        frames += rec.accepted
+        if history and rec.accepted:
+            frame_t.append(m.t)
+            frame_u.append(m.displacements())
+        if contact and rec.accepted:
+            impacts.append(m.nImpacts)
+        if log and rec.accepted and (frames % max(1, steps // 5) == 0 or rec.failed_czs):
+            log(f"newton {name}: frame {frames} t={m.t:.6g} (attempt {attempts}, tsf {max_tsf}), "
+                f"{rec.iterations} iterations, {len(m.failed_set())} CZs failed")
+    acc = [r for r in m.records if r.accepted]
+    out = dict(u=m.displacements(), failed=m.failed_set(), iterations=[r.iterations for r in acc],
+               converged=[r.converged for r in acc], failed_now=[r.failed_czs for r in acc],
+               clock=m.int_time if adapt else frames, t=m.t, attempts=attempts, frames=frames,
+               finest_step=h / max_tsf, nodes=int(m.n), czs=int(len(m.cz_nodes)), impacts=impacts)
+    if history:
+        out.update(frame_t=frame_t, frame_u=frame_u)
+    m.close()
+    return out
+
+
+def compare_runs(a, b):
+    """Relative displacement difference of run b from run a (normwise over all nodes), and
+    whether the failed-CZ sets and the end times agree."""
+    scale = float(np.linalg.norm(a["u"]))
+    return dict(rel_err_displacement=float(np.linalg.norm(b["u"] - a["u"]) / scale) if scale > 0 else 0.0,
+                same_end_time=bool(a["clock"] == b["clock"]),
+                failed_set_identical=bool(np.array_equal(a["failed"], b["failed"])))
+
+
 def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-4, ramp_steps=200, adapt=False,
-                    max_attempts=4000, log=None):
+                    max_attempts=4000, conv_eps=0.005, max_iter=10, conv_cutoff=1e-8, contact=False, log=None):
     """The same three-point-bend run in FP64 and in mixed precision up to time steps * h:
     relative displacement error of the mixed run at the end (normwise over all nodes), the
     failed-CZ sets, Newton iterations, and with adapt (the reference's time-step control) the
@@ -582,35 +934,25 @@ def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-
     out = {}
     runs = {}
     for name, mode in (("fp64", _lib.ICF_FP64), ("mixed", _lib.ICF_MIXED)):
-        col, presc = three_point_bend(consts, cells=cells, grains=grains, seed=seed, ramp_steps=ramp_steps, h=h)
-        m = ImplicitModel(col, consts, precision=mode, step=StepParams(h=h, adapt=adapt), prescribed=presc)
-        attempts, frames, max_tsf = 0, 0, 1
-        while attempts < max_attempts and (m.int_time < steps * PARTS if adapt else frames < steps):
-            max_tsf = max(max_tsf, m.tsf)
-            rec = m.step()
-            attempts += 1
-            frames += rec.accepted
-            if log and rec.accepted and (frames % max(1, steps // 5) == 0 or rec.failed_czs):
-                log(f"newton {name}: frame {frames} t={m.t:.6g} (attempt {attempts}, tsf {max_tsf}), "
-                    f"{rec.iterations} iterations, {len(m.failed_set())} CZs failed")
-        runs[name] = (m.displacements(), m.failed_set(), [r.iterations for r in m.records if r.accepted],
-                      [r.converged for r in m.records if r.accepted], m.int_time if adapt else frames, m.t)
-        out[f"attempts_{name}"] = attempts
-        out[f"discarded_{name}"] = attempts - frames
-        out[f"finest_step_{name}"] = h / max_tsf
-        out["nodes"] = int(m.n)
-        out["czs"] = int(len(m.cz_nodes))
-        m.close()
-    u64, f64, it64, c64, k64, t64 = runs["fp64"]
-    umx, fmx, itmx, cmx, kmx, tmx = runs["mixed"]
-    scale = float(np.linalg.norm(u64))
-    out.update(steps=steps, dt=h, adapt=adapt, t_end_fp64=t64, t_end_mixed=tmx, same_end_time=bool(k64 == kmx),
-               rel_err_displacement=float(np.linalg.norm(umx - u64) / scale) if scale > 0 else 0.0,
-               max_abs_displacement=float(np.abs(u64).max()),
-               failed_czs_fp64=int(len(f64)), failed_czs_mixed=int(len(fmx)),
-               failed_set_identical=bool(np.array_equal(f64, fmx)),
-               newton_iterations_fp64=int(sum(it64)), newton_iterations_mixed=int(sum(itmx)),
-               steps_converged_fp64=int(sum(c64)), steps_converged_mixed=int(sum(cmx)))
+        r = runs[name] = bend_run(consts, mode, steps=steps, cells=cells, grains=grains, seed=seed, h=h,
+                                  ramp_steps=ramp_steps, adapt=adapt, max_attempts=max_attempts, conv_eps=conv_eps,
+                                  max_iter=max_iter, conv_cutoff=conv_cutoff, log=log, name=name, contact=contact)
+        out[f"attempts_{name}"] = r["attempts"]
+        out[f"discarded_{name}"] = r["attempts"] - r["frames"]
+        out[f"finest_step_{name}"] = r["finest_step"]
+        out["nodes"] = r["nodes"]
+        out["czs"] = r["czs"]
+        if contact:
+            out[f"impacts_max_{name}"] = int(max(r["impacts"], default=0))
+            out[f"impacts_last_{name}"] = int(r["impacts"][-1]) if r["impacts"] else 0
+    r64, rmx = runs["fp64"], runs["mixed"]
+    out.update(steps=steps, dt=h, adapt=adapt, contact=contact, ramp_steps=ramp_steps, conv_eps=conv_eps,
+               conv_cutoff=conv_cutoff, max_iter=max_iter, t_end_fp64=r64["t"],
+               t_end_mixed=rmx["t"], **compare_runs(r64, rmx),
+               max_abs_displacement=float(np.abs(r64["u"]).max()),
+               failed_czs_fp64=int(len(r64["failed"])), failed_czs_mixed=int(len(rmx["failed"])),
+               newton_iterations_fp64=int(sum(r64["iterations"])), newton_iterations_mixed=int(sum(rmx["iterations"])),
+               steps_converged_fp64=int(sum(r64["converged"])), steps_converged_mixed=int(sum(rmx["converged"])))
     return out
 
 
@@ -624,6 +966,8 @@ def main(argv=None):
     ap.add_argument("--ramp", type=int, default=200, help="steps of the deflection ramp (2 mm)")
     ap.add_argument("--adapt", action="store_true", help="the reference's time-step control")
     ap.add_argument("--max-attempts", type=int, default=4000)
+    ap.add_argument("--conv-eps", type=float, default=0.005, help="Newton tolerance (ConvergenceEpsilon)")
+    ap.add_argument("--max-iter", type=int, default=10, help="Newton iterations per step at most")
     ap.add_argument("--contact", action="store_true",
                     help="platen compression with contact (platen_compression) instead of the three-point bend")
     a = ap.parse_args(argv)
@@ -633,7 +977,7 @@ def main(argv=None):
                                     log=lambda m: print(m, flush=True))
     else:
         r = precision_study(consts, steps=a.steps, ramp_steps=a.ramp, adapt=a.adapt, max_attempts=a.max_attempts,
-                            log=lambda m: print(m, flush=True))
+                            conv_eps=a.conv_eps, max_iter=a.max_iter, log=lambda m: print(m, flush=True))
     print(json.dumps(r), flush=True)
 
 

@@ -380,6 +380,12 @@ def lognormal_cubes(n_specimens, seed, median_tets=12000, min_tets=2000, max_tet
     log-standard deviation `sigma`, truncated to [min_tets, max_tets]), each a hex grid of
     n^3 cells x 6 tets with n = round((tets/6)^(1/3)) and its own Voronoi grain count drawn
     uniformly from `grains` and its own grain seed.  Specimens are laid out along x, apart."""
+    return batch(lognormal_specimens(n_specimens, seed, median_tets, min_tets, max_tets, sigma, grains, size))
+
+
+def lognormal_specimens(n_specimens, seed, median_tets=12000, min_tets=2000, max_tets=40000, sigma=0.6,
+                        grains=(8, 64), size=0.02):
+    """The specimens of lognormal_cubes as separate (unfinalized) collections."""
     rng = np.random.default_rng(seed)
     cols = []
     x = 0.0
@@ -390,4 +396,4 @@ def lognormal_cubes(n_specimens, seed, median_tets=12000, min_tets=2000, max_tet
         c = make_specimen((n, n, n), (size, size, size), g, int(rng.integers(1 << 30)), origin=(x, 0.0, 0.0))
         cols.append(c)
         x += 2 * size
-    return batch(cols)
+    return cols

@@ -177,6 +177,13 @@ int icf_pattern_changed(icf_ctx *ctx);
  * stream); -1 if the last call launched nothing. */
 double icf_last_device_ms(const icf_ctx *ctx);
 
+/* VALU issue-rate probe (no counterpart in the reference: a measurement aid beside the
+ * icf_d2d copy used as the HBM probe).  Launches `workgroups` x 256 threads that each run
+ * `iters` rounds of 16 independent FMA chains in FP64 (fp64 != 0) or FP32 and store one value
+ * each into `out` (device memory of workgroups * 256 doubles or floats).  Time it with
+ * icf_last_device_ms in blocking mode: 2 * 16 * iters * 256 * workgroups flops. */
+int icf_probe_valu(icf_ctx *ctx, int fp64, int workgroups, int iters, void *out);
+
 /* Per-kernel timing (the role of FrameInfo's KerElemForce / KerCZForce / CollForce
  * stopwatches, FrameInfo.cs:75-105): when on, every kernel launch is bracketed by HIP events
  * on the context's stream and its duration is accumulated under the kernel's name.
@@ -247,6 +254,27 @@ int icf_set_overlap(icf_ctx *ctx, int on);
 int icf_set_autotune(icf_ctx *ctx, int rounds);
 int icf_get_autotune(icf_ctx *ctx, int *keys, int *stage, int *tuning, float *best_ms);
 int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys, int stage);
+
+/* Launch shapes besides the value reductions (no counterpart in the reference, which launches
+ * everything with 128-thread blocks, GPU_Functionality.cs:38), tuned with the value reductions
+ * by icf_set_autotune (three candidates per role, timed in turn on the role's launches):
+ *   ICF_SHAPE_PHASE1   workgroup size of phase 1 of the fused call (64, 128, 256)
+ *   ICF_SHAPE_NARROW   workgroup size of the narrow phase (64, 128, 256)
+ *   ICF_SHAPE_CLOSEST  lanes per (node, tet) tuple of the closest face (4, 8, 16)
+ *   ICF_SHAPE_IMPACT   workgroup size of the impacts' phase 1 (64, 128, 256)
+ * Every candidate gives the same results bit for bit.  icf_set_shape fixes a role's value on
+ * this context (ICF_E_INVALID for other values); icf_get_shape reports the context's value
+ * (the default until tuned or set), whether the role is still being tuned and the fastest
+ * timing per candidate (best_ms[3], 0: not timed).  Finished tunings, of these roles and of the
+ * value reductions, are kept by the library per device, role, precision and size class (units
+ * of the launch to the next power of two); a context's launches of a role that it is neither
+ * tuning nor has set use the kept choice for their size class, when there is one. */
+#define ICF_SHAPE_PHASE1 0
+#define ICF_SHAPE_NARROW 1
+#define ICF_SHAPE_CLOSEST 2
+#define ICF_SHAPE_IMPACT 3
+int icf_set_shape(icf_ctx *ctx, int role, int value);
+int icf_get_shape(icf_ctx *ctx, int role, int *value, int *tuning, float *best_ms);
 int icf_profile_reset(icf_ctx *ctx);
 int icf_profile_count(const icf_ctx *ctx);
 int icf_profile_entry(const icf_ctx *ctx, int i, char *name, int cap, double *total_ms, long long *launches);

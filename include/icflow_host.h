@@ -78,15 +78,22 @@ int icfh_kdop_pairs(const double *dn, int nd_stride, const int *ie, int el_all_s
  * (PardisoLoader2/dllmain.cpp:29-79) and is NOT MKL PARDISO: it solves A x = b for the
  * symmetric matrix A given by its upper-triangular BCSR with 3x3 blocks (ja: block column
  * indices, ia: n + 1 block row starts, zero-based; a: 9 doubles per block, row-major) with a
- * sparse LDL^T (nested-dissection ordering of the block graph, 3x3 pivot blocks, small pivots
- * perturbed by 1e-8 of the largest entry followed by iterative refinement).  mtype must be -2
+ * supernodal multifrontal LDL^T (nested-dissection or reverse Cuthill-McKee ordering of the
+ * block graph, whichever fills less; 3x3 pivot blocks, small pivots perturbed by 1e-8 of the
+ * largest entry followed by iterative refinement).  mtype must be -2
  * (symmetric indefinite, the reference's value) or 2, dim must be 3; iparam4 is ignored (the
  * reference passes 0); msglvl > 0 prints sizes and timings to stderr; check != 0 verifies the
  * structure first.  Returns 0, or PARDISO's codes: -1 inconsistent input, -2 out of memory,
- * -3 ordering failed, -4 zero pivot.  x is overwritten; b is not changed.  Thread-safe (calls
- * are serialised); the ordering of the last pattern is kept between calls. */
+ * -3 ordering failed, -4 zero pivot.  x is overwritten; b is not changed.  Reentrant: threads
+ * may solve different systems at once; the analyses of the last 1024 patterns are kept. */
 int SolveDouble3(int *ja, int *ia, double *a, const int n, double *b, double *x, int mtype, int iparam4,
                  int dim, int msglvl, int check);
+
+/* Threads SolveDouble3 uses inside one call for the dense updates of large fronts (no
+ * counterpart in the reference, whose MKL reads MKL_NUM_THREADS): 0 = OpenMP's default, 1 =
+ * none (for callers that solve several systems on threads of their own).  Returns 0, or -1
+ * for a negative count. */
+int icfh_solve_threads(int threads);
 
 #ifdef __cplusplus
 }

@@ -103,7 +103,7 @@ def test_every_chunk_shape_gives_the_same_bits(specimen_consts, precision):
     for i, (keys, stg) in enumerate(((64, 2), (512, 3), (128, 3), (256, 2))):
         gf.set_chunk_keys(i >> 1, i & 1, keys, stg)
     saved = json.loads(json.dumps(gf.tuned_shapes()))
-    assert len({tuple(v) for v in saved.values()}) == 4
+    assert len({tuple(saved[r]) for r in gf.ROLES}) == 4
     gf.close()
     p2 = Pipeline(col, specimen_consts, st, precision=precision, contact=True)
     p2.gf.use_shapes(saved)
@@ -111,3 +111,55 @@ def test_every_chunk_shape_gives_the_same_bits(specimen_consts, precision):
         assert list(shape) == saved[r] and not tuning, (r, shape)
     _same(_iteration(p2), ref, "restored shapes")
     p2.gf.close()
+
+
+SHAPES = {"phase1": (64, 128, 256), "narrow": (64, 128, 256), "closest": (4, 8, 16), "impact": (64, 128, 256)}
+
+
+@pytest.mark.parametrize("precision", [_lib.ICF_FP64, _lib.ICF_MIXED], ids=["fp64", "mixed"])
+def test_every_launch_shape_gives_the_same_bits(specimen_consts, precision):
+    """The other tuned roles (icf_set_shape / icf_get_shape): phase 1's, the narrow phase's and
+    the impacts' workgroup sizes and the closest face's lanes per tuple.  The whole iteration
+    with contact (narrow phase and closest face included, their outputs compared) gives the
+    same bits under every value of every role and after tuning them; tuning times every
+    candidate of each role this iteration launches and keeps the fastest."""
+    col = small_specimen(cells=(10, 8, 6), size=(0.1, 0.08, 0.06), grains=12, seed=31)
+    st = make_state(col, specimen_consts, seed=32, platen_dz_frac=0.3, cells_z=6, size_z=0.06)
+    p = Pipeline(col, specimen_consts, st, precision=precision, contact=True)
+    gf = p.gf
+    assert gf.nImpacts > 0
+    defaults = {"phase1": 256, "narrow": 256, "closest": 8, "impact": 256}
+    for r, (v, tuning, _) in gf.shape_state().items():
+        assert v == defaults[r] and not tuning, (r, v)
+
+    def contact():
+        imp = gf.NarrowPhaseCollisionDetection(p.pairs)
+        return gf.masks.copy(), imp.copy()
+
+    ref = _iteration(p)
+    ref_masks, ref_imp = contact()
+    assert np.array_equal(ref_imp, p.impacts)
+    for role, cands in SHAPES.items():
+        for v in cands:
+            gf.set_shape(role, v)
+            assert gf.shape_state()[role][0] == v
+            masks, imp = contact()
+            assert np.array_equal(masks, ref_masks) and np.array_equal(imp, ref_imp), (role, v)
+            _same(_iteration(p), ref, (role, v))
+        gf.set_shape(role, defaults[role])
+    for role, bad in (("phase1", 32), ("narrow", 512), ("closest", 2), ("impact", 100)):
+        assert gf.lib.icf_set_shape(gf.ctx, gf.SHAPE_ROLES.index(role), bad) == _lib.ICF_E_INVALID
+    assert gf.lib.icf_set_shape(gf.ctx, 7, 64) == _lib.ICF_E_INVALID
+    # tuning every role at once: each iteration launches all four shape roles
+    gf.set_autotune(2)
+    for it in range(2 * len(CANDS) + 1):
+        masks, imp = contact()
+        assert np.array_equal(masks, ref_masks) and np.array_equal(imp, ref_imp), ("tuning", it)
+        _same(_iteration(p), ref, ("tuning", it))
+    for role, (v, tuning, best) in gf.shape_state().items():
+        assert v in SHAPES[role] and not tuning, (role, v, tuning)
+        assert all(ms > 0 for ms in best.values()), (role, best)
+        assert best[v] == min(best.values()), (role, best)
+    _same(_iteration(p), ref, "tuned")
+    gf.set_autotune(0)
+    gf.close()

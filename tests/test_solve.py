@@ -131,3 +131,92 @@ def test_newton_solver_class(lib, beam_system):
     x = newton.SolveDouble3Solver(rows, cols).solve(A, b)
     xb = newton.UpperBcsrSolver(rows, cols).solve(A, b)
     assert np.abs(x - xb).max() <= 1e-10 * np.abs(xb).max()
+
+
+def _cube_system(orc, lib, consts, cells, seed):
+    """An 8-grain cube's element + CZ Newmark system, bottom face fixed (the oracle's assembly)."""
+    from icflow_amd import specimens
+    from icflow_amd.gpu_functionality import CSR, stride_of
+    col = specimens.make_specimen((cells,) * 3, (0.02,) * 3, 8, seed)
+    x = col.meshes[0].x0
+    col.anchored = np.isclose(x[:, 2], 0.0)
+    col.finalize()
+    alt = np.ascontiguousarray(col.alt, np.int32)
+    csr = CSR(lib).build(alt[col.elastic], alt[col.czs])
+    T, C, Nn = len(col.elastic), len(col.czs), len(col.x0)
+    es, cs, ns = stride_of(T), stride_of(C), stride_of(Nn)
+    ie = np.zeros(28 * es, np.int32)
+    lib.icfh_pack_elems(csr.h, ptr(np.ascontiguousarray(col.elastic, np.int32)), ptr(alt), T, es, ptr(ie))
+    icz = np.zeros(58 * cs, np.int32)
+    lib.icfh_pack_czs(csr.h, ptr(np.ascontiguousarray(col.czs, np.int32)), ptr(alt), C, cs, ptr(icz))
+    dn = np.zeros(18 * ns)
+    rng = np.random.default_rng(seed)
+    for k in range(3):
+        dn[k * ns:k * ns + Nn] = col.x0[:, k]
+        dn[(3 + k) * ns:(3 + k) * ns + Nn] = rng.normal(size=Nn) * 1e-7
+    A = np.zeros(9 * csr.nnz)
+    b = np.zeros(3 * csr.N)
+    orc.set_constants(consts)
+    orc.elem_force(ie, dn, 1e-4, A, b, T, es, ns)
+    orc.cz_force(np.zeros(16 * cs), icz, dn, A, b, 1e-4, C, cs, ns)
+    return csr.rows.copy(), csr.cols.copy(), A, b
+
+
+def test_cube_against_sparse_lu(orc, lib, specimen_consts):
+    """A compact 3D system (nested dissection, large supernodes) against scipy's sparse LU."""
+    import scipy.sparse.linalg as sla
+    rows, cols, A, b = _cube_system(orc, lib, specimen_consts, 7, 3)
+    r, x = _solve(lib, rows, cols, A, b)
+    assert r == 0
+    M = newton.upper_bcsr_to_csc(rows, cols, A).tocsc()
+    xr = sla.spsolve(M, b)
+    assert np.linalg.norm(M @ x - b) <= 1e-13 * np.linalg.norm(b)
+    assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()
+
+
+def test_concurrent_calls_and_thread_counts_agree(orc, lib, specimen_consts, beam_system):
+    """Several threads solving different systems at once (each pattern's analysis shared from
+    the cache) give the sequential results bit for bit, with and without the solve's own
+    threads; a negative thread count is refused."""
+    from concurrent.futures import ThreadPoolExecutor
+    systems = [beam_system] + [_cube_system(orc, lib, specimen_consts, c, s) for c, s in ((4, 1), (5, 2), (6, 4))]
+    assert lib.icfh_solve_threads(-1) == -1
+    try:
+        lib.icfh_solve_threads(0)
+        seq = [_solve(lib, *s)[1] for s in systems]
+        lib.icfh_solve_threads(1)
+        with ThreadPoolExecutor(4) as pool:
+            par = list(pool.map(lambda s: _solve(lib, *s), systems * 3))
+    finally:
+        lib.icfh_solve_threads(0)
+    for k, (r, x) in enumerate(par):
+        assert r == 0
+        assert np.array_equal(x, seq[k % len(systems)])
+
+
+def test_batched_solver_per_specimen(orc, lib, specimen_consts, beam_system):
+    """BatchedSolver over a block-diagonal matrix of three systems gives each system's own
+    SolveDouble3 result bit for bit, leaves inactive specimens at zero, and refuses a matrix
+    that is not block-diagonal over the given rows."""
+    parts = [beam_system] + [_cube_system(orc, lib, specimen_consts, c, s) for c, s in ((4, 1), (5, 2))]
+    rows, cols, vals, rhs, row_off = [0], [], [], [], [0]
+    for r, c, a, b in parts:
+        rows.extend((np.asarray(r[1:]) + rows[-1]).tolist())
+        cols.extend((np.asarray(c) + row_off[-1]).tolist())
+        vals.append(a)
+        rhs.append(b)
+        row_off.append(row_off[-1] + len(r) - 1)
+    rows, cols = np.array(rows), np.array(cols)
+    vals, rhs = np.concatenate(vals), np.concatenate(rhs)
+    bs = newton.BatchedSolver(rows, cols, row_off, threads=2)
+    try:
+        dx = bs.solve(vals, rhs)
+        for k, (r, c, a, b) in enumerate(parts):
+            assert np.array_equal(dx[3 * row_off[k]:3 * row_off[k + 1]], _solve(lib, r, c, a, b)[1])
+        dx2 = bs.solve(vals, rhs, active=np.array([True, False, True]))
+        assert np.array_equal(dx2[3 * row_off[1]:3 * row_off[2]], np.zeros(3 * (row_off[2] - row_off[1])))
+        assert np.array_equal(dx2[3 * row_off[2]:], dx[3 * row_off[2]:])
+    finally:
+        bs.close()
+    with pytest.raises(ValueError):
+        newton.BatchedSolver(rows, cols, [0, row_off[1] - 1, row_off[-1]])
