@@ -367,7 +367,8 @@ def _rocprof_symbol(name):
     if base == "k_contact_impact":
         return f"{base}<{'float' if args[0] == 'mixed' else 'double'}>"
     if base == "k_phase1":
-        return f"k_phase1_fused<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}>"
+        # the third argument (nontemporal staging stores) follows from the workload's size
+        return f"k_phase1_fused<{b(args[0] == 'mixed')}, {b(args[1] == 'iso')}"
     if base in ("k_vals_force", "k_vals_force_rhs"):
         mixed = args[0] == "mixed"
         return f"{base}_fused<{b(mixed)}, {b(args[1] == 'iso')}, {'float' if mixed else 'double'}"

@@ -81,7 +81,7 @@ _SIGS = {
     "icf_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t]),
     "icf_probe_valu": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "icf_set_shape": (c_int, [c_void_p, c_int, c_int]),
-    "icf_get_shape": (c_int, [c_void_p, c_int, P_int, P_int, ctypes.POINTER(ctypes.c_float)]),
+    "icf_get_shape": (c_int, [c_void_p, c_int, P_int, P_int, c_void_p]),
     "icf_set_blocking": (c_int, [c_void_p, c_int]),
     "icf_cz_counts": (c_int, [c_void_p, P_int, P_int]),
     "icf_graph_begin": (c_int, [c_void_p]),

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, 
 }
 
 // The k-DOPs and node ids in cell order: sorted position j holds tet sidx[j], so that the tets
-// of one cell, and of neighbouring cells, are adjacent in memory for k_bp_pairs9.  Per position:
+// of one cell, and of neighbouring cells, are adjacent in memory for k_bp_cellpairs.  Per position:
 // the k-DOP as (lo, hi) pairs per direction (skd, 24 doubles), the node ids (snode), and the
 // axis-aligned box in FP32 rounded outward, lo down and hi up (box): a prefilter that never
 // rejects a pair whose FP64 boxes overlap.
@@ -109,81 +109,185 @@ __global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ lo
     box[2 * (size_t)j + 1] = make_float4(__double2float_ru(h[0]), __double2float_ru(h[1]), __double2float_ru(h[2]), 0.f);
 }
 
-// Visit each overlapping pair once, from the partner with the lower sorted position, over the
-// cell-ordered copies, with nine lanes per sorted position, one per neighbouring row of cells
-// (item i: position i / 9, row i % 9), so that a tet's rows are scanned side by side.  The 27
-// neighbouring cells are 9 rows of three: a row (x - 1 .. x + 1; x is the key's low field) is
-// one key range, so it takes one binary search, and only positions after the lane's own are
-// visited.  A candidate passes the outward-rounded FP32 box prefilter, then (when asked) the
-// shared-node exclusion, which removes most neighbours of a conforming mesh before the FP64
-// 12-direction test (the predicate of the reference's k-DOP overlap).  Keys are
-// (min << 32 | max) of the original indices.  Pairs are appended in any order (the caller sorts the keys): the
-// lanes of a wave that find a pair in the same iteration take consecutive places from one
-// atomicAdd on *total, and keys beyond cap are only counted (the caller then grows the
-// buffer and searches again).
-__global__ __launch_bounds__(256) void k_bp_pairs9(const double2 *__restrict__ skd, int nT,
-                                                   const unsigned long long *__restrict__ skey,
-                                                   const int *__restrict__ sidx, const int4 *__restrict__ snode,
-                                                   const float4 *__restrict__ box, int exclude_shared,
-                                                   const double *__restrict__ red,
-                                                   unsigned long long *__restrict__ total,
-                                                   unsigned long long *__restrict__ out, long long cap)
+// Cells of the sorted positions: flag[j] = 1 where a new cell key starts.
+__global__ __launch_bounds__(256) void k_bp_cellflag(const unsigned long long *__restrict__ skey, int nT,
+                                                     int *__restrict__ flag)
 {
-    const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= 9LL * nT) return;
-    const int ja = (int)(item / 9), r = (int)(item % 9);
-    const int lane = __lane_id();
-    const double cs = bp_cell_size(red);
-    int ca[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) ca[k] = (int)floor((skd[(size_t)ja * BP_DIRS + k].x - red[1 + k]) / cs);
-    const int cy = ca[1] + (r % 3) - 1, cz = ca[2] + (r / 3) - 1;
-    if (cy < 0 || cz < 0) return;
-    const unsigned long long row = ((unsigned long long)cy << BP_CELL_BITS) | ((unsigned long long)cz << (2 * BP_CELL_BITS));
-    const unsigned long long k0 = row | (unsigned long long)(ca[0] > 0 ? ca[0] - 1 : 0);
-    const unsigned long long k1 = row | (unsigned long long)(ca[0] + 1);
-    int lo_i = 0, hi_i = nT;
-    while (lo_i < hi_i) {
-        const int mid = (lo_i + hi_i) >> 1;
-        if (skey[mid] < k0) lo_i = mid + 1;
-        else hi_i = mid;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nT) flag[j] = (j == 0 || skey[j] != skey[j - 1]) ? 1 : 0;
+}
+
+// From the exclusive scan of the flags: each cell's key and first position; cstart[ncells] = nT.
+__global__ __launch_bounds__(256) void k_bp_cells(const unsigned long long *__restrict__ skey, int nT,
+                                                  const int *__restrict__ flag, const int *__restrict__ cidx,
+                                                  unsigned long long *__restrict__ ckey, int *__restrict__ cstart)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nT) return;
+    if (flag[j]) {
+        ckey[cidx[j]] = skey[j];
+        cstart[cidx[j]] = j;
     }
-    const float4 bl = box[2 * (size_t)ja], bh = box[2 * (size_t)ja + 1];
-    const int4 na = snode[ja];
-    const unsigned a = (unsigned)sidx[ja];
-    for (int j = lo_i > ja ? lo_i : ja + 1; j < nT && skey[j] <= k1; j++) {
-        bool hit = true;
-        const float4 ol = box[2 * (size_t)j], oh = box[2 * (size_t)j + 1];
-        if (bl.x > oh.x || bl.y > oh.y || bl.z > oh.z || bh.x < ol.x || bh.y < ol.y || bh.z < ol.z) hit = false;
-        if (hit && exclude_shared) {
-            const int4 nb = snode[j];
-            const int m[4] = {nb.x, nb.y, nb.z, nb.w};
-#pragma unroll
-            for (int j2 = 0; j2 < 4; j2++) hit = hit && m[j2] != na.x && m[j2] != na.y && m[j2] != na.z && m[j2] != na.w;
-        }
-        if (hit) {
-#pragma unroll
-            for (int d = 0; d < BP_DIRS; d++) {
-                const double2 u = skd[(size_t)ja * BP_DIRS + d], v = skd[(size_t)j * BP_DIRS + d];
-                hit = hit && !(u.x > v.y) && !(u.y < v.x);
+    if (j == nT - 1) cstart[cidx[j] + flag[j]] = nT;
+}
+
+// One workgroup per cell: every overlapping pair with both tets in the cell, or one in the cell
+// and the other in one of the 13 neighbouring cells whose key is larger (each pair of cells
+// visited once), over the cell-ordered copies.  The candidates (the cell itself and the 13
+// cells, concatenated) and the cell's own tets are brought into LDS, BP_C and BP_A at a time,
+// with coalesced loads; the lanes then take (own, candidate) pairs in turn, BP_T at a time.  A
+// pair passes the outward-rounded FP32 box prefilter, then (when asked) the shared-node
+// exclusion, which removes most neighbours of a conforming mesh before the FP64 12-direction
+// test (the predicate of the reference's k-DOP overlap).  Pairs collect in LDS as (min << 32
+// | max) keys of the original indices and leave in batches: one atomicAdd on *total takes
+// the batch's places in out (keys beyond cap are only counted; the caller then grows the
+// buffer and searches again).  The order of the keys is the caller's to fix (it sorts them).
+constexpr int BP_A = 128, BP_C = 256, BP_T = 256, BP_BUF = 1024;
+__global__ __launch_bounds__(BP_T) void k_bp_cellpairs(const double2 *__restrict__ skd, int nT,
+                                                       const int *__restrict__ sidx, const int4 *__restrict__ snode,
+                                                       const float4 *__restrict__ box, int exclude_shared,
+                                                       const unsigned long long *__restrict__ ckey,
+                                                       const int *__restrict__ cstart, int ncells,
+                                                       unsigned long long *__restrict__ total,
+                                                       unsigned long long *__restrict__ out, long long cap)
+{
+    __shared__ float4 abox[2 * BP_A], cbox[2 * BP_C];
+    __shared__ int4 anode[BP_A], cnode[BP_C];
+    __shared__ int cj[BP_C];
+    __shared__ int nb_lo[14], nb_hi[14], nb_pre[15];
+    __shared__ unsigned long long pbuf[BP_BUF];
+    __shared__ int wave_cnt[BP_T / 64];
+    __shared__ int pcount;
+    __shared__ unsigned long long flush_base;
+    const int c = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    const int s0 = cstart[c], s1 = cstart[c + 1];
+    if (tid < 14) {
+        // neighbour t: t = 0 the cell itself, 1-13 the offsets (dx, dy, dz) whose key is larger
+        int lo = 0, hi = 0;
+        const unsigned long long k = ckey[c];
+        const long long cx = (long long)(k & ((1ULL << BP_CELL_BITS) - 1));
+        const long long cy = (long long)((k >> BP_CELL_BITS) & ((1ULL << BP_CELL_BITS) - 1));
+        const long long cz = (long long)(k >> (2 * BP_CELL_BITS));
+        if (tid == 0) {
+            lo = s0;
+            hi = s1;
+        } else {
+            const int q = tid + 13;            // offsets 14..26 of the 27 in (dz, dy, dx) order
+            const long long nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
+            if (nx >= 0 && ny >= 0 && nz >= 0) {
+                const unsigned long long nk = (unsigned long long)nx | ((unsigned long long)ny << BP_CELL_BITS) |
+                                              ((unsigned long long)nz << (2 * BP_CELL_BITS));
+                int l = 0, h = ncells;
+                while (l < h) {
+                    const int m = (l + h) >> 1;
+                    if (ckey[m] < nk) l = m + 1;
+                    else h = m;
+                }
+                if (l < ncells && ckey[l] == nk) {
+                    lo = cstart[l];
+                    hi = cstart[l + 1];
+                }
             }
         }
-        const unsigned long long m = __ballot(hit);
-        if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(total, (unsigned long long)__popcll(m));
-            const unsigned lo32 = __shfl((unsigned)(base & 0xffffffffULL), leader, 64);
-            const unsigned hi32 = __shfl((unsigned)(base >> 32), leader, 64);
-            if (hit) {
-                const unsigned long long at = (((unsigned long long)hi32 << 32) | lo32) +
-                                              (unsigned long long)__popcll(m & ((1ULL << lane) - 1));
-                const unsigned b = (unsigned)sidx[j];
-                const unsigned p = a < b ? a : b, q = a < b ? b : a;
-                if (at < (unsigned long long)cap) out[at] = ((unsigned long long)p << 32) | q;
+        nb_lo[tid] = lo;
+        nb_hi[tid] = hi;
+    }
+    if (tid == 0) pcount = 0;
+    __syncthreads();
+    if (tid == 0) {
+        nb_pre[0] = 0;
+        for (int t = 0; t < 14; t++) nb_pre[t + 1] = nb_pre[t] + nb_hi[t] - nb_lo[t];
+    }
+    __syncthreads();
+    const int K = nb_pre[14];
+    // move the collected keys out: one atomicAdd for the batch, then a coalesced copy
+    auto flush = [&]() {
+        __syncthreads();
+        const int n = pcount;
+        if (tid == 0) flush_base = n ? atomicAdd(total, (unsigned long long)n) : 0ULL;
+        __syncthreads();
+        const unsigned long long base = flush_base;
+        for (int i = tid; i < n; i += BP_T)
+            if (base + i < (unsigned long long)cap) out[base + i] = pbuf[i];
+        __syncthreads();
+        if (tid == 0) pcount = 0;
+        __syncthreads();
+    };
+    for (int a0 = s0; a0 < s1; a0 += BP_A) {
+        const int na = min(BP_A, s1 - a0);
+        __syncthreads();
+        for (int i = tid; i < na; i += BP_T) {
+            abox[2 * i] = box[2 * (size_t)(a0 + i)];
+            abox[2 * i + 1] = box[2 * (size_t)(a0 + i) + 1];
+            anode[i] = snode[a0 + i];
+        }
+        for (int k0 = 0; k0 < K; k0 += BP_C) {
+            const int nk = min(BP_C, K - k0);
+            __syncthreads();
+            for (int i = tid; i < nk; i += BP_T) {
+                const int k = k0 + i;
+                int t = 0;
+                while (nb_pre[t + 1] <= k) t++;
+                const int jb = nb_lo[t] + (k - nb_pre[t]);
+                cj[i] = t == 0 ? -1 - jb : jb;   // the cell's own tets: pairs with larger positions only
+                cbox[2 * i] = box[2 * (size_t)jb];
+                cbox[2 * i + 1] = box[2 * (size_t)jb + 1];
+                cnode[i] = snode[jb];
+            }
+            __syncthreads();
+            const int npairs = na * nk;
+            for (int p0 = 0; p0 < npairs; p0 += BP_T) {
+                const int p = p0 + tid;
+                bool hit = false;
+                int ja = 0, jb = 0;
+                if (p < npairs) {
+                    const int ia = p / nk, ik = p % nk;
+                    ja = a0 + ia;
+                    const int code = cj[ik];
+                    jb = code < 0 ? -1 - code : code;
+                    if (code >= 0 || jb > ja) {
+                        const float4 bl = abox[2 * ia], bh = abox[2 * ia + 1];
+                        const float4 ol = cbox[2 * ik], oh = cbox[2 * ik + 1];
+                        hit = !(bl.x > oh.x || bl.y > oh.y || bl.z > oh.z || bh.x < ol.x || bh.y < ol.y ||
+                                bh.z < ol.z);
+                        if (hit && exclude_shared) {
+                            const int4 na4 = anode[ia], nb4 = cnode[ik];
+                            const int m[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+#pragma unroll
+                            for (int j2 = 0; j2 < 4; j2++)
+                                hit = hit && m[j2] != na4.x && m[j2] != na4.y && m[j2] != na4.z && m[j2] != na4.w;
+                        }
+                        if (hit) {
+#pragma unroll
+                            for (int d = 0; d < BP_DIRS; d++) {
+                                const double2 u = skd[(size_t)ja * BP_DIRS + d], v = skd[(size_t)jb * BP_DIRS + d];
+                                hit = hit && !(u.x > v.y) && !(u.y < v.x);
+                            }
+                        }
+                    }
+                }
+                // this round's places in the batch: lane prefix within the wave, then the waves
+                const unsigned long long m = __ballot(hit);
+                if (lane == 0) wave_cnt[wave] = __popcll(m);
+                __syncthreads();
+                int at = pcount, round = 0;
+                for (int w = 0; w < BP_T / 64; w++) {
+                    if (w < wave) at += wave_cnt[w];
+                    round += wave_cnt[w];
+                }
+                at += __popcll(m & ((1ULL << lane) - 1));
+                if (hit) {
+                    const unsigned a = (unsigned)sidx[ja], b = (unsigned)sidx[jb];
+                    pbuf[at] = ((unsigned long long)(a < b ? a : b) << 32) | (a < b ? b : a);
+                }
+                __syncthreads();
+                if (tid == 0) pcount += round;
+                __syncthreads();
+                if (pcount > BP_BUF - BP_T) flush();
             }
         }
     }
+    flush();
 }
 
 __global__ __launch_bounds__(256) void k_bp_unpack(const unsigned long long *__restrict__ keys, long long n,

@@ -120,7 +120,7 @@ __device__ inline void apply_K0(const double g[12], double V, const double y[12]
 // Phase 1 for tets.  rhs (12 per tet) = gravity - f - dampK * Df vn - V M (dampM vn + an)
 // (element_kernels2.h:228-251), written FP64 to st_rhs; tangent record to st_tan
 // (double, or float when MIXED).
-template <bool MIXED, bool ISO>
+template <bool MIXED, bool ISO, bool NT>
 __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restrict__ ie,
         const double *__restrict__ dn, int n, int tp, int es, int ns, double *__restrict__ st_rhs,
         void *__restrict__ st_tan)
@@ -218,12 +218,17 @@ __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restr
             if (r % 3 == 2) rhs[r] += gz;
         }
     }
-    // node i's part at plane i (tet_rhs_at), so that each store is one coalesced run
+    // node i's part at plane i (tet_rhs_at), so that each store is one coalesced run; NT:
+    // nontemporal stores, when the staged records are too large to stay in the caches until
+    // phase 2 reads them, so that they do not evict the node fields phase 1 gathers
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         double *o = st_rhs + ((size_t)i * tp + u) * 3;
 #pragma unroll
-        for (int k = 0; k < 3; k++) o[k] = rhs[3 * i + k];
+        for (int k = 0; k < 3; k++) {
+            if (NT) __builtin_nontemporal_store(rhs[3 * i + k], &o[k]);
+            else o[k] = rhs[3 * i + k];
+        }
     }
 
     if (ISO) {
@@ -239,8 +244,18 @@ __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restr
         if (MIXED) {
             float4 *t = (float4 *)st_tan;
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                t[(size_t)i * tp + u] = make_float4((float)gt[3 * i], (float)gt[3 * i + 1], (float)gt[3 * i + 2], (float)V);
+            for (int i = 0; i < 4; i++) {
+                const float4 v = make_float4((float)gt[3 * i], (float)gt[3 * i + 1], (float)gt[3 * i + 2], (float)V);
+                float *tf = (float *)&t[(size_t)i * tp + u];
+                if (NT) {
+                    __builtin_nontemporal_store(v.x, tf);
+                    __builtin_nontemporal_store(v.y, tf + 1);
+                    __builtin_nontemporal_store(v.z, tf + 2);
+                    __builtin_nontemporal_store(v.w, tf + 3);
+                } else {
+                    t[(size_t)i * tp + u] = v;
+                }
+            }
         } else {
             double2 *t = (double2 *)st_tan;
 #pragma unroll
@@ -270,12 +285,12 @@ __device__ __forceinline__ void tet_phase1(int bid_, int nb_, const int *__restr
     }
 }
 
-template <bool MIXED, bool ISO>
+template <bool MIXED, bool ISO, bool NT>
 __global__ __launch_bounds__(256) void k_tet_phase1(const int *__restrict__ ie, const double *__restrict__ dn,
                                                     int n, int tp, int es, int ns, double *__restrict__ st_rhs,
                                                     void *__restrict__ st_tan)
 {
-    tet_phase1<MIXED, ISO>(blockIdx.x, gridDim.x, ie, dn, n, tp, es, ns, st_rhs, st_tan);
+    tet_phase1<MIXED, ISO, NT>(blockIdx.x, gridDim.x, ie, dn, n, tp, es, ns, st_rhs, st_tan);
 }
 
 // Phase 2: LHS block (i,j) of an isotropic tet record.  LHS = stiff * R K R^T + mcoef V M

@@ -34,9 +34,22 @@ __device__ inline bool interleave2(int w, int na, int nb, int &local)
     return false;
 }
 
+// interleave2 when `ilv`, else phase A's workgroups first, then phase B's (same grid; B's
+// padding workgroups get local >= nb).
+__device__ inline bool split2(int w, int na, int nb, bool ilv, int &local)
+{
+    if (ilv) return interleave2(w, na, nb, local);
+    if (w < na) {
+        local = w;
+        return false;
+    }
+    local = w - na;
+    return true;
+}
+
 // Phase 1 of the fused call: tets (nbt workgroups) interleaved with CZs (nbc).  Workgroup 0 also
 // clears the two CZ counters that the last launch of the call adds up.
-template <bool MIXED, bool ISO>
+template <bool MIXED, bool ISO, bool NT>
 __global__ __launch_bounds__(256) void k_phase1_fused(int nbt, int nbc, const int *__restrict__ ie,
                                                       const double *__restrict__ dn, int nt, int tp, int es, int ns,
                                                       double *__restrict__ tet_rhs, void *__restrict__ tet_tan,
@@ -47,7 +60,7 @@ __global__ __launch_bounds__(256) void k_phase1_fused(int nbt, int nbc, const in
     if (blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = 0;
     int local;
     if (!interleave2(blockIdx.x, nbt, nbc, local))
-        tet_phase1<MIXED, ISO>(local, nbt, ie, dn, nt, tp, es, ns, tet_rhs, tet_tan);
+        tet_phase1<MIXED, ISO, NT>(local, nbt, ie, dn, nt, tp, es, ns, tet_rhs, tet_tan);
     else if (local < nbc)
         cz_phase1<MIXED>(local, nbc, dcz, icz, dn, nc, cs, ns, cz_rhs, cz_tan);
 }
@@ -63,10 +76,10 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *_
                                                            const uint32_t *__restrict__ fcodes,
                                                            const int *__restrict__ fkeys, int nfkeys,
                                                            double *__restrict__ F, int ns, Stage sv,
-                                                           double *__restrict__ tsum)
+                                                           double *__restrict__ tsum, int ilv)
 {
     int local;
-    if (!interleave2(blockIdx.x, nbf, nchunks, local))
+    if (!split2(blockIdx.x, nbf, nchunks, ilv != 0, local))
         reduce_vec3<true, true>(local, nbf, foff, fcodes, fkeys, nfkeys, F, ns, 1, sv, tsum);
     else if (local < nchunks)
         reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(local, nchunks, off, codes, nullptr, chunks, nchunks, A, sv);
@@ -81,13 +94,14 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc,
         const int *__restrict__ foff, const uint32_t *__restrict__ fcodes, const int *__restrict__ fkeys, int nfkeys,
         double *__restrict__ F, int ns, const int *__restrict__ node_row, const int *__restrict__ roff,
         const uint32_t *__restrict__ rcodes, const int *__restrict__ cz_begin, double *__restrict__ b,
-        const int *__restrict__ damaged, const int *__restrict__ failed, int nc, int *__restrict__ counts, Stage sv)
+        const int *__restrict__ damaged, const int *__restrict__ failed, int nc, int *__restrict__ counts, Stage sv,
+        int ilv)
 {
     const int w = blockIdx.x;
     int local;
     if (w < nbc)
         count_flags_n<KEYS>(w, nbc, damaged, failed, nc, counts);
-    else if (!interleave2(w - nbc, nbf, nchunks, local))
+    else if (!split2(w - nbc, nbf, nchunks, ilv != 0, local))
         reduce_force_rhs(local, nbf, foff, fcodes, fkeys, nfkeys, F, ns, node_row, roff, rcodes, cz_begin, b, sv);
     else if (local < nchunks)
         reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(local, nchunks, off, codes, nullptr, chunks, nchunks, A, sv);

@@ -177,14 +177,19 @@ hipError_t scratch_for_device(int device, void **out)
 
 }  // namespace
 
-// Launch-shape roles besides the value reductions (icf_set_shape / icf_get_shape), three
-// candidates each: the workgroup size (64, 128 or 256 lanes) of phase 1 of the fused call, of
-// the narrow phase and of the impacts' phase 1, and the lanes per (node, tet) tuple of the
-// closest face (4, 8 or 16).  Every candidate gives the same results (each unit's work does
-// not depend on the shape).
-constexpr int NSHAPE = 4, NSHAPE_CAND = 3;
-constexpr int shape_value(int role, int c) { return role == ICF_SHAPE_CLOSEST ? 4 << c : 64 << c; }
-constexpr int SHAPE_DEFAULT[NSHAPE] = {2, 2, 1, 2};   // 256, 256, 8 lanes, 256
+// Launch-shape roles besides the value reductions (icf_set_shape / icf_get_shape): the
+// workgroup size (64, 128 or 256 lanes) of phase 1 of the fused call, of the narrow phase and
+// of the impacts' phase 1, the lanes per (node, tet) tuple of the closest face (4, 8 or 16),
+// and the order of the fused value launch's two phases (0: node forces first, then the BCSR
+// values; 1: interleaved in rounds of eight workgroups).  Every candidate gives the same
+// results (each unit's and key's work does not depend on the shape).
+constexpr int NSHAPE = 5, NSHAPE_CAND = 3;
+constexpr int shape_value(int role, int c)
+{
+    return role == ICF_SHAPE_CLOSEST ? 4 << c : role == ICF_SHAPE_ORDER ? c : 64 << c;
+}
+constexpr int shape_ncand(int role) { return role == ICF_SHAPE_ORDER ? 2 : 3; }
+constexpr int SHAPE_DEFAULT[NSHAPE] = {2, 2, 1, 2, 1};   // 256, 256, 8 lanes, 256, interleaved
 
 struct icf_ctx {
     int id = 0;
@@ -211,6 +216,7 @@ struct icf_ctx {
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
     DevBuf pat_a, pat_b;           // pattern build / broad phase / NL2set: 64-bit keys
     DevBuf bp_f, bp_k, bp_i, bp_l;   // broad phase and NL2set scratch (doubles, keys, ints, int64)
+    DevBuf bp_c;                     // broad phase cells (flags, cell indices, cell starts)
     long long bp_cap = 0;            // pair keys the broad phase's buffer holds (grows with use)
     DevBuf tsum;                     // per-node tet sums of the fused call
     DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
@@ -848,6 +854,16 @@ template <typename F> struct OnLane {
 // Plane stride of the tet staging records for n tets.
 int tet_plane(long long n) { return (int)((std::max(n, 1LL) + 63) / 64 * 64); }
 
+// Staged tet records of more than this many bytes are written with nontemporal stores: they
+// would not stay in the caches until phase 2 reads them anyway (measured: 3.5M tets faster
+// with them, 384k tets faster without).
+constexpr size_t STREAM_STAGING_BYTES = 128u << 20;
+bool stream_tet_staging(const icf_ctx *ctx, int n)
+{
+    const size_t rec = (ctx->hc.iso ? TET_ISO_REC : TET_GEN_REC) * (ctx->precision == ICF_MIXED ? 4 : 8);
+    return (size_t)tet_plane(n) * (rec + sizeof(double) * TET_RHS_REC) > STREAM_STAGING_BYTES;
+}
+
 int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int ns, int reserve = 0)
 {
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
@@ -860,7 +876,12 @@ int stage_tets(icf_ctx *ctx, const int *ie, const double *dn, int n, int es, int
     dim3 g(blocks_for(n, 256)), b(256);
     double *sr = (double *)ctx->tet_rhs.p;
     void *st = ctx->tet_tan.p;
-#define ICF_T1(MX, IS, NAME) LAUNCH(ctx, NAME, (k_tet_phase1<MX, IS>), g, b, ie, dn, n, tp, es, ns, sr, st)
+    const bool so = stream_tet_staging(ctx, n);
+#define ICF_T1(MX, IS, NAME)                                                                                   \
+    do {                                                                                                      \
+        if (so) LAUNCH(ctx, NAME, (k_tet_phase1<MX, IS, true>), g, b, ie, dn, n, tp, es, ns, sr, st);         \
+        else LAUNCH(ctx, NAME, (k_tet_phase1<MX, IS, false>), g, b, ie, dn, n, tp, es, ns, sr, st);           \
+    } while (0)
     if (mixed && iso) ICF_T1(true, true, "k_tet_phase1<mixed,iso>");
     else if (mixed) ICF_T1(true, false, "k_tet_phase1<mixed,gen>");
     else if (iso) ICF_T1(false, true, "k_tet_phase1<fp64,iso>");
@@ -989,9 +1010,16 @@ void phase1_fused(icf_ctx *ctx, const int *ie, const double *dn, int nt, int es,
     double *tr = (double *)ctx->tet_rhs.p, *cr = (double *)ctx->cz_rhs.p;
     void *tt = ctx->tet_tan.p, *ct = ctx->cz_tan.p;
     const int tp = ctx->tplane = tet_plane(nt);
+    const bool so = stream_tet_staging(ctx, nt);
 #define ICF_P1(MX, IS, NAME)                                                                                      \
-    LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS>), g, b, nbt, nbc, ie, dn, nt, tp, es, ns, tr, tt, dcz, icz, nc, cs, cr, \
-           ct, counts)
+    do {                                                                                                          \
+        if (so)                                                                                                   \
+            LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS, true>), g, b, nbt, nbc, ie, dn, nt, tp, es, ns, tr, tt, dcz, \
+                   icz, nc, cs, cr, ct, counts);                                                                  \
+        else                                                                                                      \
+            LAUNCH(ctx, NAME, (k_phase1_fused<MX, IS, false>), g, b, nbt, nbc, ie, dn, nt, tp, es, ns, tr, tt, dcz, \
+                   icz, nc, cs, cr, ct, counts);                                                                  \
+    } while (0)
     if (mixed && iso) ICF_P1(true, true, "k_phase1<mixed,iso>");
     else if (mixed) ICF_P1(true, false, "k_phase1<mixed,gen>");
     else if (iso) ICF_P1(false, true, "k_phase1<fp64,iso>");
@@ -1029,6 +1057,7 @@ bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap 
     const int k = mixed ? 0 : 1;
     double *ts = (double *)ctx->tsum.p;
     const int c = tune_begin(ctx, false, !mixed, mv->nlaunch);
+    const int ilv = shape_begin(ctx, ICF_SHAPE_ORDER, mv->nlaunch);
     with_cand(c, [&](auto kc, auto sc) {
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
@@ -1036,13 +1065,14 @@ bool vals_force_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const SlotMap 
         const dim3 g(interleave2_grid(nbf, nch)), b(K);   // not empty: at least one map has keys
 #define ICF_VF(MX, IS, L, NAME)                                                                                   \
     LAUNCH(ctx, NAME, (k_vals_force_fused<MX, IS, L, K, S>), g, b, nbf, mv->off, mv->codes, mv->chunks[c][k], nch, A, \
-           mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts)
+           mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, st, ts, ilv)
         if (mixed && iso) ICF_VF(true, true, float, "k_vals_force<mixed,iso>");
         else if (mixed) ICF_VF(true, false, float, "k_vals_force<mixed,gen>");
         else if (iso) ICF_VF(false, true, double, "k_vals_force<fp64,iso>");
         else ICF_VF(false, false, double, "k_vals_force<fp64,gen>");
 #undef ICF_VF
     });
+    shape_end(ctx, ICF_SHAPE_ORDER);
     tune_end(ctx, false, !mixed);
     return true;
 }
@@ -1059,6 +1089,7 @@ bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const Slot
     const bool mixed = ctx->precision == ICF_MIXED, iso = ctx->hc.iso != 0;
     const int k = mixed ? 0 : 1;
     const int c = tune_begin(ctx, false, !mixed, mv->nlaunch);
+    const int ilv = shape_begin(ctx, ICF_SHAPE_ORDER, mv->nlaunch);
     with_cand(c, [&](auto kc, auto sc) {
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
@@ -1068,13 +1099,14 @@ bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const Slot
 #define ICF_VFR(MX, IS, L, NAME)                                                                                   \
     LAUNCH(ctx, NAME, (k_vals_force_rhs_fused<MX, IS, L, K, S>), g, bl, nbf, nbc, mv->off, mv->codes, mv->chunks[c][k], \
            nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, mr->node_row, mr->off, mr->codes, mr->cz_begin, \
-           b, dmg, fld, nCZs, counts, st)
+           b, dmg, fld, nCZs, counts, st, ilv)
         if (mixed && iso) ICF_VFR(true, true, float, "k_vals_force_rhs<mixed,iso>");
         else if (mixed) ICF_VFR(true, false, float, "k_vals_force_rhs<mixed,gen>");
         else if (iso) ICF_VFR(false, true, double, "k_vals_force_rhs<fp64,iso>");
         else ICF_VFR(false, false, double, "k_vals_force_rhs<fp64,gen>");
 #undef ICF_VFR
     });
+    shape_end(ctx, ICF_SHAPE_ORDER);
     tune_end(ctx, false, !mixed);
     return true;
 }
@@ -1201,7 +1233,7 @@ int icf_create(int device, icf_ctx **out)
     icf_ctx *ctx = new icf_ctx();
     for (auto &row : ctx->tune) row[1].sel = CAND_DEFAULT_FP64;   // FP64 staging roles
     for (int r = 0; r < NSHAPE; r++) {
-        ctx->shape[r].ncand = NSHAPE_CAND;
+        ctx->shape[r].ncand = shape_ncand(r);
         ctx->shape[r].sel = SHAPE_DEFAULT[r];
     }
     ctx->id = g_next_ctx_id++;
@@ -1237,7 +1269,7 @@ int icf_destroy(icf_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
-                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l,
+                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l, &ctx->bp_c,
                       &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids, &ctx->batch.d};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1664,10 +1696,11 @@ int icf_set_shape(icf_ctx *ctx, int role, int value)
     if (!ctx) return ICF_E_INVALID;
     if (role < 0 || role >= NSHAPE) return fail(ctx, ICF_E_INVALID, "icf_set_shape: unknown role");
     int c = 0;
-    while (c < NSHAPE_CAND && shape_value(role, c) != value) c++;
-    if (c == NSHAPE_CAND)
+    while (c < shape_ncand(role) && shape_value(role, c) != value) c++;
+    if (c == shape_ncand(role))
         return fail(ctx, ICF_E_INVALID, role == ICF_SHAPE_CLOSEST ? "icf_set_shape: lanes per tuple must be 4, 8 or 16"
-                                                                   : "icf_set_shape: workgroup size must be 64, 128 or 256");
+                                        : role == ICF_SHAPE_ORDER ? "icf_set_shape: the order must be 0 or 1"
+                                                                  : "icf_set_shape: workgroup size must be 64, 128 or 256");
     icf_ctx::Tune &t = ctx->shape[role];
     if (t.active && !ctx->capturing) tune_fold(ctx, t, 4 + role);
     t.active = false;
@@ -1686,7 +1719,7 @@ int icf_get_shape(icf_ctx *ctx, int role, int *value, int *tuning, float *best_m
     if (value) *value = shape_value(role, t.sel);
     if (tuning) *tuning = t.active ? 1 : 0;
     if (best_ms)
-        for (int c = 0; c < NSHAPE_CAND; c++) best_ms[c] = t.best[c];
+        for (int c = 0; c < NSHAPE_CAND; c++) best_ms[c] = c < t.ncand ? t.best[c] : 0.f;
     return 0;
 }
 
@@ -2389,6 +2422,8 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     if (int r = grow(ctx, ctx->bp_k, sizeof(unsigned long long) * 2 * (size_t)nT)) return r;
     if (int r = grow(ctx, ctx->bp_i, sizeof(int) * (6 * (size_t)nT + 4))) return r;
     if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2)) return r;
+    if (int r = grow(ctx, ctx->bp_c, sizeof(int) * (3 * (size_t)nT + 2))) return r;
+    if (nT > (1 << 23)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^23 surface tets");
     double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT, *ext = hi + (size_t)12 * nT,
            *mn = ext + nT, *red = mn + (size_t)3 * nT;
     double2 *skd = (double2 *)(red + 4);                  // 12 double2 = 24 doubles per tet
@@ -2400,12 +2435,13 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
     hipLaunchKernelGGL(k_bp_kdop, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, dn, nd_stride, ie,
                        el_all_stride, nT, lo, hi, ext, mn, d_err, nd_stride);
-    size_t b1 = 0, b2 = 0, b3 = 0;
+    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     HIPCHK(ctx, hipcub::DeviceReduce::Max(nullptr, b1, ext, red, nT, ctx->stream));
     HIPCHK(ctx, hipcub::DeviceReduce::Min(nullptr, b2, mn, red + 1, nT, ctx->stream));
     HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b3, key, skey, idx, sidx, nT, 0, 3 * BP_CELL_BITS,
                                                    ctx->stream));
-    size_t tb = std::max(std::max(b1, b2), b3);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, (int *)nullptr, (int *)nullptr, nT, ctx->stream));
+    size_t tb = std::max(std::max(b1, b2), std::max(b3, b4));
     if (int r = grow(ctx, ctx->scan_tmp, tb)) return r;
     HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b1, ext, red, nT, ctx->stream));
     for (int k = 0; k < 3; k++)
@@ -2415,22 +2451,37 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
                                                    3 * BP_CELL_BITS, ctx->stream));
     hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, sidx, ie,
                        el_all_stride, skd, snode, box);
-    // pairs appended by nine lanes per tet; a second search when they outgrew the buffer
-    auto *tot = (unsigned long long *)ctx->bp_l.p;
+    // cells of the sorted positions (key and first position), then per cell its pairs: counted,
+    // placed by a scan, written
+    int *flag = (int *)ctx->bp_c.p, *cidx = flag + nT, *cstart = cidx + nT;
+    auto *ckey = key;                                      // the unsorted keys are no longer needed
+    long long *cnt = (long long *)ctx->bp_l.p;
+    hipLaunchKernelGGL(k_bp_cellflag, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skey, nT, flag);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b4, flag, cidx, nT, ctx->stream));
+    hipLaunchKernelGGL(k_bp_cells, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skey, nT, flag, cidx, ckey,
+                       cstart);
+    int last[2] = {0, 0}, err = 0;
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(last, cidx + nT - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(last + 1, flag + nT - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
+    const int ncells = last[0] + last[1];
+    // pairs per cell, collected in LDS and appended in batches; a second search when they
+    // outgrew the buffer
+    auto *tot = (unsigned long long *)cnt;
     long long cap = std::max<long long>(ctx->bp_cap, 4LL * nT);
     long long total = 0;
     for (int pass = 0; pass < 2; pass++) {
         if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)cap)) return r;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)tot, 2, 0);
-        hipLaunchKernelGGL(k_bp_pairs9, dim3(blocks_for(9LL * nT, 256)), dim3(256), 0, ctx->stream, skd, nT, skey,
-                           sidx, snode, box, exclude_shared, red, tot, (unsigned long long *)ctx->pat_a.p, cap);
+        hipLaunchKernelGGL(k_bp_cellpairs, dim3(ncells), dim3(BP_T), 0, ctx->stream, skd, nT, sidx, snode, box,
+                           exclude_shared, ckey, cstart, ncells, tot, (unsigned long long *)ctx->pat_a.p, cap);
         unsigned long long got = 0;
-        int err = 0;
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipMemcpyAsync(&got, tot, sizeof(got), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
         total = (long long)got;
         if (total <= cap) break;
         if (pass == 1) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: pair count changed between two searches");

@@ -277,7 +277,9 @@ __device__ __forceinline__ void reduce_vals(int bid_, int nb_, const int *__rest
         return;
     }
     // overwrite mode: the chunk's blocks A[p0 .. p1) are contiguous; write them through LDS
-    // as consecutive doubles (coalesced) instead of one 72-byte block per thread
+    // as consecutive doubles (coalesced) instead of one 72-byte block per thread, as
+    // nontemporal stores: A is written once and not read again here, so it need not take L2
+    // lines from the staged records the other chunks still gather
     double *out = reinterpret_cast<double *>(lds);
     __syncthreads();
     if (live) {
@@ -287,7 +289,7 @@ __device__ __forceinline__ void reduce_vals(int bid_, int nb_, const int *__rest
     __syncthreads();
     const int nout = (p1 - p0) * 9;
     double *dst = A + (size_t)p0 * 9;
-    for (int i = threadIdx.x; i < nout; i += KEYS) dst[i] = out[i];
+    for (int i = threadIdx.x; i < nout; i += KEYS) __builtin_nontemporal_store(out[i], &dst[i]);
 }
 
 template <bool MIXED, bool ISO, bool ACC, typename L, int KEYS, int STAGE>
@@ -318,10 +320,11 @@ __device__ __forceinline__ void reduce_vec3(int bid_, int nb_, const int *__rest
     } else {
         a0 = a1 = a2 = 0.;
     }
+    auto part = [&](int c) { return rhs_of(sv, codes[c]); };
     int c = b;
     for (; c + 4 <= e; c += 4) {          // loads of four contributions in flight, added in order
-        const double *r0 = rhs_of(sv, codes[c]), *r1 = rhs_of(sv, codes[c + 1]);
-        const double *r2 = rhs_of(sv, codes[c + 2]), *r3 = rhs_of(sv, codes[c + 3]);
+        const double *r0 = part(c), *r1 = part(c + 1);
+        const double *r2 = part(c + 2), *r3 = part(c + 3);
         const double x0 = r0[0], y0 = r0[1], z0 = r0[2], x1 = r1[0], y1 = r1[1], z1 = r1[2];
         const double x2 = r2[0], y2 = r2[1], z2 = r2[2], x3 = r3[0], y3 = r3[1], z3 = r3[2];
         a0 += x0; a1 += y0; a2 += z0;
@@ -336,7 +339,7 @@ __device__ __forceinline__ void reduce_vec3(int bid_, int nb_, const int *__rest
         }
     }
     for (; c < e; c++) {
-        const double *r = rhs_of(sv, codes[c]);
+        const double *r = part(c);
         a0 += r[0]; a1 += r[1]; a2 += r[2];
         if (TSUM) { t0 += r[0]; t1 += r[1]; t2 += r[2]; }
     }
@@ -365,10 +368,11 @@ __device__ __forceinline__ void reduce_force_rhs(int bid_, int nb_, const int *_
     const int r = node_row[s];
     double *dst = F + s;
     double a0 = dst[0], a1 = dst[ns], a2 = dst[2 * (size_t)ns], t0 = 0., t1 = 0., t2 = 0.;
+    auto part = [&](int c) { return rhs_of(sv, fcodes[c]); };
     int c = b;
     for (; c + 4 <= e; c += 4) {          // loads of four contributions in flight, added in order
-        const double *r0 = rhs_of(sv, fcodes[c]), *r1 = rhs_of(sv, fcodes[c + 1]);
-        const double *r2 = rhs_of(sv, fcodes[c + 2]), *r3 = rhs_of(sv, fcodes[c + 3]);
+        const double *r0 = part(c), *r1 = part(c + 1);
+        const double *r2 = part(c + 2), *r3 = part(c + 3);
         const double x0 = r0[0], y0 = r0[1], z0 = r0[2], x1 = r1[0], y1 = r1[1], z1 = r1[2];
         const double x2 = r2[0], y2 = r2[1], z2 = r2[2], x3 = r3[0], y3 = r3[1], z3 = r3[2];
         a0 += x0; a1 += y0; a2 += z0;
@@ -381,7 +385,7 @@ __device__ __forceinline__ void reduce_force_rhs(int bid_, int nb_, const int *_
         t0 += x3; t1 += y3; t2 += z3;
     }
     for (; c < e; c++) {
-        const double *q = rhs_of(sv, fcodes[c]);
+        const double *q = part(c);
         a0 += q[0]; a1 += q[1]; a2 += q[2];
         t0 += q[0]; t1 += q[1]; t2 += q[2];
     }

@@ -160,15 +160,15 @@ class GPUFunctionality:
               "icf_set_chunk_keys")
 
     # launch-shape roles besides the value reductions (icf_set_shape / icf_get_shape)
-    SHAPE_ROLES = ("phase1", "narrow", "closest", "impact")
+    SHAPE_ROLES = ("phase1", "narrow", "closest", "impact", "order")
     SHAPE_CANDIDATES = {"phase1": (64, 128, 256), "narrow": (64, 128, 256), "closest": (4, 8, 16),
-                        "impact": (64, 128, 256)}
+                        "impact": (64, 128, 256), "order": (0, 1)}
 
     def set_shape(self, role, value):
         check(self.lib.icf_set_shape(self.ctx, self.SHAPE_ROLES.index(role), int(value)), self.ctx, "icf_set_shape")
 
     def shape_state(self):
-        """{role: (value in use, tuning?, {candidate: fastest ms})} for the four shape roles."""
+        """{role: (value in use, tuning?, {candidate: fastest ms})} for the shape roles."""
         out = {}
         for i, r in enumerate(self.SHAPE_ROLES):
             v, t = ctypes.c_int(), ctypes.c_int()

@@ -257,15 +257,17 @@ int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys, int sta
 
 /* Launch shapes besides the value reductions (no counterpart in the reference, which launches
  * everything with 128-thread blocks, GPU_Functionality.cs:38), tuned with the value reductions
- * by icf_set_autotune (three candidates per role, timed in turn on the role's launches):
+ * by icf_set_autotune (the candidates of each role timed in turn on the role's launches):
  *   ICF_SHAPE_PHASE1   workgroup size of phase 1 of the fused call (64, 128, 256)
  *   ICF_SHAPE_NARROW   workgroup size of the narrow phase (64, 128, 256)
  *   ICF_SHAPE_CLOSEST  lanes per (node, tet) tuple of the closest face (4, 8, 16)
  *   ICF_SHAPE_IMPACT   workgroup size of the impacts' phase 1 (64, 128, 256)
+ *   ICF_SHAPE_ORDER    the fused value launch's phases: 0 node forces first, then the BCSR
+ *                      values; 1 interleaved in rounds of eight workgroups (0, 1)
  * Every candidate gives the same results bit for bit.  icf_set_shape fixes a role's value on
  * this context (ICF_E_INVALID for other values); icf_get_shape reports the context's value
  * (the default until tuned or set), whether the role is still being tuned and the fastest
- * timing per candidate (best_ms[3], 0: not timed).  Finished tunings, of these roles and of the
+ * timing per candidate (best_ms[3], 0: not timed or no such candidate).  Finished tunings, of these roles and of the
  * value reductions, are kept by the library per device, role, precision and size class (units
  * of the launch to the next power of two); a context's launches of a role that it is neither
  * tuning nor has set use the kept choice for their size class, when there is one. */
@@ -273,6 +275,7 @@ int icf_set_chunk_keys(icf_ctx *ctx, int accumulate, int fp64, int keys, int sta
 #define ICF_SHAPE_NARROW 1
 #define ICF_SHAPE_CLOSEST 2
 #define ICF_SHAPE_IMPACT 3
+#define ICF_SHAPE_ORDER 4
 int icf_set_shape(icf_ctx *ctx, int role, int value);
 int icf_get_shape(icf_ctx *ctx, int role, int *value, int *tuning, float *best_ms);
 int icf_profile_reset(icf_ctx *ctx);

@@ -113,13 +113,15 @@ def test_every_chunk_shape_gives_the_same_bits(specimen_consts, precision):
     p2.gf.close()
 
 
-SHAPES = {"phase1": (64, 128, 256), "narrow": (64, 128, 256), "closest": (4, 8, 16), "impact": (64, 128, 256)}
+SHAPES = {"phase1": (64, 128, 256), "narrow": (64, 128, 256), "closest": (4, 8, 16), "impact": (64, 128, 256),
+          "order": (0, 1)}
 
 
 @pytest.mark.parametrize("precision", [_lib.ICF_FP64, _lib.ICF_MIXED], ids=["fp64", "mixed"])
 def test_every_launch_shape_gives_the_same_bits(specimen_consts, precision):
     """The other tuned roles (icf_set_shape / icf_get_shape): phase 1's, the narrow phase's and
-    the impacts' workgroup sizes and the closest face's lanes per tuple.  The whole iteration
+    the impacts' workgroup sizes, the closest face's lanes per tuple and the order of the
+    fused value launch's phases.  The whole iteration
     with contact (narrow phase and closest face included, their outputs compared) gives the
     same bits under every value of every role and after tuning them; tuning times every
     candidate of each role this iteration launches and keeps the fastest."""
@@ -128,7 +130,7 @@ def test_every_launch_shape_gives_the_same_bits(specimen_consts, precision):
     p = Pipeline(col, specimen_consts, st, precision=precision, contact=True)
     gf = p.gf
     assert gf.nImpacts > 0
-    defaults = {"phase1": 256, "narrow": 256, "closest": 8, "impact": 256}
+    defaults = {"phase1": 256, "narrow": 256, "closest": 8, "impact": 256, "order": 1}
     for r, (v, tuning, _) in gf.shape_state().items():
         assert v == defaults[r] and not tuning, (r, v)
 
@@ -147,7 +149,7 @@ def test_every_launch_shape_gives_the_same_bits(specimen_consts, precision):
             assert np.array_equal(masks, ref_masks) and np.array_equal(imp, ref_imp), (role, v)
             _same(_iteration(p), ref, (role, v))
         gf.set_shape(role, defaults[role])
-    for role, bad in (("phase1", 32), ("narrow", 512), ("closest", 2), ("impact", 100)):
+    for role, bad in (("phase1", 32), ("narrow", 512), ("closest", 2), ("impact", 100), ("order", 2)):
         assert gf.lib.icf_set_shape(gf.ctx, gf.SHAPE_ROLES.index(role), bad) == _lib.ICF_E_INVALID
     assert gf.lib.icf_set_shape(gf.ctx, 7, 64) == _lib.ICF_E_INVALID
     # tuning every role at once: each iteration launches all four shape roles
