@@ -482,6 +482,64 @@ def broad_phase_timing(r, reps=5):
     return {"pairs": int(n.value), "same_as_host": same, "ms_per_call": round(ms, 3)}
 
 
+def device_iteration_timing(r, reps=5):
+    """One Newton iteration's device work in the reference's order (ImplicitModel3.cs:376-397),
+    contact detection included: broad phase over the surface tets (icf_broad_phase, for the
+    host BVH), the narrow phase on a copy of the pairs, NL2set, closest face, pattern and pcsr
+    packing (icf_build_pattern, icf_pack_pcsr, for CreateStructure and TransferPCSR), the
+    fused assembly, the damage counts and the collision response.  Blocking calls: the host
+    reads the pair and tuple counts.  The impact rows keep the first iteration's node ids
+    (the state does not move here, so the impacts are the same).  Milliseconds per part and
+    per iteration (host clock)."""
+    gf, lib, ctx, chk, col = r.gf, r.lib, r.gf.ctx, r.check, r.col
+    nS, Q, ts, P = len(col.surf), r.Q, gf.tet_stride, r.nT
+    T, C, N, nnzb = len(col.elastic), len(col.czs), gf.csr.N, gf.csr.nnz
+    es, cs, cr = gf.el_elastic_stride, gf.cz_stride, gf.cr_stride
+    pairs = gf.buf("it_pairs", 2 * Q, np.int32)
+    alt = gf.buf("it_alt", len(col.alt), np.int32)
+    alt.upload(np.ascontiguousarray(col.alt, np.int32))
+    rows, cols = gf.buf("it_rows", N + 1, np.int32), gf.buf("it_cols", nnzb, np.int32)
+    nq, nt, nz, nd, nf = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib.icf_set_blocking(ctx, 1)
+
+    def narrow():
+        chk(lib.icf_d2d(ctx, r.itet.p, 0, pairs.p, 0, 8 * nq.value), ctx, "icf_d2d")
+        return lib.icf_narrow_phase(ctx, nq.value, gf.g_dn.p, gf.g_ie.p, r.itet.p, gf.el_all_stride, gf.nd_stride)
+
+    parts = [
+        ("broad_phase", lambda: lib.icf_broad_phase(ctx, gf.g_dn.p, gf.nd_stride, gf.g_ie.p, gf.el_all_stride, nS,
+                                                    1, pairs.p, Q, ctypes.byref(nq))),
+        ("narrow_phase", narrow),
+        ("nl2set", lambda: lib.icf_nl2set(ctx, r.itet.p, pairs.p, nq.value, gf.g_ie.p, gf.el_all_stride, r.nl2.p,
+                                          ts, ctypes.byref(nt))),
+        ("closest_face", lambda: lib.icf_find_closest_face(ctx, nt.value, ts, r.nl2.p, gf.g_dn.p, gf.g_ie.p,
+                                                           gf.g_ifc.p, gf.nd_stride, gf.el_all_stride,
+                                                           gf.fc_stride)),
+        ("pattern", lambda: lib.icf_build_pattern(ctx, alt.p, len(col.alt), gf.g_ie_pcsr.p, T, es, gf.g_icz.p, C,
+                                                  cs, r.icr.p, P, cr, N, rows.p, cols.p, nnzb, ctypes.byref(nz))),
+        ("pack_pcsr", lambda: lib.icf_pack_pcsr(ctx, alt.p, len(col.alt), rows.p, cols.p, N, gf.g_ie_pcsr.p, T, es,
+                                                gf.g_icz.p, C, cs, r.icr.p, P, cr)),
+        ("assembly", lambda: lib.icf_assemble_elems_and_czs(
+            ctx, gf.g_ie_pcsr.p, T, es, gf.g_dcz.p, gf.g_icz.p, C, cs, gf.g_dn.p, gf.nd_stride, r.h, r.vals.p, nnzb,
+            r.rhs.p, N, None, None)),
+        ("cz_counts", lambda: lib.icf_cz_counts(ctx, ctypes.byref(nd), ctypes.byref(nf))),
+        ("collision_response", lambda: lib.icf_collision_response(ctx, r.icr.p, gf.g_dn.p, r.h, r.vals.p, r.rhs.p, P,
+                                                                  cr, gf.nd_stride, r.k, r.eps)),
+    ]
+    ms = {name: 0.0 for name, _ in parts}
+    for rep in range(reps + 1):                  # the first pass is a warm-up
+        for name, fn in parts:
+            t0 = time.perf_counter()
+            chk(fn(), ctx, name)
+            if rep:
+                ms[name] += 1e3 * (time.perf_counter() - t0)
+        if nq.value != Q or nt.value != P or nz.value != nnzb:
+            raise RuntimeError(f"device iteration: {nq.value} pairs, {nt.value} tuples, {nz.value} blocks; "
+                               f"expected {Q}, {P}, {nnzb}")
+    out = {k: round(v / reps, 3) for k, v in ms.items()}
+    return {"ms_per_iteration": round(sum(out.values()), 3), "parts_ms": out}
+
+
 def host_threads():
     """Threads for the all-core CPU baseline: OMP_NUM_THREADS when set (the GPU box sets it to
     the CPU share of its one GPU), else the CPUs this process may run on."""
@@ -591,6 +649,12 @@ def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
                         "achieved": round(dbytes / avg_s / 1e9, 1),
                         "frac": round(dbytes / avg_s / 1e9 / HBM_PEAK_GBS, 4),
                         "traffic": int(traffic) if traffic else None}}
+    if contact:
+        out["device_broad_phase"] = broad_phase_timing(r)
+        log(f"{name}: device broad phase {out['device_broad_phase']}")
+        # last: icf_pack_pcsr resets the CZ flags the unit count above reads
+        out["device_iteration"] = device_iteration_timing(r)
+        log(f"{name}: device iteration {out['device_iteration']}")
     r.gf.close()
     return out
 
@@ -823,6 +887,10 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(r, args.cpu_seconds, units)
     if cpu:
         log(f"cpu baseline: {cpu}")
+    # last on this runner: icf_pack_pcsr resets the CZ flags the unit count reads
+    dev_it = device_iteration_timing(r) if contact else None
+    if dev_it:
+        log(f"device iteration: {dev_it}")
     line = {
         "metric": "Newton-iter assembly Melem+CZ/s, mixed vs FP64 at <=1e-5 rel err; HBM %peak",
         "value": round(units / sec / 1e6, 3),
@@ -869,6 +937,7 @@ def main():
         "roofline": roof,
         "hbm_copy_probe": bw,
         "device_broad_phase": bp,
+        "device_iteration": dev_it,
         "precision_study": study,
         "precision_study_fracture": study_damage,
         "precision_study_fracture_converged": study_conv,

@@ -114,6 +114,8 @@ _SIGS = {
     "SolveDouble3": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                              c_int]),
     "icfh_solve_threads": (c_int, [c_int]),
+    "icfh_solve_analyse": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(c_double)]),
     "icf_nodes_infer": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double]),
     "icf_nodes_add_dx": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "icf_nodes_accept": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_double]),

@@ -6,298 +6,393 @@
 // BVHN.cs:166-196 with kDOP24.cs:84-140), which is the set icfh_kdop_pairs finds on the host.
 // The k-DOP extents are sums and differences of coordinates and the overlap test is a
 // comparison, so both produce the same pairs; the order is (a, b) ascending.  The search is
-// a uniform grid of cell size = the largest AABB extent: tets are binned by the cell of
-// their AABB's minimum corner, so overlapping tets lie in neighbouring cells.
+// a uniform grid of cells at least as large as the largest AABB extent: tets are binned by
+// the cell of their AABB's minimum corner, so overlapping tets lie in neighbouring cells.
 //
 // NL2set: the (node, tet) tuples of GPU_Functionality.NarrowPhaseCollisionDetection
 // (GPU_Functionality.cs:565-600), de-duplicated keeping each tuple's first occurrence, in
-// the order of first occurrence (a HashSet without removals enumerates in insertion order).
+// the order of first occurrence (a HashSet without removals enumerates in insertion order):
+// a device hash table keeps each key's smallest insertion position, and the positions that
+// are their key's first are compacted in order.
 #pragma once
 
 namespace icf {
 
 constexpr int BP_DIRS = 12;
 
+// Bounds of the k-DOP pass, kept as order-preserving integer images of the doubles so that
+// atomicMin / atomicMax on them are exact: {largest AABB extent, min of the AABB minimum
+// corners (x, y, z), max of the same (x, y, z)}.
+constexpr int BP_NBOUNDS = 7;
+__host__ __device__ inline unsigned long long bp_ord(double v)
+{
+    unsigned long long u;
+    memcpy(&u, &v, sizeof(u));
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double bp_unord(unsigned long long u)
+{
+    u = (u >> 63) ? (u & 0x7fffffffffffffffULL) : ~u;
+    double v;
+    memcpy(&v, &u, sizeof(v));
+    return v;
+}
+
+__global__ void k_bp_bounds_init(unsigned long long *__restrict__ b)
+{
+    const int k = threadIdx.x;
+    // min slots start high, max slots low; slot BP_NBOUNDS is the bad-node-id flag
+    if (k <= BP_NBOUNDS) b[k] = (k >= 1 && k <= 3) ? ~0ULL : 0ULL;
+}
+
+// Per surface tet: the 12-direction k-DOP (lo, hi planes) from its 4 nodes' current
+// positions, and the block's contribution to the bounds (one atomic per bound per block).
 __global__ __launch_bounds__(256) void k_bp_kdop(const double *__restrict__ dn, int ns, const int *__restrict__ ie,
                                                  int es, int nT, double *__restrict__ lo, double *__restrict__ hi,
-                                                 double *__restrict__ ext, double *__restrict__ mn,
-                                                 int *__restrict__ err, int nNodesBound)
+                                                 unsigned long long *__restrict__ bounds, int nNodesBound)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nT) return;
-    double l[BP_DIRS], h[BP_DIRS];
-#pragma unroll
-    for (int d = 0; d < BP_DIRS; d++) {
-        l[d] = 1e300;
-        h[d] = -1e300;
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int n = ie[t + (size_t)es * v];
-        if (n < 0 || (nNodesBound > 0 && n >= nNodesBound)) {
-            atomicOr(err, 1);
-            return;
-        }
-        const double x = dn[n + (size_t)ns * ND_XC], y = dn[n + (size_t)ns * (ND_XC + 1)],
-                     z = dn[n + (size_t)ns * (ND_XC + 2)];
-        const double p[BP_DIRS] = {x, y, z, x + y, x + z, y + z, x - y, x - z, y - z, x + y - z, x + z - y, y + z - x};
+    double b[BP_NBOUNDS] = {0, 1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    if (t < nT) {
+        double l[BP_DIRS], h[BP_DIRS];
 #pragma unroll
         for (int d = 0; d < BP_DIRS; d++) {
-            l[d] = fmin(l[d], p[d]);
-            h[d] = fmax(h[d], p[d]);
+            l[d] = 1e300;
+            h[d] = -1e300;
+        }
+        bool ok = true;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int n = ie[t + (size_t)es * v];
+            if (n < 0 || (nNodesBound > 0 && n >= nNodesBound)) {
+                ok = false;
+                break;
+            }
+            const double x = dn[n + (size_t)ns * ND_XC], y = dn[n + (size_t)ns * (ND_XC + 1)],
+                         z = dn[n + (size_t)ns * (ND_XC + 2)];
+            const double p[BP_DIRS] = {x, y, z, x + y, x + z, y + z, x - y, x - z, y - z, x + y - z, x + z - y, y + z - x};
+#pragma unroll
+            for (int d = 0; d < BP_DIRS; d++) {
+                l[d] = fmin(l[d], p[d]);
+                h[d] = fmax(h[d], p[d]);
+            }
+        }
+        if (!ok) {
+            atomicMax(&bounds[BP_NBOUNDS], 1ULL);
+        } else {
+#pragma unroll
+            for (int d = 0; d < BP_DIRS; d++) {
+                lo[(size_t)d * nT + t] = l[d];
+                hi[(size_t)d * nT + t] = h[d];
+            }
+            b[0] = fmax(h[0] - l[0], fmax(h[1] - l[1], h[2] - l[2]));
+#pragma unroll
+            for (int k = 0; k < 3; k++) b[1 + k] = b[4 + k] = l[k];
         }
     }
+    // block reduction: across the wave, then the block's waves through LDS
+    __shared__ double part[256 / 64][BP_NBOUNDS];
 #pragma unroll
-    for (int d = 0; d < BP_DIRS; d++) {
-        lo[(size_t)d * nT + t] = l[d];
-        hi[(size_t)d * nT + t] = h[d];
+    for (int k = 0; k < BP_NBOUNDS; k++) {
+        double v = b[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double w = __shfl_xor(v, o);
+            v = (k == 0 || k >= 4) ? fmax(v, w) : fmin(v, w);
+        }
+        b[k] = v;
     }
-    ext[t] = fmax(h[0] - l[0], fmax(h[1] - l[1], h[2] - l[2]));
-#pragma unroll
-    for (int k = 0; k < 3; k++) mn[(size_t)k * nT + t] = l[k];
+    const int wave = threadIdx.x >> 6;
+    if (__lane_id() == 0)
+        for (int k = 0; k < BP_NBOUNDS; k++) part[wave][k] = b[k];
+    __syncthreads();
+    if (threadIdx.x < BP_NBOUNDS) {
+        const int k = threadIdx.x;
+        double v = part[0][k];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = (k == 0 || k >= 4) ? fmax(v, part[w][k]) : fmin(v, part[w][k]);
+        if (k == 0 || k >= 4) atomicMax(&bounds[k], bp_ord(v));
+        else atomicMin(&bounds[k], bp_ord(v));
+    }
 }
 
-// red = {largest extent, min x, min y, min z}
-__device__ inline double bp_cell_size(const double *red) { return red[0] > 0 ? red[0] * 1.000001 : 1.0; }
+// The search grid: cells of size cs (at least the largest AABB extent), n[k] cells per axis
+// over the AABB minimum corners, stored with a margin -- one cell below and two above in x,
+// one below and one above in y and z -- so that the neighbour rows a cell reads never wrap.
+// Cell (x, y, z) of the margin grid has the linear key x + gx (y + gy z).  Tets are binned
+// by the cell of their AABB's minimum corner; with cs >= every extent, overlapping tets lie
+// in the same or neighbouring cells.
+struct BpGrid {
+    double cs, org[3];
+    int n[3], gx, gy, gz;
+};
 
-constexpr int BP_CELL_BITS = 21;
-
-__global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, int nT, const double *__restrict__ red,
-                                                 unsigned long long *__restrict__ key, int *__restrict__ idx,
-                                                 int *__restrict__ err)
+// Per tet: its cell key and its rank among the tets of its cell (count[key] is bumped
+// atomically, so the order within a cell varies from call to call; the pairs do not).
+__global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, int nT, BpGrid g,
+                                                 unsigned *__restrict__ key, int *__restrict__ rank,
+                                                 int *__restrict__ count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nT) return;
-    const double cs = bp_cell_size(red);
-    unsigned long long k = 0;
+    int c[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const double c = floor((lo[(size_t)a * nT + t] - red[1 + a]) / cs);
-        if (!(c >= 0 && c < (double)((1 << BP_CELL_BITS) - 2))) {
-            atomicOr(err, 1);
-            return;
-        }
-        k |= (unsigned long long)c << (BP_CELL_BITS * a);
+        const double f = floor((lo[(size_t)a * nT + t] - g.org[a]) / g.cs);
+        c[a] = f < 0 ? 0 : (f > (double)(g.n[a] - 1) ? g.n[a] - 1 : (int)f);
     }
+    const unsigned k = (unsigned)(c[0] + 1) + (unsigned)g.gx * ((unsigned)(c[1] + 1) + (unsigned)g.gy * (unsigned)(c[2] + 1));
     key[t] = k;
-    idx[t] = t;
+    rank[t] = atomicAdd(&count[k], 1);
 }
 
-// The k-DOPs and node ids in cell order: sorted position j holds tet sidx[j], so that the tets
-// of one cell, and of neighbouring cells, are adjacent in memory for k_bp_cellpairs.  Per position:
-// the k-DOP as (lo, hi) pairs per direction (skd, 24 doubles), the node ids (snode), and the
-// axis-aligned box in FP32 rounded outward, lo down and hi up (box): a prefilter that never
-// rejects a pair whose FP64 boxes overlap.
+// The k-DOPs and node ids in cell order (a counting sort: first[key] is the exclusive scan
+// of the cell counts): tet t goes to position j = first[key[t]] + rank[t], so that the tets
+// of one cell, and of neighbouring cells, are adjacent in memory for the pair search.  Per
+// position: the tet (sidx), the k-DOP as (lo, hi) pairs per direction (skd, 24 doubles), the
+// node ids (snode), and the k-DOP in FP32 rounded outward, lo down and hi up (sk32: 12 lo then
+// 12 hi): a prefilter that never rejects a pair whose FP64 k-DOPs overlap.  first[L] for
+// L = 0 .. G is then the first position of the cells with keys >= L.
 __global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ lo, const double *__restrict__ hi,
-                                                   int nT, const int *__restrict__ sidx, const int *__restrict__ ie,
-                                                   int es, double2 *__restrict__ skd, int4 *__restrict__ snode,
-                                                   float4 *__restrict__ box)
+                                                   int nT, const unsigned *__restrict__ key,
+                                                   const int *__restrict__ rank, const int *__restrict__ first,
+                                                   const int *__restrict__ ie, int es, int *__restrict__ sidx,
+                                                   double2 *__restrict__ skd, int4 *__restrict__ snode,
+                                                   float4 *__restrict__ sk32)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nT) return;
-    const int t = sidx[j];
-    double l[BP_DIRS], h[BP_DIRS];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const int j = first[key[t]] + rank[t];
+    sidx[j] = t;
+    float f[2 * BP_DIRS];
 #pragma unroll
     for (int d = 0; d < BP_DIRS; d++) {
-        l[d] = lo[(size_t)d * nT + t];
-        h[d] = hi[(size_t)d * nT + t];
-        skd[(size_t)j * BP_DIRS + d] = make_double2(l[d], h[d]);
+        const double l = lo[(size_t)d * nT + t], h = hi[(size_t)d * nT + t];
+        skd[(size_t)j * BP_DIRS + d] = make_double2(l, h);
+        f[d] = __double2float_rd(l);
+        f[BP_DIRS + d] = __double2float_ru(h);
     }
     snode[j] = make_int4(ie[t], ie[t + (size_t)es], ie[t + 2 * (size_t)es], ie[t + 3 * (size_t)es]);
-    box[2 * (size_t)j] = make_float4(__double2float_rd(l[0]), __double2float_rd(l[1]), __double2float_rd(l[2]), 0.f);
-    box[2 * (size_t)j + 1] = make_float4(__double2float_ru(h[0]), __double2float_ru(h[1]), __double2float_ru(h[2]), 0.f);
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+        sk32[(size_t)j * 6 + q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
 }
 
-// Cells of the sorted positions: flag[j] = 1 where a new cell key starts.
-__global__ __launch_bounds__(256) void k_bp_cellflag(const unsigned long long *__restrict__ skey, int nT,
-                                                     int *__restrict__ flag)
+// A wave's LDS writes made visible to the wave's other lanes (no workgroup barrier).
+__device__ inline void bp_wave_sync()
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nT) flag[j] = (j == 0 || skey[j] != skey[j - 1]) ? 1 : 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// From the exclusive scan of the flags: each cell's key and first position; cstart[ncells] = nT.
-__global__ __launch_bounds__(256) void k_bp_cells(const unsigned long long *__restrict__ skey, int nT,
-                                                  const int *__restrict__ flag, const int *__restrict__ cidx,
-                                                  unsigned long long *__restrict__ ckey, int *__restrict__ cstart)
+// The pair search, first pass: one wave per cell of the grid's interior, no workgroup
+// barriers.  The candidates are the cell's own tets and those of the 13 neighbouring cells
+// with a larger key (each pair of cells visited once): 5 runs of consecutive keys -- (x, x+1)
+// of the cell's own row, and the rows (x-1 .. x+1) at (y+1, z), (y-1, z+1), (y, z+1),
+// (y+1, z+1) -- hence 5 ranges of sorted positions; in the first range only larger positions
+// pair up.  The 64 lanes hold 64 candidates at a time (FP32 k-DOP and node ids in registers,
+// loaded coalesced); the cell's own tets, BPW_OWN at a time, are staged in the wave's LDS and
+// read by all lanes at once.  A pair passes if its outward-rounded FP32 k-DOPs overlap in all
+// 12 directions and (when asked) the tets share no node.  The passing (own, candidate)
+// positions collect in the wave's LDS buffer and leave in batches, one atomicAdd on *total
+// per batch (positions beyond cap are only counted; the caller then grows the buffer and
+// searches again).  The second pass applies the exact FP64 test.
+constexpr int BPW_WAVES = 4, BPW_OWN = 32, BPW_BUF = 256;
+__global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__restrict__ sk32,
+                                                                const int4 *__restrict__ snode, int exclude_shared,
+                                                                BpGrid g, const int *__restrict__ first,
+                                                                unsigned long long *__restrict__ total,
+                                                                unsigned long long *__restrict__ out, long long cap)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nT) return;
-    if (flag[j]) {
-        ckey[cidx[j]] = skey[j];
-        cstart[cidx[j]] = j;
-    }
-    if (j == nT - 1) cstart[cidx[j] + flag[j]] = nT;
-}
-
-// One workgroup per cell: every overlapping pair with both tets in the cell, or one in the cell
-// and the other in one of the 13 neighbouring cells whose key is larger (each pair of cells
-// visited once), over the cell-ordered copies.  The candidates (the cell itself and the 13
-// cells, concatenated) and the cell's own tets are brought into LDS, BP_C and BP_A at a time,
-// with coalesced loads; the lanes then take (own, candidate) pairs in turn, BP_T at a time.  A
-// pair passes the outward-rounded FP32 box prefilter, then (when asked) the shared-node
-// exclusion, which removes most neighbours of a conforming mesh before the FP64 12-direction
-// test (the predicate of the reference's k-DOP overlap).  Pairs collect in LDS as (min << 32
-// | max) keys of the original indices and leave in batches: one atomicAdd on *total takes
-// the batch's places in out (keys beyond cap are only counted; the caller then grows the
-// buffer and searches again).  The order of the keys is the caller's to fix (it sorts them).
-constexpr int BP_A = 128, BP_C = 256, BP_T = 256, BP_BUF = 1024;
-__global__ __launch_bounds__(BP_T) void k_bp_cellpairs(const double2 *__restrict__ skd, int nT,
-                                                       const int *__restrict__ sidx, const int4 *__restrict__ snode,
-                                                       const float4 *__restrict__ box, int exclude_shared,
-                                                       const unsigned long long *__restrict__ ckey,
-                                                       const int *__restrict__ cstart, int ncells,
-                                                       unsigned long long *__restrict__ total,
-                                                       unsigned long long *__restrict__ out, long long cap)
-{
-    __shared__ float4 abox[2 * BP_A], cbox[2 * BP_C];
-    __shared__ int4 anode[BP_A], cnode[BP_C];
-    __shared__ int cj[BP_C];
-    __shared__ int nb_lo[14], nb_hi[14], nb_pre[15];
-    __shared__ unsigned long long pbuf[BP_BUF];
-    __shared__ int wave_cnt[BP_T / 64];
-    __shared__ int pcount;
-    __shared__ unsigned long long flush_base;
-    const int c = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-    const int s0 = cstart[c], s1 = cstart[c + 1];
-    if (tid < 14) {
-        // neighbour t: t = 0 the cell itself, 1-13 the offsets (dx, dy, dz) whose key is larger
-        int lo = 0, hi = 0;
-        const unsigned long long k = ckey[c];
-        const long long cx = (long long)(k & ((1ULL << BP_CELL_BITS) - 1));
-        const long long cy = (long long)((k >> BP_CELL_BITS) & ((1ULL << BP_CELL_BITS) - 1));
-        const long long cz = (long long)(k >> (2 * BP_CELL_BITS));
-        if (tid == 0) {
-            lo = s0;
-            hi = s1;
-        } else {
-            const int q = tid + 13;            // offsets 14..26 of the 27 in (dz, dy, dx) order
-            const long long nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
-            if (nx >= 0 && ny >= 0 && nz >= 0) {
-                const unsigned long long nk = (unsigned long long)nx | ((unsigned long long)ny << BP_CELL_BITS) |
-                                              ((unsigned long long)nz << (2 * BP_CELL_BITS));
-                int l = 0, h = ncells;
-                while (l < h) {
-                    const int m = (l + h) >> 1;
-                    if (ckey[m] < nk) l = m + 1;
-                    else h = m;
-                }
-                if (l < ncells && ckey[l] == nk) {
-                    lo = cstart[l];
-                    hi = cstart[l + 1];
-                }
-            }
-        }
-        nb_lo[tid] = lo;
-        nb_hi[tid] = hi;
-    }
-    if (tid == 0) pcount = 0;
-    __syncthreads();
-    if (tid == 0) {
-        nb_pre[0] = 0;
-        for (int t = 0; t < 14; t++) nb_pre[t + 1] = nb_pre[t] + nb_hi[t] - nb_lo[t];
-    }
-    __syncthreads();
-    const int K = nb_pre[14];
-    // move the collected keys out: one atomicAdd for the batch, then a coalesced copy
+    __shared__ float4 own[BPW_WAVES][BPW_OWN][7];
+    __shared__ unsigned long long pbuf[BPW_WAVES][BPW_BUF];
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    const int ncell = g.n[0] * g.n[1] * g.n[2];
+    const int i = (int)blockIdx.x * BPW_WAVES + wave;
+    if (i >= ncell) return;
+    const int x = i % g.n[0] + 1, y = (i / g.n[0]) % g.n[1] + 1, z = i / (g.n[0] * g.n[1]) + 1;
+    const int L = x + g.gx * (y + g.gy * z);
+    const int s0 = first[L], s1 = first[L + 1];
+    if (s0 == s1) return;
+    const int gxy = g.gx * g.gy;
+    const int r_lo[5] = {s0, first[L - 1 + g.gx], first[L - 1 - g.gx + gxy], first[L - 1 + gxy],
+                         first[L - 1 + g.gx + gxy]};
+    const int r_hi[5] = {first[L + 2], first[L + 2 + g.gx], first[L + 2 - g.gx + gxy], first[L + 2 + gxy],
+                         first[L + 2 + g.gx + gxy]};
+    int r_pre[6];
+    r_pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 5; r++) r_pre[r + 1] = r_pre[r] + r_hi[r] - r_lo[r];
+    const int K = r_pre[5];
+    float4 (*ow)[7] = own[wave];
+    unsigned long long *buf = pbuf[wave];
+    const unsigned long long below = (1ULL << lane) - 1;
+    int cnt = 0;
     auto flush = [&]() {
-        __syncthreads();
-        const int n = pcount;
-        if (tid == 0) flush_base = n ? atomicAdd(total, (unsigned long long)n) : 0ULL;
-        __syncthreads();
-        const unsigned long long base = flush_base;
-        for (int i = tid; i < n; i += BP_T)
-            if (base + i < (unsigned long long)cap) out[base + i] = pbuf[i];
-        __syncthreads();
-        if (tid == 0) pcount = 0;
-        __syncthreads();
+        bp_wave_sync();
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(total, (unsigned long long)cnt);
+        base = __shfl(base, 0);
+        for (int q = lane; q < cnt; q += 64)
+            if (base + q < (unsigned long long)cap) out[base + q] = buf[q];
+        bp_wave_sync();
+        cnt = 0;
     };
-    for (int a0 = s0; a0 < s1; a0 += BP_A) {
-        const int na = min(BP_A, s1 - a0);
-        __syncthreads();
-        for (int i = tid; i < na; i += BP_T) {
-            abox[2 * i] = box[2 * (size_t)(a0 + i)];
-            abox[2 * i + 1] = box[2 * (size_t)(a0 + i) + 1];
-            anode[i] = snode[a0 + i];
+    for (int a0 = s0; a0 < s1; a0 += BPW_OWN) {
+        const int na = min(BPW_OWN, s1 - a0);
+        bp_wave_sync();   // the previous own tets are no longer read
+        if (lane < na) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) ow[lane][q] = sk32[(size_t)(a0 + lane) * 6 + q];
+            const int4 n4 = snode[a0 + lane];
+            ow[lane][6] = make_float4(__int_as_float(n4.x), __int_as_float(n4.y), __int_as_float(n4.z),
+                                      __int_as_float(n4.w));
         }
-        for (int k0 = 0; k0 < K; k0 += BP_C) {
-            const int nk = min(BP_C, K - k0);
-            __syncthreads();
-            for (int i = tid; i < nk; i += BP_T) {
-                const int k = k0 + i;
-                int t = 0;
-                while (nb_pre[t + 1] <= k) t++;
-                const int jb = nb_lo[t] + (k - nb_pre[t]);
-                cj[i] = t == 0 ? -1 - jb : jb;   // the cell's own tets: pairs with larger positions only
-                cbox[2 * i] = box[2 * (size_t)jb];
-                cbox[2 * i + 1] = box[2 * (size_t)jb + 1];
-                cnode[i] = snode[jb];
+        bp_wave_sync();
+        for (int k0 = 0; k0 < K; k0 += 64) {
+            const int k = k0 + lane;
+            const bool valid = k < K;
+            int jb = s0, own_row = 0;
+#pragma unroll
+            for (int r = 0; r < 5; r++)
+                if (k >= r_pre[r] && k < r_pre[r + 1]) {
+                    jb = r_lo[r] + (k - r_pre[r]);
+                    own_row = r == 0;
+                }
+            float c[2 * BP_DIRS];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const float4 v = sk32[(size_t)jb * 6 + q];
+                c[4 * q] = v.x;
+                c[4 * q + 1] = v.y;
+                c[4 * q + 2] = v.z;
+                c[4 * q + 3] = v.w;
             }
-            __syncthreads();
-            const int npairs = na * nk;
-            for (int p0 = 0; p0 < npairs; p0 += BP_T) {
-                const int p = p0 + tid;
-                bool hit = false;
-                int ja = 0, jb = 0;
-                if (p < npairs) {
-                    const int ia = p / nk, ik = p % nk;
-                    ja = a0 + ia;
-                    const int code = cj[ik];
-                    jb = code < 0 ? -1 - code : code;
-                    if (code >= 0 || jb > ja) {
-                        const float4 bl = abox[2 * ia], bh = abox[2 * ia + 1];
-                        const float4 ol = cbox[2 * ik], oh = cbox[2 * ik + 1];
-                        hit = !(bl.x > oh.x || bl.y > oh.y || bl.z > oh.z || bh.x < ol.x || bh.y < ol.y ||
-                                bh.z < ol.z);
-                        if (hit && exclude_shared) {
-                            const int4 na4 = anode[ia], nb4 = cnode[ik];
-                            const int m[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            const int4 cn = snode[jb];
+            for (int ia = 0; ia < na; ia++) {
+                const int ja = a0 + ia;
+                float a[2 * BP_DIRS];
 #pragma unroll
-                            for (int j2 = 0; j2 < 4; j2++)
-                                hit = hit && m[j2] != na4.x && m[j2] != na4.y && m[j2] != na4.z && m[j2] != na4.w;
-                        }
-                        if (hit) {
+                for (int q = 0; q < 6; q++) {
+                    const float4 v = ow[ia][q];
+                    a[4 * q] = v.x;
+                    a[4 * q + 1] = v.y;
+                    a[4 * q + 2] = v.z;
+                    a[4 * q + 3] = v.w;
+                }
+                const float4 an4 = ow[ia][6];
+                // the wave's mask of passing lanes, one ballot per comparison (each a compare
+                // into a mask and a scalar AND)
+                unsigned long long mk = __ballot(valid & (!own_row | (jb > ja)));
 #pragma unroll
-                            for (int d = 0; d < BP_DIRS; d++) {
-                                const double2 u = skd[(size_t)ja * BP_DIRS + d], v = skd[(size_t)jb * BP_DIRS + d];
-                                hit = hit && !(u.x > v.y) && !(u.y < v.x);
-                            }
-                        }
-                    }
+                for (int d = 0; d < BP_DIRS; d++) {
+                    mk &= ~__ballot(a[d] > c[BP_DIRS + d]);
+                    mk &= ~__ballot(a[BP_DIRS + d] < c[d]);
                 }
-                // this round's places in the batch: lane prefix within the wave, then the waves
-                const unsigned long long m = __ballot(hit);
-                if (lane == 0) wave_cnt[wave] = __popcll(m);
-                __syncthreads();
-                int at = pcount, round = 0;
-                for (int w = 0; w < BP_T / 64; w++) {
-                    if (w < wave) at += wave_cnt[w];
-                    round += wave_cnt[w];
+                if (exclude_shared) {
+                    const int an[4] = {__float_as_int(an4.x), __float_as_int(an4.y), __float_as_int(an4.z),
+                                       __float_as_int(an4.w)};
+                    const int m[4] = {cn.x, cn.y, cn.z, cn.w};
+#pragma unroll
+                    for (int j2 = 0; j2 < 4; j2++)
+#pragma unroll
+                        for (int j1 = 0; j1 < 4; j1++) mk &= ~__ballot(m[j2] == an[j1]);
                 }
-                at += __popcll(m & ((1ULL << lane) - 1));
-                if (hit) {
-                    const unsigned a = (unsigned)sidx[ja], b = (unsigned)sidx[jb];
-                    pbuf[at] = ((unsigned long long)(a < b ? a : b) << 32) | (a < b ? b : a);
-                }
-                __syncthreads();
-                if (tid == 0) pcount += round;
-                __syncthreads();
-                if (pcount > BP_BUF - BP_T) flush();
+                if (mk == 0) continue;
+                const bool hit = (mk >> lane) & 1;
+                if (hit) buf[cnt + __popcll(mk & below)] = ((unsigned long long)ja << 32) | (unsigned)jb;
+                cnt += __popcll(mk);
+                if (cnt > BPW_BUF - 64) flush();
             }
         }
     }
-    flush();
+    if (cnt) flush();
 }
 
-__global__ __launch_bounds__(256) void k_bp_unpack(const unsigned long long *__restrict__ keys, long long n,
-                                                   int *__restrict__ pairs)
+// The pair search, second pass: per candidate (own, candidate) position pair of the first
+// pass, the exact test -- the FP64 12-direction overlap, the predicate of the reference's
+// k-DOP test.  The FP32 copies decide first where they can: the first pass's outward-rounded
+// values, each moved one step inward (nextafter), still overlap only if the FP64 values do
+// (lo <= nextafter(lo rounded down, +inf) and hi >= nextafter(hi rounded up, -inf)), so the
+// FP64 k-DOPs are read only for the pairs that are within an FP32 step of touching.  A pair
+// that passes becomes (a, b), a < b its original indices (io[i] = a << 32 | b), and takes its
+// rank among the pairs of the same a (count[a] bumped atomically); rank[i] = -1 for a pair
+// that fails.
+__global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ skd, const float4 *__restrict__ sk32,
+                                                  const int *__restrict__ sidx, long long n,
+                                                  unsigned long long *__restrict__ io, int *__restrict__ rank,
+                                                  int *__restrict__ count)
 {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const unsigned long long v = keys[k];
-    pairs[2 * k] = (int)(v >> 32);
-    pairs[2 * k + 1] = (int)(v & 0xffffffffULL);
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long v = io[i];
+    const int ja = (int)(v >> 32), jb = (int)(v & 0xffffffffULL);
+    float fa[2 * BP_DIRS], fb[2 * BP_DIRS];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const float4 x = sk32[(size_t)ja * 6 + q], y = sk32[(size_t)jb * 6 + q];
+        fa[4 * q] = x.x, fa[4 * q + 1] = x.y, fa[4 * q + 2] = x.z, fa[4 * q + 3] = x.w;
+        fb[4 * q] = y.x, fb[4 * q + 1] = y.y, fb[4 * q + 2] = y.z, fb[4 * q + 3] = y.w;
+    }
+    bool sure = true;
+#pragma unroll
+    for (int d = 0; d < BP_DIRS; d++) {
+        sure = sure & (nextafterf(fa[d], INFINITY) <= nextafterf(fb[BP_DIRS + d], -INFINITY)) &
+               (nextafterf(fb[d], INFINITY) <= nextafterf(fa[BP_DIRS + d], -INFINITY));
+    }
+    bool hit = sure;
+    if (!sure) {
+        hit = true;
+#pragma unroll
+        for (int d = 0; d < BP_DIRS; d++) {
+            const double2 u = skd[(size_t)ja * BP_DIRS + d], w = skd[(size_t)jb * BP_DIRS + d];
+            hit = hit & !(u.x > w.y) & !(u.y < w.x);
+        }
+    }
+    int r = -1;
+    if (hit) {
+        const unsigned oa = (unsigned)sidx[ja], ob = (unsigned)sidx[jb];
+        const unsigned a = oa < ob ? oa : ob, b = oa < ob ? ob : oa;
+        io[i] = ((unsigned long long)a << 32) | b;
+        r = atomicAdd(&count[a], 1);
+    }
+    rank[i] = r;
+}
+
+// The pairs grouped by a, in no particular order within a group: pair i goes to
+// start[a] + rank[i] (start = exclusive scan of the counts).
+__global__ __launch_bounds__(256) void k_bp_scatter(const unsigned long long *__restrict__ io,
+                                                    const int *__restrict__ rank, long long n,
+                                                    const int *__restrict__ start,
+                                                    unsigned long long *__restrict__ grp)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = rank[i];
+    if (r < 0) return;
+    const unsigned long long v = io[i];
+    grp[start[v >> 32] + r] = v;
+}
+
+// Each grouped pair's final place: its group's start plus the number of b's in the group
+// smaller than its own (the b's of a group are distinct), so the pairs come out in (a, b)
+// order; those below max_pairs are written.
+__global__ __launch_bounds__(256) void k_bp_place(const unsigned long long *__restrict__ grp,
+                                                  const int *__restrict__ start, int nT, long long max_pairs,
+                                                  int *__restrict__ pairs)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)start[nT]) return;
+    const unsigned long long v = grp[p];
+    const int a = (int)(v >> 32), b = (int)(v & 0xffffffffULL);
+    const int s = start[a], e = start[a + 1];
+    int r = 0;
+    for (int q = s; q < e; q++) r += (int)(grp[q] & 0xffffffffULL) < b;
+    const long long at = (long long)s + r;
+    if (at < max_pairs) {
+        pairs[2 * at] = a;
+        pairs[2 * at + 1] = b;
+    }
 }
 
 // NL2set, pass 1: tuples per candidate pair (set bits of its mask).
@@ -308,11 +403,22 @@ __global__ __launch_bounds__(256) void k_nl2_count(const int *__restrict__ masks
     cnt[i] = __popc((unsigned)masks[2LL * i] & 0xffu);
 }
 
-// Pass 2: (node << 32 | tet) keys with their insertion position 8 i + slot, in insertion order.
+// Pass 2: the (node << 32 | tet) keys in insertion order (position o), each inserted into an
+// open-addressing table (linear probing, at most half full) that keeps the smallest
+// position of every key: its first occurrence.
+constexpr unsigned long long NL2_EMPTY = ~0ULL;   // no key: node and tet ids are >= 0
+__device__ inline unsigned long long nl2_hash(unsigned long long k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    return k;
+}
 __global__ __launch_bounds__(256) void k_nl2_emit(const int *__restrict__ masks, const int *__restrict__ pairs,
                                                   int nPairs, const int *__restrict__ ie, int es,
                                                   const int *__restrict__ off, unsigned long long *__restrict__ key,
-                                                  int *__restrict__ pos)
+                                                  unsigned long long *__restrict__ tkey, int *__restrict__ tpos,
+                                                  unsigned long long tmask)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPairs) return;
@@ -325,20 +431,50 @@ __global__ __launch_bounds__(256) void k_nl2_emit(const int *__restrict__ masks,
         // bits 0-3: vertex s of B inside A; bits 4-7: vertex s-4 of A inside B
         const int node = s < 4 ? ie[tB + (size_t)es * s] : ie[tA + (size_t)es * (s - 4)];
         const int tet = s < 4 ? tA : tB;
-        key[o] = ((unsigned long long)(unsigned)node << 32) | (unsigned)tet;
-        pos[o] = 8 * i + s;
+        const unsigned long long k = ((unsigned long long)(unsigned)node << 32) | (unsigned)tet;
+        key[o] = k;
+        for (unsigned long long h = nl2_hash(k) & tmask;; h = (h + 1) & tmask) {
+            const unsigned long long prev = atomicCAS(&tkey[h], NL2_EMPTY, k);
+            if (prev == NL2_EMPTY || prev == k) {
+                atomicMin(&tpos[h], o);
+                break;
+            }
+        }
         o++;
     }
 }
 
-// nl2 rows 0-1 from the first-occurrence-ordered keys.
+// Pass 3: keep[o] = 1 where position o is its key's first occurrence (keep[n] = 0 is set by
+// the caller, so the exclusive scan of keep ends with the count).
+__global__ __launch_bounds__(256) void k_nl2_keep(const unsigned long long *__restrict__ key, int n,
+                                                  const unsigned long long *__restrict__ tkey,
+                                                  const int *__restrict__ tpos, unsigned long long tmask,
+                                                  int *__restrict__ keep)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const unsigned long long k = key[o];
+    unsigned long long h = nl2_hash(k) & tmask;
+    while (tkey[h] != k) h = (h + 1) & tmask;
+    keep[o] = tpos[h] == o ? 1 : 0;
+}
+
+// nl2 rows 0-1: the kept keys at their places in first-occurrence order.
 __global__ __launch_bounds__(256) void k_nl2_write(const unsigned long long *__restrict__ key, int n,
+                                                   const int *__restrict__ keep, const int *__restrict__ dst,
                                                    int *__restrict__ nl2, int ts)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    nl2[k] = (int)(key[k] >> 32);
-    nl2[k + (size_t)ts] = (int)(key[k] & 0xffffffffULL);
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n || !keep[o]) return;
+    const int k = dst[o];
+    nl2[k] = (int)(key[o] >> 32);
+    nl2[k + (size_t)ts] = (int)(key[o] & 0xffffffffULL);
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, long long n, unsigned long long v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 }  // namespace icf

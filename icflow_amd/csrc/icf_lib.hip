@@ -8,6 +8,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <math.h>
+#include <climits>
 #include <string>
 #include <vector>
 #include <list>
@@ -215,8 +216,8 @@ struct icf_ctx {
     DevConsts hc;
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
     DevBuf pat_a, pat_b;           // pattern build / broad phase / NL2set: 64-bit keys
-    DevBuf bp_f, bp_k, bp_i, bp_l;   // broad phase and NL2set scratch (doubles, keys, ints, int64)
-    DevBuf bp_c;                     // broad phase cells (flags, cell indices, cell starts)
+    DevBuf bp_f, bp_i, bp_l;         // broad phase and NL2set scratch (doubles, ints, int64)
+    DevBuf bp_c;                     // broad phase grid (first sorted position per cell key)
     long long bp_cap = 0;            // pair keys the broad phase's buffer holds (grows with use)
     DevBuf tsum;                     // per-node tet sums of the fused call
     DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
@@ -1269,7 +1270,7 @@ int icf_destroy(icf_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
-                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_k,   &ctx->bp_i, &ctx->bp_l, &ctx->bp_c,
+                      &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_i, &ctx->bp_l, &ctx->bp_c,
                       &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids, &ctx->batch.d};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -2327,41 +2328,45 @@ int icf_build_pattern(icf_ctx *ctx, const int *altId, int nNodes, const int *ie,
         p.out0 = M;
         M += (long long)p.n * pat_pairs(p.K, p.strict);
     }
-    if (M >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: more than 2^31 candidate pairs");
-    int *d_err = ctx->small + 1, *d_num = ctx->small + 5;
+    if (M >= (1LL << 31) - 1) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: 2^31 or more candidate pairs");
+    int *d_err = ctx->small + 1;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+    int cb = 1;
+    while ((1LL << cb) <= (long long)N) cb++;
     int num = 0;
-    unsigned long long last = PAT_NONE;
     if (M > 0) {
         if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)M)) return r;
         if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)M)) return r;
+        if (int r = grow(ctx, ctx->bp_l, sizeof(int) * 2 * ((size_t)M + 1))) return r;
         auto *a = (unsigned long long *)ctx->pat_a.p, *b = (unsigned long long *)ctx->pat_b.p;
+        int *flag = (int *)ctx->bp_l.p, *pos = flag + M + 1;
         for (const PatSrc &p : src)
             if (p.n)
                 hipLaunchKernelGGL(k_pat_emit, dim3(blocks_for((long long)p.n * pat_pairs(p.K, p.strict), 256)),
-                                   dim3(256), 0, ctx->stream, p, altId, nNodes, N, a, d_err);
+                                   dim3(256), 0, ctx->stream, p, altId, nNodes, N, cb, a, d_err);
         size_t sb = 0, ub = 0;
-        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sb, a, b, (int)M, 0, 64, ctx->stream));
-        HIPCHK(ctx, hipcub::DeviceSelect::Unique(nullptr, ub, b, a, d_num, (int)M, ctx->stream));
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sb, a, b, (int)M, 0, 2 * cb, ctx->stream));
+        HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, ub, flag, pos, (int)M + 1, ctx->stream));
         if (int r = grow(ctx, ctx->scan_tmp, sb > ub ? sb : ub)) return r;
-        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, sb, a, b, (int)M, 0, 64, ctx->stream));
-        HIPCHK(ctx, hipcub::DeviceSelect::Unique(ctx->scan_tmp.p, ub, b, a, d_num, (int)M, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(&num, d_num, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        if (num > 0)
-            HIPCHK(ctx, hipMemcpyAsync(&last, a + num - 1, sizeof(last), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, sb, a, b, (int)M, 0, 2 * cb, ctx->stream));
+        // the distinct blocks of the sorted keys, in order, back into a
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, flag + M, 1, 0);
+        hipLaunchKernelGGL(k_pat_flag, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, b, (int)M, cb, flag);
+        HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, ub, flag, pos, (int)M + 1, ctx->stream));
+        hipLaunchKernelGGL(k_pat_compact, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, b, (int)M, flag, pos, a);
+        HIPCHK(ctx, hipMemcpyAsync(&num, pos + M, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     }
     int err = 0;
     HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (err) return fail(ctx, ICF_E_INDEX, "icf_build_pattern: a node id or altId is out of range");
-    const int n_nz = num - (last == PAT_NONE && num > 0 ? 1 : 0);
+    const int n_nz = num;
     *nnz = n_nz;
     if (n_nz > max_nnz) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: more blocks than max_nnz");
     const unsigned long long *u = (const unsigned long long *)ctx->pat_a.p;
     if (n_nz > 0)
-        hipLaunchKernelGGL(k_pat_cols, dim3(blocks_for(n_nz, 256)), dim3(256), 0, ctx->stream, u, n_nz, csr_cols);
-    hipLaunchKernelGGL(k_pat_rows, dim3(blocks_for((long long)N + 1, 256)), dim3(256), 0, ctx->stream, u, n_nz, N,
+        hipLaunchKernelGGL(k_pat_cols, dim3(blocks_for(n_nz, 256)), dim3(256), 0, ctx->stream, u, n_nz, cb, csr_cols);
+    hipLaunchKernelGGL(k_pat_rows, dim3(blocks_for((long long)N + 1, 256)), dim3(256), 0, ctx->stream, u, n_nz, N, cb,
                        csr_rows);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -2417,90 +2422,114 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     *nOut = 0;
     if (nTets == 0) return 0;
     const int nT = nTets;
-    // scratch: lo, hi (12 nT each), ext (nT), mn (3 nT), red (4), cell-ordered lo, hi (12 nT each)
-    if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)60 * nT + 4))) return r;
-    if (int r = grow(ctx, ctx->bp_k, sizeof(unsigned long long) * 2 * (size_t)nT)) return r;
-    if (int r = grow(ctx, ctx->bp_i, sizeof(int) * (6 * (size_t)nT + 4))) return r;
-    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2)) return r;
-    if (int r = grow(ctx, ctx->bp_c, sizeof(int) * (3 * (size_t)nT + 2))) return r;
     if (nT > (1 << 23)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^23 surface tets");
-    double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT, *ext = hi + (size_t)12 * nT,
-           *mn = ext + nT, *red = mn + (size_t)3 * nT;
-    double2 *skd = (double2 *)(red + 4);                  // 12 double2 = 24 doubles per tet
-    float4 *box = (float4 *)(red + 4 + (size_t)24 * nT);  // 2 float4 = 4 doubles per tet
-    auto *key = (unsigned long long *)ctx->bp_k.p, *skey = key + nT;
-    int *idx = (int *)ctx->bp_i.p, *sidx = idx + nT;
-    int4 *snode = (int4 *)(sidx + nT + ((4 - ((2 * nT) & 3)) & 3));   // 16-byte aligned, 4 ints per tet
-    int *d_err = ctx->small + 1;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+    // scratch: lo, hi (12 nT each), bounds (7), cell-ordered k-DOPs in FP64 (24 nT) and FP32
+    // (12 nT doubles' worth)
+    if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)60 * nT + 8))) return r;
+    if (int r = grow(ctx, ctx->bp_i, sizeof(int) * (8 * (size_t)nT + 4))) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2)) return r;
+    double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT;
+    auto *bounds = (unsigned long long *)(hi + (size_t)12 * nT);
+    double2 *skd = (double2 *)(hi + (size_t)12 * nT + 8);          // 12 double2 = 24 doubles per tet
+    float4 *sk32 = (float4 *)(hi + (size_t)36 * nT + 8);           // 6 float4 = 12 doubles per tet
+    auto *key = (unsigned *)ctx->bp_i.p;
+    int *rank = (int *)(key + nT), *sidx = rank + nT;
+    int4 *snode = (int4 *)(sidx + nT + ((4 - ((3 * nT) & 3)) & 3));   // 4 nT ints, 16-byte aligned
+    hipLaunchKernelGGL(k_bp_bounds_init, dim3(1), dim3(64), 0, ctx->stream, bounds);
     hipLaunchKernelGGL(k_bp_kdop, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, dn, nd_stride, ie,
-                       el_all_stride, nT, lo, hi, ext, mn, d_err, nd_stride);
-    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-    HIPCHK(ctx, hipcub::DeviceReduce::Max(nullptr, b1, ext, red, nT, ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceReduce::Min(nullptr, b2, mn, red + 1, nT, ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b3, key, skey, idx, sidx, nT, 0, 3 * BP_CELL_BITS,
-                                                   ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, (int *)nullptr, (int *)nullptr, nT, ctx->stream));
-    size_t tb = std::max(std::max(b1, b2), std::max(b3, b4));
-    if (int r = grow(ctx, ctx->scan_tmp, tb)) return r;
-    HIPCHK(ctx, hipcub::DeviceReduce::Max(ctx->scan_tmp.p, b1, ext, red, nT, ctx->stream));
-    for (int k = 0; k < 3; k++)
-        HIPCHK(ctx, hipcub::DeviceReduce::Min(ctx->scan_tmp.p, b2, mn + (size_t)k * nT, red + 1 + k, nT, ctx->stream));
-    hipLaunchKernelGGL(k_bp_cell, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, nT, red, key, idx, d_err);
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, b3, key, skey, idx, sidx, nT, 0,
-                                                   3 * BP_CELL_BITS, ctx->stream));
-    hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, sidx, ie,
-                       el_all_stride, skd, snode, box);
-    // cells of the sorted positions (key and first position), then per cell its pairs: counted,
-    // placed by a scan, written
-    int *flag = (int *)ctx->bp_c.p, *cidx = flag + nT, *cstart = cidx + nT;
-    auto *ckey = key;                                      // the unsorted keys are no longer needed
-    long long *cnt = (long long *)ctx->bp_l.p;
-    hipLaunchKernelGGL(k_bp_cellflag, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skey, nT, flag);
-    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b4, flag, cidx, nT, ctx->stream));
-    hipLaunchKernelGGL(k_bp_cells, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, skey, nT, flag, cidx, ckey,
-                       cstart);
-    int last[2] = {0, 0}, err = 0;
+                       el_all_stride, nT, lo, hi, bounds, nd_stride);
+    unsigned long long hb[BP_NBOUNDS + 1];
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(last, cidx + nT - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(last + 1, flag + nT - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    if (err) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id, or the grid exceeds 2^21 cells per axis");
-    const int ncells = last[0] + last[1];
-    // pairs per cell, collected in LDS and appended in batches; a second search when they
-    // outgrew the buffer
-    auto *tot = (unsigned long long *)cnt;
+    if (hb[BP_NBOUNDS]) return fail(ctx, ICF_E_INDEX, "icf_broad_phase: bad node id");
+    // the grid: cells at least the largest extent, coarsened (any size >= the extents finds
+    // the same pairs) until the margin grid has at most 2^24 cells
+    BpGrid g;
+    const double ext = bp_unord(hb[0]);
+    double span[3];
+    for (int k = 0; k < 3; k++) {
+        g.org[k] = bp_unord(hb[1 + k]);
+        span[k] = bp_unord(hb[4 + k]) - g.org[k];
+        if (!std::isfinite(g.org[k]) || !std::isfinite(span[k]))
+            return fail(ctx, ICF_E_INVALID, "icf_broad_phase: non-finite node positions");
+    }
+    g.cs = ext > 0 ? ext * 1.000001 : 1.0;
+    for (;;) {
+        double cells = 1;
+        for (int k = 0; k < 3; k++) {
+            const double c = std::floor(span[k] / g.cs) + 1;
+            g.n[k] = c < (double)(1 << 24) ? (int)c : (1 << 24);
+            cells *= (double)g.n[k] + (k == 0 ? 3 : 2);
+        }
+        if (cells <= (double)(1 << 24)) break;
+        g.cs *= 1.25;
+    }
+    g.gx = g.n[0] + 3;
+    g.gy = g.n[1] + 2;
+    g.gz = g.n[2] + 2;
+    const int G = g.gx * g.gy * g.gz;
+    // cell counts, their exclusive scan (first), and the tets' cell-ordered copies
+    if (int r = grow(ctx, ctx->bp_c, sizeof(int) * 2 * ((size_t)G + 1))) return r;
+    int *count = (int *)ctx->bp_c.p, *first = count + G + 1;
+    size_t b3 = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b3, count, first, G + 1, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, b3)) return r;
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for((long long)G + 1, 256)), dim3(256), 0, ctx->stream, count, G + 1,
+                       0);
+    hipLaunchKernelGGL(k_bp_cell, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, nT, g, key, rank, count);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b3, count, first, G + 1, ctx->stream));
+    hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, key, rank,
+                       first, ie, el_all_stride, sidx, skd, snode, sk32);
+    // candidate pairs per cell (FP32 prefilter), collected in LDS and appended in batches; a
+    // second search when they outgrew the buffer
+    const int ncell = g.n[0] * g.n[1] * g.n[2];
+    auto *tot = (unsigned long long *)ctx->bp_l.p;
     long long cap = std::max<long long>(ctx->bp_cap, 4LL * nT);
-    long long total = 0;
+    long long ncand = 0;
     for (int pass = 0; pass < 2; pass++) {
         if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)cap)) return r;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)tot, 2, 0);
-        hipLaunchKernelGGL(k_bp_cellpairs, dim3(ncells), dim3(BP_T), 0, ctx->stream, skd, nT, sidx, snode, box,
-                           exclude_shared, ckey, cstart, ncells, tot, (unsigned long long *)ctx->pat_a.p, cap);
+        hipLaunchKernelGGL(k_bp_cellwave, dim3((ncell + BPW_WAVES - 1) / BPW_WAVES), dim3(64 * BPW_WAVES), 0,
+                           ctx->stream, sk32, snode, exclude_shared, g, first, tot,
+                           (unsigned long long *)ctx->pat_a.p, cap);
         unsigned long long got = 0;
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipMemcpyAsync(&got, tot, sizeof(got), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        total = (long long)got;
-        if (total <= cap) break;
+        ncand = (long long)got;
+        if (ncand <= cap) break;
         if (pass == 1) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: pair count changed between two searches");
-        cap = total + total / 4;
+        cap = ncand + ncand / 4;
     }
-    ctx->bp_cap = std::max(ctx->bp_cap, total + total / 4);
-    *nOut = total;
-    if (total > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
-    if (total == 0) return 0;
-    if (total >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 pairs");
-    if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)total)) return r;
-    auto *pa = (unsigned long long *)ctx->pat_a.p, *pb = (unsigned long long *)ctx->pat_b.p;
+    ctx->bp_cap = std::max(ctx->bp_cap, ncand + ncand / 4);
+    if (ncand == 0) return 0;
+    if (ncand >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 candidate pairs");
+    // the exact test, then the passing pairs grouped by a (a counting sort) and placed by
+    // their rank of b within the group: the pairs in (a, b) order
+    if (int r = grow(ctx, ctx->bp_c, sizeof(int) * (2 * ((size_t)nT + 1) + (size_t)ncand))) return r;
+    if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)ncand)) return r;
+    int *acount = (int *)ctx->bp_c.p, *start = acount + nT + 1, *rnk = start + nT + 1;
+    auto *pa = (unsigned long long *)ctx->pat_a.p, *grp = (unsigned long long *)ctx->pat_b.p;
     size_t b5 = 0;
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, b5, pa, pb, (int)total, 0, 64, ctx->stream));
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b5, acount, start, nT + 1, ctx->stream));
     if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, b5, pa, pb, (int)total, 0, 64, ctx->stream));
-    hipLaunchKernelGGL(k_bp_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, pb, total, pairs);
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for((long long)nT + 1, 256)), dim3(256), 0, ctx->stream, acount,
+                       nT + 1, 0);
+    hipLaunchKernelGGL(k_bp_exact, dim3(blocks_for(ncand, 256)), dim3(256), 0, ctx->stream, skd, sk32, sidx, ncand,
+                       pa, rnk, acount);
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b5, acount, start, nT + 1, ctx->stream));
+    hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(ncand, 256)), dim3(256), 0, ctx->stream, pa, rnk, ncand, start,
+                       grp);
+    if (max_pairs > 0)
+        hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(ncand, 256)), dim3(256), 0, ctx->stream, grp, start, nT,
+                           max_pairs, pairs);
+    int nv = 0;
     HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(&nv, start + nT, sizeof(nv), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *nOut = (long long)nv;
+    if ((long long)nv > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
     return 0;
 }
 
@@ -2529,30 +2558,34 @@ int icf_nl2set(icf_ctx *ctx, const int *masks, const int *pairs, int nPairs, con
     HIPCHK(ctx, hipMemcpyAsync(&total, off + nPairs, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (total == 0) return 0;
-    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * 2 * (size_t)total)) return r;
-    if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * 2 * (size_t)total)) return r;
-    if (int r = grow(ctx, ctx->bp_l, sizeof(int) * 4 * (size_t)total + sizeof(int))) return r;
-    auto *k0 = (unsigned long long *)ctx->pat_a.p, *k1 = k0 + total;
-    auto *k2 = (unsigned long long *)ctx->pat_b.p, *k3 = k2 + total;
-    int *p0 = (int *)ctx->bp_l.p, *p1 = p0 + total, *p2 = p1 + total, *p3 = p2 + total, *d_num = p3 + total;
+    if (total > (1 << 28)) return fail(ctx, ICF_E_CAPACITY, "icf_nl2set: more than 2^28 tuples");
+    // the table: a power of two at least twice the tuples
+    unsigned long long H = 1;
+    while (H < 2ULL * (unsigned long long)total) H <<= 1;
+    if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * ((size_t)total + H))) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(int) * ((size_t)H + 2 * (size_t)total + 2))) return r;
+    auto *key = (unsigned long long *)ctx->pat_a.p, *tkey = key + total;
+    int *tpos = (int *)ctx->bp_l.p, *keep = tpos + H, *dst = keep + total + 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for((long long)H, 256)), dim3(256), 0, ctx->stream, tkey, (long long)H,
+                       NL2_EMPTY);
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for((long long)H, 256)), dim3(256), 0, ctx->stream, tpos, (int)H,
+                       INT_MAX);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, keep + total, 1, 0);
     hipLaunchKernelGGL(k_nl2_emit, dim3(blocks_for(nPairs, 256)), dim3(256), 0, ctx->stream, masks, pairs, nPairs, ie,
-                       el_all_stride, off, k0, p0);
-    size_t s1 = 0, s2 = 0, s3 = 0;
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, s1, k0, k1, p0, p1, total, 0, 64, ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceSelect::UniqueByKey(nullptr, s2, k1, p1, k2, p2, d_num, total, ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, s3, p2, p3, k2, k3, total, 0, 32, ctx->stream));
-    if (int r = grow(ctx, ctx->scan_tmp, std::max(s1, std::max(s2, s3)))) return r;
-    // by key, stable (equal keys keep insertion order), then the first of each key, then back
-    // into insertion order
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, s1, k0, k1, p0, p1, total, 0, 64, ctx->stream));
-    HIPCHK(ctx, hipcub::DeviceSelect::UniqueByKey(ctx->scan_tmp.p, s2, k1, p1, k2, p2, d_num, total, ctx->stream));
+                       el_all_stride, off, key, tkey, tpos, H - 1);
+    hipLaunchKernelGGL(k_nl2_keep, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, key, total, tkey, tpos,
+                       H - 1, keep);
+    size_t s1 = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, s1, keep, dst, total + 1, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, s1)) return r;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, s1, keep, dst, total + 1, ctx->stream));
     int num = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&num, d_num, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&num, dst + total, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     *nOut = num;
     if (num > tet_stride) return fail(ctx, ICF_E_CAPACITY, "icf_nl2set: more tuples than tet_stride");
-    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->scan_tmp.p, s3, p2, p3, k2, k3, num, 0, 32, ctx->stream));
-    hipLaunchKernelGGL(k_nl2_write, dim3(blocks_for(num, 256)), dim3(256), 0, ctx->stream, k3, num, nl2, tet_stride);
+    hipLaunchKernelGGL(k_nl2_write, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, key, total, keep, dst,
+                       nl2, tet_stride);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;

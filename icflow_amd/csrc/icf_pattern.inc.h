@@ -19,7 +19,9 @@
 
 namespace icf {
 
-constexpr unsigned long long PAT_NONE = ~0ULL;
+// Keys (row << cb | column) with cb bits per index, (1 << cb) > N; the largest 2cb-bit value
+// marks a pair that adds no block and sorts after every block.
+__host__ __device__ inline unsigned long long pat_none(int cb) { return (1ULL << (2 * cb)) - 1; }
 
 // One unit source of the pattern: node ids base[u + stride * (field0 + i)], i < K.
 struct PatSrc {
@@ -33,7 +35,8 @@ struct PatSrc {
 __host__ __device__ inline int pat_pairs(int K, int strict) { return strict ? K * (K - 1) / 2 : K * (K + 1) / 2; }
 
 __global__ __launch_bounds__(256) void k_pat_emit(PatSrc s, const int *__restrict__ alt, int nNodes, int N,
-                                                  unsigned long long *__restrict__ keys, int *__restrict__ err)
+                                                  int cb, unsigned long long *__restrict__ keys,
+                                                  int *__restrict__ err)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int P = pat_pairs(s.K, s.strict);
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(256) void k_pat_emit(PatSrc s, const int *__restric
     const int j = s.strict ? i + 1 + p : i + p;
     const int ni = s.base[u + (size_t)s.stride * (s.field0 + i)];
     const int nj = s.base[u + (size_t)s.stride * (s.field0 + j)];
-    unsigned long long key = PAT_NONE;
+    unsigned long long key = pat_none(cb);
     if (ni < 0 || ni >= nNodes || nj < 0 || nj >= nNodes) {
         atomicOr(err, 1);
     } else {
@@ -58,21 +61,40 @@ __global__ __launch_bounds__(256) void k_pat_emit(PatSrc s, const int *__restric
         if (a >= 0 && b >= 0 && !(s.strict && a == b)) {
             const int r = a < b ? a : b, c = a < b ? b : a;
             if (c >= N) atomicOr(err, 1);
-            else key = ((unsigned long long)r << 32) | (unsigned)c;
+            else key = ((unsigned long long)r << cb) | (unsigned)c;
         }
     }
     keys[s.out0 + t] = key;
 }
 
+// The sorted keys' distinct blocks: flag[i] = 1 where key i differs from key i-1 and is a
+// block (flag[M] = 0 is set by the caller); then each flagged key goes to its scan position.
+__global__ __launch_bounds__(256) void k_pat_flag(const unsigned long long *__restrict__ keys, int M, int cb,
+                                                  int *__restrict__ flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const unsigned long long k = keys[i];
+    flag[i] = (k != pat_none(cb) && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_pat_compact(const unsigned long long *__restrict__ keys, int M,
+                                                     const int *__restrict__ flag, const int *__restrict__ pos,
+                                                     unsigned long long *__restrict__ ukeys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M && flag[i]) ukeys[pos[i]] = keys[i];
+}
+
 // csr_cols[k] = column of unique key k (k < nnz); csr_rows[r] = first k with row >= r.
-__global__ __launch_bounds__(256) void k_pat_cols(const unsigned long long *__restrict__ ukeys, int nnz,
+__global__ __launch_bounds__(256) void k_pat_cols(const unsigned long long *__restrict__ ukeys, int nnz, int cb,
                                                   int *__restrict__ cols)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nnz) cols[k] = (int)(ukeys[k] & 0xffffffffULL);
+    if (k < nnz) cols[k] = (int)(ukeys[k] & ((1ULL << cb) - 1));
 }
 
-__global__ __launch_bounds__(256) void k_pat_rows(const unsigned long long *__restrict__ ukeys, int nnz, int N,
+__global__ __launch_bounds__(256) void k_pat_rows(const unsigned long long *__restrict__ ukeys, int nnz, int N, int cb,
                                                   int *__restrict__ rows)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,7 +102,7 @@ __global__ __launch_bounds__(256) void k_pat_rows(const unsigned long long *__re
     int lo = 0, hi = nnz;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if ((int)(ukeys[mid] >> 32) < r) lo = mid + 1;
+        if ((int)(ukeys[mid] >> cb) < r) lo = mid + 1;
         else hi = mid;
     }
     rows[r] = lo;

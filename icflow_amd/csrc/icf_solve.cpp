@@ -927,6 +927,23 @@ extern "C" int icfh_solve_threads(int threads)
     return 0;
 }
 
+extern "C" int icfh_solve_analyse(const int *ja, const int *ia, int n, long long *factor_doubles,
+                                  long long *peak_doubles, double *flops)
+{
+    if (n <= 0 || !ia || !ja || !factor_doubles || !peak_doubles || !flops) return -1;
+    try {
+        int err = 0;
+        std::shared_ptr<const Analysis> an = get_analysis(ia, ja, n, &err);
+        if (!an) return err;
+        *factor_doubles = an->lsize;
+        *peak_doubles = an->lsize + (long long)an->stack_peak + (long long)an->max_front + (long long)an->max_wbuf;
+        *flops = an->flops;
+    } catch (const std::bad_alloc &) {
+        return -2;
+    }
+    return 0;
+}
+
 extern "C" int SolveDouble3(int *ja, int *ia, double *a, const int n, double *b, double *x, int mtype, int iparam4,
                             int dim, int msglvl, int check)
 {

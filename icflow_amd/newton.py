@@ -957,8 +957,12 @@ def precision_study(consts, steps=20, cells=(40, 8, 8), grains=16, seed=0, h=1e-
 
 
 def main(argv=None):
-    """python -m icflow_amd.newton [--steps N] [--ramp R] [--adapt]: the precision study as one
-    JSON line (GPU required)."""
+    """python -m icflow_amd.newton [--steps N] [--ramp R] [--adapt] [--self-contact] [--batched S]:
+    the FP64-vs-mixed precision study as one JSON line (GPU required).  Examples: the 40x8x8
+    beam for 200 steps, 2 mm over 200 steps, with time-step control: --steps 200 --ramp 200
+    --adapt; the same beam with self-contact for 300 steps: --steps 300 --ramp 200 --adapt
+    --self-contact; the 256 batched specimens, 5 fixed steps to 2e-3 strain over 4:
+    --batched 256 --steps 5 --ramp 4 --strain 2e-3."""
     import argparse
     import json
     ap = argparse.ArgumentParser()
@@ -968,16 +972,25 @@ def main(argv=None):
     ap.add_argument("--max-attempts", type=int, default=4000)
     ap.add_argument("--conv-eps", type=float, default=0.005, help="Newton tolerance (ConvergenceEpsilon)")
     ap.add_argument("--max-iter", type=int, default=10, help="Newton iterations per step at most")
+    ap.add_argument("--conv-cutoff", type=float, default=1e-8, help="absolute |dx|^2 cutoff (ConvergenceCutoff)")
     ap.add_argument("--contact", action="store_true",
                     help="platen compression with contact (platen_compression) instead of the three-point bend")
+    ap.add_argument("--self-contact", action="store_true",
+                    help="the three-point bend with self-contact (exterior and CZ faces)")
+    ap.add_argument("--batched", type=int, default=0, help="batched specimens in tension instead (count)")
+    ap.add_argument("--strain", type=float, default=3e-4, help="end strain of the batched tension")
     a = ap.parse_args(argv)
     consts = _lib.consts_from_prms(**_lib.SPECIMEN_PRMS)
-    if a.contact:
+    if a.batched:
+        r = batched_precision_study(consts, n_specimens=a.batched, steps=a.steps, strain=a.strain,
+                                    ramp_steps=a.ramp, log=lambda m: print(m, flush=True))
+    elif a.contact:
         r = contact_precision_study(consts, steps=a.steps, adapt=a.adapt, max_attempts=a.max_attempts,
                                     log=lambda m: print(m, flush=True))
     else:
         r = precision_study(consts, steps=a.steps, ramp_steps=a.ramp, adapt=a.adapt, max_attempts=a.max_attempts,
-                            conv_eps=a.conv_eps, max_iter=a.max_iter, log=lambda m: print(m, flush=True))
+                            conv_eps=a.conv_eps, max_iter=a.max_iter, conv_cutoff=a.conv_cutoff,
+                            contact=a.self_contact, log=lambda m: print(m, flush=True))
     print(json.dumps(r), flush=True)
 
 

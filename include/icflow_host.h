@@ -95,6 +95,15 @@ int SolveDouble3(int *ja, int *ia, double *a, const int n, double *b, double *x,
  * for a negative count. */
 int icfh_solve_threads(int threads);
 
+/* The ordering and symbolic analysis SolveDouble3 would do for the pattern (ja, ia, n) alone,
+ * kept in the same cache, with the sizes it implies -- the figures PARDISO reports after its
+ * analysis phase (phase 11: factor size, peak memory, factorisation flops), which the
+ * reference's wrapper does not expose: doubles of L's panels, doubles at the factorisation's
+ * peak (panels, update stack, largest front and panel), floating-point operations of the
+ * factorisation.  Returns 0, -1 for bad arguments, -2 out of memory, -3 ordering failed. */
+int icfh_solve_analyse(const int *ja, const int *ia, int n, long long *factor_doubles,
+                       long long *peak_doubles, double *flops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -220,3 +220,42 @@ def test_batched_solver_per_specimen(orc, lib, specimen_consts, beam_system):
         bs.close()
     with pytest.raises(ValueError):
         newton.BatchedSolver(rows, cols, [0, row_off[1] - 1, row_off[-1]])
+
+
+def _analyse(lib, rows, cols):
+    import ctypes
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    fd, pk, fl = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+    r = lib.icfh_solve_analyse(ptr(cols), ptr(rows), len(rows) - 1, ctypes.byref(fd), ctypes.byref(pk),
+                               ctypes.byref(fl))
+    return r, fd.value, pk.value, fl.value
+
+
+def test_analysis_sizes(lib, beam_system):
+    """icfh_solve_analyse: a fully coupled pattern of n block rows is one dense front -- (3n)^2
+    doubles of panel and (3n)^3 multiply-adds (M K^2 with M = K = 3n); a block-diagonal pattern
+    has no fill (9 doubles and 27 multiply-adds per block row); the beam's figures are
+    consistent (peak >= factor), and a solve after the analysis (the cached analysis) still
+    matches the dense solve."""
+    n = 10
+    rows = np.arange(n + 1, dtype=np.int32) * 0
+    cols = []
+    for r in range(n):
+        cols += list(range(r, n))
+        rows[r + 1] = len(cols)
+    r, fd, pk, fl = _analyse(lib, rows, np.array(cols, np.int32))
+    assert r == 0 and fd == (3 * n) ** 2 and fl == (3 * n) ** 3 and pk >= fd
+    diag_rows, diag_cols = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    r, fd, pk, fl = _analyse(lib, diag_rows, diag_cols)
+    assert r == 0 and fd == 9 * n and fl == 27 * n
+    brows, bcols, A, b = beam_system
+    r, fd, pk, fl = _analyse(lib, brows, bcols)
+    assert r == 0 and fd > 9 * (len(brows) - 1) and pk >= fd and fl > 0
+    r, x = _solve(lib, brows, bcols, A, b)
+    assert r == 0
+    xd = np.linalg.solve(_dense(brows, bcols, A), b)
+    assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
+    import ctypes
+    z = ctypes.c_longlong()
+    assert lib.icfh_solve_analyse(None, None, n, ctypes.byref(z), ctypes.byref(z), None) == -1
