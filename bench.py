@@ -180,6 +180,31 @@ class Runner:
         chk(lib.icf_collision_response(ctx, self.icr.p, gf.g_dn.p, self.h, self.vals.p, self.rhs.p, self.nT,
                                        gf.cr_stride, gf.nd_stride, self.k, self.eps), ctx, "icf_collision_response")
 
+    def transfers(self):
+        """Host buffers for the per-iteration copies of the reference's sequence
+        (ImplicitModel3.cs:381-397): up, the node buffer (TransferNodesToGPU), the elements' and
+        CZs' pcsr arrays and the CZ history (TransferPCSR, GPU_Functionality.cs:345-416) and the
+        impacts' rows (collisionResponse); down, the values and rhs
+        (TransferLinearSystemToHost)."""
+        gf = self.gf
+        up = [(gf.g_dn, gf.c_dn), (gf.g_ie_pcsr, gf.c_ie_pcsr)]
+        if gf.cz_stride:
+            up += [(gf.g_icz, gf.c_icz), (gf.g_dcz, gf.c_dcz)]
+        if self.nT:
+            up.append((self.icr, gf.c_icr))
+        self.up = [(d, np.ascontiguousarray(h)) for d, h in up]
+        self.down = [(self.vals, np.empty(9 * gf.csr.nnz)), (self.rhs, np.empty(3 * gf.csr.N))]
+        return sum(h.nbytes for _, h in self.up), sum(h.nbytes for _, h in self.down)
+
+    def step_with_transfers(self):
+        """One drop-in iteration with its host copies: uploads, the device calls, downloads."""
+        lib, ctx, chk = self.lib, self.gf.ctx, self.check
+        for d, h in self.up:
+            chk(lib.icf_h2d(ctx, d.p, 0, _ptr(h), h.nbytes), ctx, "icf_h2d")
+        self.step()
+        for d, h in self.down:
+            chk(lib.icf_d2h(ctx, _ptr(h), d.p, 0, h.nbytes), ctx, "icf_d2h")
+
     def failed_czs(self):
         d, f = ctypes.c_int(), ctypes.c_int()
         self.check(self.lib.icf_cz_counts(self.gf.ctx, ctypes.byref(d), ctypes.byref(f)), self.gf.ctx, "icf_cz_counts")
@@ -214,7 +239,8 @@ def _ptr(a):
 
 def timed(runner, steps, warmup, how):
     """Seconds per step.  how = "dropin": blocking calls (each returns after its work, as the
-    reference's Run()); "graph": replays of one captured iteration graph; "async": the calls
+    reference's Run()); "transfers": the same with the reference's per-iteration host copies
+    around them (Runner.transfers); "graph": replays of one captured iteration graph; "async": the calls
     enqueued without waiting (icf_set_blocking(0)); "profile": the calls
     enqueued without waiting, each kernel bracketed by HIP events on the library's stream (the
     library runs the fused call on one stream while profiling, so each kernel is timed alone)."""
@@ -227,6 +253,10 @@ def timed(runner, steps, warmup, how):
     elif how in ("profile", "async"):
         lib.icf_set_blocking(ctx, 0)
         fn = runner.step
+    elif how == "transfers":
+        lib.icf_set_blocking(ctx, 1)
+        runner.transfers()
+        fn = runner.step_with_transfers
     else:
         lib.icf_set_blocking(ctx, 1)
         fn = runner.step
@@ -380,8 +410,11 @@ def _rocprof_symbol(name):
 
 def key_limit(gf, name):
     """The chunk shape (workgroup size, staged contributions per key) the autotuner chose for a
-    value-reduction kernel, or None for kernels without one."""
+    value-reduction kernel, the lanes per tuple of the closest-face search, or None for kernels
+    without one."""
     base = name.split("<")[0]
+    if base == "k_closest_face":      # templated on its lanes per tuple, a tuned launch shape
+        return gf.shape_state()["closest"][0]
     if base not in ("k_vals_force", "k_vals_force_rhs", "k_contact_sums", "k_reduce_vals"):
         return None
     stage = "fp64" if "fp64" in name else "fp32"
@@ -389,11 +422,19 @@ def key_limit(gf, name):
     return gf.autotune_state()[f"{'accumulate' if acc else 'overwrite'},{stage}"][0]
 
 
+def _shaped_symbol(name, shape):
+    """The rocprofv3 symbol of a kernel with its tuned shape: (workgroup size, staging) as the
+    last two template arguments, or one integer template argument."""
+    if isinstance(shape, int):
+        return f"{_rocprof_symbol(name)}<{shape}>"
+    return _rocprof_symbol(name) + (f", {shape[0]}, {shape[1]}>" if shape else "")
+
+
 def profiled_traffic(name, shape=None, path=os.path.join(ROOT, "profiles", "assembly_compression40_kernels.md")):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-probe factors), or None.  shape: the
     (workgroup size, staging) of a kernel templated on them (its last two template arguments)."""
-    sym = _rocprof_symbol(name) + (f", {shape[0]}, {shape[1]}>" if shape else "")
+    sym = _shaped_symbol(name, shape)
     try:
         for line in open(path):
             head = f"| `{sym}"
@@ -407,7 +448,7 @@ def profiled_traffic(name, shape=None, path=os.path.join(ROOT, "profiles", "asse
 def profiled_valu(name, shape=None, path=None):
     """VALU wave-instructions per dispatch of a kernel from the committed rocprofv3 summary
     (SQ_INSTS_VALU of the wave-counter pass), or None."""
-    sym = _rocprof_symbol(name) + (f", {shape[0]}, {shape[1]}>" if shape else "")
+    sym = _shaped_symbol(name, shape)
     try:
         for line in open(path):
             head = f"| `{sym}"
@@ -540,6 +581,37 @@ def device_iteration_timing(r, reps=5):
     return {"ms_per_iteration": round(sum(out.values()), 3), "parts_ms": out}
 
 
+def host_solve_once(r):
+    """One FP64 iteration assembled on the device, then solved once on the host by SolveDouble3
+    (the reference's PardisoLoader2 signature, not PARDISO), timed apart from the assembly:
+    the analysis alone (icfh_solve_analyse: factor size, peak memory, multiply-adds), then the
+    call (factorisation and substitution, the analysis kept), and the relative residual."""
+    from icflow_amd import _lib, newton
+    lib, gf = r.lib, r.gf
+    r.set_mode(_lib.ICF_FP64)
+    r.step()
+    N, nnz = gf.csr.N, gf.csr.nnz
+    A = r.vals.download(9 * nnz).copy()
+    b = r.rhs.download(3 * N).copy()
+    rows = np.ascontiguousarray(gf.csr.rows, np.int32)
+    cols = np.ascontiguousarray(gf.csr.cols, np.int32)
+    fd, pk, fl = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+    t = time.time()
+    rc = lib.icfh_solve_analyse(_ptr(cols), _ptr(rows), N, ctypes.byref(fd), ctypes.byref(pk), ctypes.byref(fl))
+    ta = time.time() - t
+    log(f"host solve: analysis {ta:.1f} s, factor {fd.value * 8e-9:.1f} GB, {fl.value * 1e-9:.0f} G multiply-adds")
+    x = np.zeros_like(b)
+    t = time.time()
+    rc2 = lib.SolveDouble3(_ptr(cols), _ptr(rows), _ptr(A), N, _ptr(b), _ptr(x), -2, 0, 3, 2, 1)
+    ts = time.time() - t
+    M = newton.upper_bcsr_to_csc(rows, cols, A)
+    res = float(np.linalg.norm(M @ x - b) / np.linalg.norm(b))
+    return {"rc": [int(rc), int(rc2)], "block_rows": int(N), "blocks": int(nnz), "analysis_s": round(ta, 2),
+            "factorise_and_solve_s": round(ts, 1), "factor_GB": round(fd.value * 8e-9, 2),
+            "peak_GB": round(pk.value * 8e-9, 2), "gmadds": round(fl.value * 1e-9, 1), "rel_residual": res,
+            "threads": host_threads()}
+
+
 def host_threads():
     """Threads for the all-core CPU baseline: OMP_NUM_THREADS when set (the GPU box sets it to
     the CPU share of its one GPU), else the CPUs this process may run on."""
@@ -619,6 +691,7 @@ def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
         t = {how: timed(r, args.steps, args.warmup, how) for how in ("async", "graph", "profile")}
         issue = min(("async", "graph"), key=lambda h: t[h][0])
         res[mname] = (issue, t[issue][0], t["profile"][1])
+    xfer = timed(r, min(args.steps, 5), 1, "transfers")[0] if "transfers" in args.timings.split(",") else None
     T, C, P = len(col.elastic), len(col.czs), r.nT
     units = T + C - r.failed_czs() + P
     alg = algorithmic_bytes(r)
@@ -642,6 +715,8 @@ def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
            "fp64": {"value": round(units / res["fp64"][1] / 1e6, 3), "ms_per_step": round(res["fp64"][1] * 1e3, 4),
                     "issue": res["fp64"][0]},
            "mixed_over_fp64": round(res["fp64"][1] / sec, 3),
+           "dropin_with_transfers_ms_per_step": round(xfer * 1e3, 3) if xfer else None,
+           "transfer_bytes_per_step": dict(zip(("host_to_device", "device_to_host"), r.transfers())) if xfer else None,
            "iteration_roofline_frac": round(iteration_bytes(r) / sec / 1e9 / HBM_PEAK_GBS, 4),
            "kernels_us": {k: round(1e3 * ms / max(n, 1), 2) for k, (ms, n) in prof.items()},
            "kernels_util": util,
@@ -649,6 +724,9 @@ def measure_extra(name, consts, col, st, contact, args, traffic_md=None):
                         "achieved": round(dbytes / avg_s / 1e9, 1),
                         "frac": round(dbytes / avg_s / 1e9 / HBM_PEAK_GBS, 4),
                         "traffic": int(traffic) if traffic else None}}
+    if getattr(args, "solve_once", False):
+        out["host_solve"] = host_solve_once(r)
+        log(f"{name}: host solve {out['host_solve']}")
     if contact:
         out["device_broad_phase"] = broad_phase_timing(r)
         log(f"{name}: device broad phase {out['device_broad_phase']}")
@@ -704,10 +782,12 @@ def main():
     ap.add_argument("--no-overlap", action="store_true",
                     help="icf_set_overlap(0): one kernel at a time, on the library's stream (no fused "
                          "launches, no second stream)")
-    ap.add_argument("--timings", default="dropin,async,graph,profile",
+    ap.add_argument("--timings", default="dropin,async,graph,profile,transfers",
                     help="ways of issuing the timed steps (profile is always run: it gives the kernel times)")
     ap.add_argument("--tune-rounds", type=int, default=3,
                     help="icf_set_autotune rounds before timing each mode (0: keep the default key limit)")
+    ap.add_argument("--solve-once", action="store_true",
+                    help="for each extra workload, one FP64 iteration solved on the host by SolveDouble3, timed")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run icf_d2d over this many MiB (a plain copy of known size, for "
                          "calibrating hardware byte counters)")
@@ -728,6 +808,8 @@ def main():
     if args.no_overlap:
         r.check(r.lib.icf_set_overlap(r.gf.ctx, 0), r.gf.ctx, "icf_set_overlap")
     hows = [h for h in ("dropin", "async", "graph") if h in args.timings.split(",")] + ["profile"]
+    if "transfers" in args.timings.split(","):
+        hows.append("transfers")
     if not ({"async", "graph"} & set(hows)):
         hows.insert(0, "graph")
 
@@ -754,7 +836,8 @@ def main():
         if args.tune_rounds > 0:
             tuned[name] = autotune(r, args.tune_rounds, name)
         log(f"timing {name}: {args.warmup} warmup + {args.steps} steps each way")
-        res = {how: timed(r, args.steps, args.warmup, how) for how in hows}
+        res = {how: timed(r, min(args.steps, 5) if how == "transfers" else args.steps,
+                          min(args.warmup, 1) if how == "transfers" else args.warmup, how) for how in hows}
         out[name] = res
         prof = res["profile"][1]
         for kname, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
@@ -797,7 +880,8 @@ def main():
         profiled = args.specimens == 256
         md = os.path.join(ROOT, "profiles", "assembly_batched256_kernels.md")
     traffic = profiled_traffic(dname, dkeys, md) if profiled else None
-    roof = dict(kernel=dname + (f" ({dkeys[0]}-lane workgroups, {dkeys[1]} staged per key)" if dkeys else ""),
+    roof = dict(kernel=dname + (f" ({dkeys[0]}-lane workgroups, {dkeys[1]} staged per key)"
+                                if isinstance(dkeys, tuple) else ""),
                 bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=int(traffic) if traffic else None,
                 alg_bytes=int(dbytes), avg_us=round(avg_s * 1e6, 2),
@@ -918,6 +1002,10 @@ def main():
         "graph_ms_per_step": {k: round(out[k]["graph"][0] * 1e3, 4) for k in out if "graph" in out[k]},
         "async_ms_per_step": {k: round(out[k]["async"][0] * 1e3, 4) for k in out if "async" in out[k]},
         "dropin_ms_per_step": {k: round(out[k]["dropin"][0] * 1e3, 4) for k in out if "dropin" in out[k]},
+        "dropin_with_transfers_ms_per_step": (
+            {k: round(out[k]["transfers"][0] * 1e3, 3) for k in out if "transfers" in out[k]} or None),
+        "transfer_bytes_per_step": (dict(zip(("host_to_device", "device_to_host"), r.transfers()))
+                                    if "transfers" in hows else None),
         "fp64": ({"value": round(units / best("fp64")[1] / 1e6, 3), "ms_per_step": round(best("fp64")[1] * 1e3, 4),
                   "issue": best("fp64")[0]} if "fp64" in out else None),
         "mixed_over_fp64": (round(best("fp64")[1] / best("mixed")[1], 3)

@@ -739,9 +739,14 @@ struct Factor {
 
 constexpr int NB = 32;   // block columns factorised together before the rest of a panel is updated
 
-int factorise(const Analysis &an, const double *a, Factor &fx)
+int factorise(const Analysis &an, const double *a, Factor &fx, int msglvl = 0)
 {
     const int n = an.n;
+    // msglvl > 1: a progress line at every tenth of the factorisation's flops
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    double done = 0;
+    int tenth = 1;
     double amax = 0;
     for (long long i = 0; i < (long long)an.ia[n] * 9; i++) amax = std::max(amax, std::fabs(a[i]));
     const double delta = 1e-8 * (amax > 0 ? amax : 1.0);
@@ -856,6 +861,12 @@ int factorise(const Analysis &an, const double *a, Factor &fx)
             std::memmove(U + (size_t)Mu * y, F + (size_t)M * (K + y) + K, sizeof(double) * Mu);
         upd[s] = (size_t)Mu * Mu;
         sp = base + upd[s];
+        if (msglvl > 1) {
+            done += (double)M * K * K + (double)(M - K) * (M - K) * K;
+            for (; tenth <= 10 && done >= an.flops * tenth / 10; tenth++)
+                std::fprintf(stderr, "SolveDouble3: factorisation %d%% (%.1f s)\n", 10 * tenth,
+                             std::chrono::duration<double>(clk::now() - t0).count());
+        }
     }
     return 0;
 }
@@ -969,7 +980,7 @@ extern "C" int SolveDouble3(int *ja, int *ia, double *a, const int n, double *b,
         const Analysis &an = *anp;
         const auto t1 = clk::now();
         Factor fx;
-        if (int r = factorise(an, a, fx)) return r;
+        if (int r = factorise(an, a, fx, msglvl)) return r;
         const auto t2 = clk::now();
         // solve, with iterative refinement after perturbed pivots
         std::vector<double> w, r((size_t)n * 3), c((size_t)n * 3);

@@ -82,7 +82,8 @@ int icfh_kdop_pairs(const double *dn, int nd_stride, const int *ie, int el_all_s
  * block graph, whichever fills less; 3x3 pivot blocks, small pivots perturbed by 1e-8 of the
  * largest entry followed by iterative refinement).  mtype must be -2
  * (symmetric indefinite, the reference's value) or 2, dim must be 3; iparam4 is ignored (the
- * reference passes 0); msglvl > 0 prints sizes and timings to stderr; check != 0 verifies the
+ * reference passes 0); msglvl > 0 prints sizes and timings to stderr (msglvl > 1 also a progress
+ * line per tenth of the factorisation); check != 0 verifies the
  * structure first.  Returns 0, or PARDISO's codes: -1 inconsistent input, -2 out of memory,
  * -3 ordering failed, -4 zero pivot.  x is overwritten; b is not changed.  Reentrant: threads
  * may solve different systems at once; the analyses of the last 1024 patterns are kept. */

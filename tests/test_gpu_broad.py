@@ -63,8 +63,9 @@ def test_device_broad_phase_and_nl2set_match_host(specimen_consts, exclude_share
 
 def test_device_broad_phase_crowded_cells():
     """100 tets on top of one another (distinct node ids) and 20 apart: every one of the 100
-    overlaps every other, so some tets have more partners than the counting pass keeps per
-    thread, which takes the second (writing) pass.  Same pairs, same order as the host grid."""
+    overlaps every other, so one cell holds more own tets than a wave stages at once and more
+    pairs than a wave's buffer holds before it flushes.  Same pairs, same order as the host
+    grid."""
     from icflow_amd.gpu_functionality import GPUFunctionality
     rng = np.random.default_rng(3)
     unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.01
@@ -97,4 +98,42 @@ def test_device_broad_phase_crowded_cells():
                    "broad_phase")
         assert n.value == Q
         assert np.array_equal(d_pairs.download(2 * Q).reshape(-1, 2), host)
+    gf.close()
+
+
+def test_device_broad_phase_sparse_scene_coarsens_the_grid():
+    """Five clusters of small tets at the corners of a 3 m cube: cells of the largest extent
+    (1 cm) would need about 300^3 grid cells, more than the 2^24 the device grid takes, so the
+    cells are coarsened; the pairs (within the clusters) are the host grid's, in order."""
+    from icflow_amd.gpu_functionality import GPUFunctionality
+    rng = np.random.default_rng(5)
+    unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.01
+    corners = [np.array(c, float) for c in ((0, 0, 0), (3, 0, 0), (0, 3, 0), (0, 0, 3), (3, 3, 3))]
+    tets = [unit + c + 0.004 * rng.random(3) for c in corners for _ in range(12)]
+    nT = len(tets)
+    nN = 4 * nT
+    ns, es = stride_of(nN), stride_of(nT)
+    c_dn = np.zeros(18 * ns)
+    for t, x in enumerate(tets):
+        for i in range(4):
+            for k in range(3):
+                c_dn[4 * t + i + ns * (15 + k)] = x[i, k]
+    c_ie = np.zeros(5 * es, np.int32)
+    for t in range(nT):
+        for i in range(4):
+            c_ie[t + es * i] = 4 * t + i
+    gf = GPUFunctionality(0)
+    lib, ctx = gf.lib, gf.ctx
+    d_dn, d_ie = gf.buf("sp_dn", c_dn.size, np.float64), gf.buf("sp_ie", c_ie.size, np.int32)
+    d_dn.upload(c_dn)
+    d_ie.upload(c_ie)
+    host = broad_phase(gf, c_dn, ns, c_ie, es, nT, exclude_shared=True)
+    Q = len(host)
+    assert 0 < Q <= 5 * 12 * 11 // 2
+    d_pairs = gf.buf("sp_pairs", 2 * Q, np.int32)
+    n = ctypes.c_longlong()
+    _lib.check(lib.icf_broad_phase(ctx, d_dn.p, ns, d_ie.p, es, nT, 1, d_pairs.p, Q, ctypes.byref(n)), ctx,
+               "broad_phase")
+    assert n.value == Q
+    assert np.array_equal(d_pairs.download(2 * Q).reshape(-1, 2), host)
     gf.close()
