@@ -216,6 +216,13 @@ struct icf_ctx {
     DevConsts hc;
     DevBuf tet_rhs, tet_tan, cz_rhs, cz_tan, ct, scan_tmp;
     DevBuf pat_a, pat_b;           // pattern build / broad phase / NL2set: 64-bit keys
+    // the pattern's static part (elements and CZs): their distinct sorted keys, kept with a copy
+    // of the node-id fields and altIds they came from; reused while that copy still matches
+    struct PatCache {
+        bool valid = false;
+        int nNodes = 0, nElems = 0, el_stride = 0, nCZs = 0, cz_stride = 0, N = 0, cb = 0, nkeys = 0;
+        DevBuf snap, keys;
+    } patc;
     DevBuf bp_f, bp_i, bp_l;         // broad phase and NL2set scratch (doubles, ints, int64)
     DevBuf bp_c;                     // broad phase grid (first sorted position per cell key)
     long long bp_cap = 0;            // pair keys the broad phase's buffer holds (grows with use)
@@ -1271,7 +1278,8 @@ int icf_destroy(icf_ctx *ctx)
     for (SlotMap &m : ctx->maps) free_map(m);
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
                       &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_i, &ctx->bp_l, &ctx->bp_c,
-                      &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids, &ctx->batch.d};
+                      &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids, &ctx->batch.d,
+                      &ctx->patc.snap, &ctx->patc.keys};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->small) (void)hipFree(ctx->small);
@@ -1475,6 +1483,7 @@ int icf_pattern_changed(icf_ctx *ctx)
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_pattern_changed during capture");
     for (SlotMap &m : ctx->maps) m.suspect = true;
     ctx->adj.valid = false;
+    ctx->patc.valid = false;
     ctx->epoch++;
     return 0;
 }
@@ -2320,41 +2329,108 @@ int icf_build_pattern(icf_ctx *ctx, const int *altId, int nNodes, const int *ie,
         return fail(ctx, ICF_E_INVALID, "icf_build_pattern: invalid argument");
     if (ctx->capturing) return fail(ctx, ICF_E_CAPTURE, "icf_build_pattern returns a host value");
     *nnz = 0;
+    // the static part (elements, CZs) and the impacts; the static part's distinct keys are kept
+    // and reused while its node ids and the altIds are unchanged, so that a call with only new
+    // impacts sorts the impact keys and merges them in
     PatSrc src[3] = {{ie, nElems, el_stride, EL_NODES, 4, 0, 0},
-                     {icz, nCZs, cz_stride, CZI_VRTS, 6, 0, 0},
+                     {icz, nCZs, cz_stride, CZI_VRTS, 6, 0, (long long)nElems * pat_pairs(4, 0)},
                      {icr, nImpacts, cr_stride, CR_NODES, 4, 1, 0}};
-    long long M = 0;
-    for (PatSrc &p : src) {
-        p.out0 = M;
-        M += (long long)p.n * pat_pairs(p.K, p.strict);
-    }
-    if (M >= (1LL << 31) - 1) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: 2^31 or more candidate pairs");
-    int *d_err = ctx->small + 1;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 1, 0);
+    const long long Ms = (long long)nElems * pat_pairs(4, 0) + (long long)nCZs * pat_pairs(6, 0);
+    const long long Mi = (long long)nImpacts * pat_pairs(4, 1);
+    if (Ms + Mi >= (1LL << 31) - 1)
+        return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: 2^31 or more candidate pairs");
+    int *d_err = ctx->small + 1, *d_diff = ctx->small + 2;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, d_err, 2, 0);
     int cb = 1;
     while ((1LL << cb) <= (long long)N) cb++;
-    int num = 0;
-    if (M > 0) {
-        if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)M)) return r;
-        if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)M)) return r;
+    const PatFields pf{altId, nNodes, ie, nElems, el_stride, icz, nCZs, cz_stride};
+    const long long nsnap = (long long)nNodes + 4LL * nElems + 6LL * nCZs;
+    auto &pc = ctx->patc;
+    bool hit = pc.valid && pc.nNodes == nNodes && pc.nElems == nElems && pc.el_stride == el_stride &&
+               pc.nCZs == nCZs && pc.cz_stride == cz_stride && pc.N == N && pc.cb == cb;
+    if (hit && nsnap > 0) {
+        hipLaunchKernelGGL(k_pat_snap, dim3(blocks_for(nsnap, 256)), dim3(256), 0, ctx->stream, pf, nsnap,
+                           (int *)pc.snap.p, 1, d_diff);
+        int diff = 0;
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(&diff, d_diff, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        hit = diff == 0;
+    }
+    // sorted keys -> their distinct blocks, in order (flag, scan, compact); returns the count
+    auto distinct = [&](const unsigned long long *keys, long long M, unsigned long long *out, int *count) -> int {
         if (int r = grow(ctx, ctx->bp_l, sizeof(int) * 2 * ((size_t)M + 1))) return r;
-        auto *a = (unsigned long long *)ctx->pat_a.p, *b = (unsigned long long *)ctx->pat_b.p;
         int *flag = (int *)ctx->bp_l.p, *pos = flag + M + 1;
-        for (const PatSrc &p : src)
-            if (p.n)
-                hipLaunchKernelGGL(k_pat_emit, dim3(blocks_for((long long)p.n * pat_pairs(p.K, p.strict), 256)),
-                                   dim3(256), 0, ctx->stream, p, altId, nNodes, N, cb, a, d_err);
-        size_t sb = 0, ub = 0;
-        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sb, a, b, (int)M, 0, 2 * cb, ctx->stream));
+        size_t ub = 0;
         HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, ub, flag, pos, (int)M + 1, ctx->stream));
-        if (int r = grow(ctx, ctx->scan_tmp, sb > ub ? sb : ub)) return r;
-        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, sb, a, b, (int)M, 0, 2 * cb, ctx->stream));
-        // the distinct blocks of the sorted keys, in order, back into a
+        if (int r = grow(ctx, ctx->scan_tmp, ub)) return r;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, flag + M, 1, 0);
-        hipLaunchKernelGGL(k_pat_flag, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, b, (int)M, cb, flag);
+        hipLaunchKernelGGL(k_pat_flag, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, keys, (int)M, cb, flag);
         HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, ub, flag, pos, (int)M + 1, ctx->stream));
-        hipLaunchKernelGGL(k_pat_compact, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, b, (int)M, flag, pos, a);
-        HIPCHK(ctx, hipMemcpyAsync(&num, pos + M, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        hipLaunchKernelGGL(k_pat_compact, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, keys, (int)M, flag,
+                           pos, out);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(count, pos + M, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return 0;
+    };
+    auto sort_keys = [&](unsigned long long *in, unsigned long long *out, long long M) -> int {
+        size_t sb = 0;
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sb, in, out, (int)M, 0, 2 * cb, ctx->stream));
+        if (int r = grow(ctx, ctx->scan_tmp, sb)) return r;
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, sb, in, out, (int)M, 0, 2 * cb, ctx->stream));
+        return 0;
+    };
+    if (!hit) {
+        pc.valid = false;
+        int nk = 0;
+        if (Ms > 0) {
+            if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)Ms)) return r;
+            if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)Ms)) return r;
+            if (int r = grow(ctx, pc.keys, sizeof(unsigned long long) * (size_t)Ms)) return r;
+            auto *a = (unsigned long long *)ctx->pat_a.p, *b = (unsigned long long *)ctx->pat_b.p;
+            for (int k = 0; k < 2; k++)
+                if (src[k].n)
+                    hipLaunchKernelGGL(k_pat_emit, dim3(blocks_for((long long)src[k].n * pat_pairs(src[k].K, 0), 256)),
+                                       dim3(256), 0, ctx->stream, src[k], altId, nNodes, N, cb, a, d_err);
+            if (int r = sort_keys(a, b, Ms)) return r;
+            if (int r = distinct(b, Ms, (unsigned long long *)pc.keys.p, &nk)) return r;
+        }
+        int err = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (err) return fail(ctx, ICF_E_INDEX, "icf_build_pattern: a node id or altId is out of range");
+        if (nsnap > 0) {
+            if (int r = grow(ctx, pc.snap, sizeof(int) * (size_t)nsnap)) return r;
+            hipLaunchKernelGGL(k_pat_snap, dim3(blocks_for(nsnap, 256)), dim3(256), 0, ctx->stream, pf, nsnap,
+                               (int *)pc.snap.p, 0, d_diff);
+            HIPCHK(ctx, hipGetLastError());
+        }
+        pc.nNodes = nNodes, pc.nElems = nElems, pc.el_stride = el_stride, pc.nCZs = nCZs;
+        pc.cz_stride = cz_stride, pc.N = N, pc.cb = cb, pc.nkeys = nk;
+        pc.valid = true;
+    }
+    // the impacts' keys, sorted and merged into the static part's
+    int num = pc.nkeys;
+    const unsigned long long *uniq = (const unsigned long long *)pc.keys.p;
+    if (Mi > 0) {
+        const long long Mt = pc.nkeys + Mi;
+        if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)std::max(2 * Mi, Mt))) return r;
+        if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)Mt)) return r;
+        auto *a = (unsigned long long *)ctx->pat_a.p, *b = (unsigned long long *)ctx->pat_b.p;
+        hipLaunchKernelGGL(k_pat_emit, dim3(blocks_for(Mi, 256)), dim3(256), 0, ctx->stream, src[2], altId, nNodes, N,
+                           cb, a, d_err);
+        if (int r = sort_keys(a, a + Mi, Mi)) return r;
+        size_t mb = 0;
+        HIPCHK(ctx, hipcub::DeviceMerge::MergeKeys(nullptr, mb, (const unsigned long long *)pc.keys.p, pc.nkeys,
+                                                   (const unsigned long long *)(a + Mi), (int)Mi, b,
+                                                   ::rocprim::less<unsigned long long>(), ctx->stream));
+        if (int r = grow(ctx, ctx->scan_tmp, mb)) return r;
+        HIPCHK(ctx, hipcub::DeviceMerge::MergeKeys(ctx->scan_tmp.p, mb, (const unsigned long long *)pc.keys.p,
+                                                   pc.nkeys, (const unsigned long long *)(a + Mi), (int)Mi, b,
+                                                   ::rocprim::less<unsigned long long>(), ctx->stream));
+        if (int r = distinct(b, Mt, a, &num)) return r;
+        uniq = a;
     }
     int err = 0;
     HIPCHK(ctx, hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -2363,7 +2439,7 @@ int icf_build_pattern(icf_ctx *ctx, const int *altId, int nNodes, const int *ie,
     const int n_nz = num;
     *nnz = n_nz;
     if (n_nz > max_nnz) return fail(ctx, ICF_E_CAPACITY, "icf_build_pattern: more blocks than max_nnz");
-    const unsigned long long *u = (const unsigned long long *)ctx->pat_a.p;
+    const unsigned long long *u = uniq;
     if (n_nz > 0)
         hipLaunchKernelGGL(k_pat_cols, dim3(blocks_for(n_nz, 256)), dim3(256), 0, ctx->stream, u, n_nz, cb, csr_cols);
     hipLaunchKernelGGL(k_pat_rows, dim3(blocks_for((long long)N + 1, 256)), dim3(256), 0, ctx->stream, u, n_nz, N, cb,

@@ -67,6 +67,35 @@ __global__ __launch_bounds__(256) void k_pat_emit(PatSrc s, const int *__restric
     keys[s.out0 + t] = key;
 }
 
+// The node-id fields a pattern's static part is built from, as one sequence: the altId of
+// every node, then the 4 node ids of each element (field by field), then the 6 of each CZ.
+struct PatFields {
+    const int *alt;
+    int nNodes;
+    const int *ie;
+    int nElems, el_stride;
+    const int *icz;
+    int nCZs, cz_stride;
+};
+__device__ inline int pat_field(const PatFields &f, long long i)
+{
+    if (i < f.nNodes) return f.alt[i];
+    i -= f.nNodes;
+    if (i < 4LL * f.nElems) return f.ie[i % f.nElems + (size_t)f.el_stride * (EL_NODES + i / f.nElems)];
+    i -= 4LL * f.nElems;
+    return f.icz[i % f.nCZs + (size_t)f.cz_stride * (CZI_VRTS + i / f.nCZs)];
+}
+// compare == 0: snap = the fields; else *diff = 1 if any field differs from snap.
+__global__ __launch_bounds__(256) void k_pat_snap(PatFields f, long long n, int *__restrict__ snap, int compare,
+                                                  int *__restrict__ diff)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = pat_field(f, i);
+    if (!compare) snap[i] = v;
+    else if (snap[i] != v) atomicOr(diff, 1);
+}
+
 // The sorted keys' distinct blocks: flag[i] = 1 where key i differs from key i-1 and is a
 // block (flag[M] = 0 is set by the caller); then each flagged key goes to its scan position.
 __global__ __launch_bounds__(256) void k_pat_flag(const unsigned long long *__restrict__ keys, int M, int cb,

@@ -71,6 +71,64 @@ def test_device_pattern_and_packing_match_host(orc, specimen_consts):
     gf.close()
 
 
+def test_pattern_static_part_kept_and_rechecked(orc, specimen_consts):
+    """icf_build_pattern keeps the static part (elements and CZs) between calls and merges the
+    impacts in: with impacts, without, with them again, and after one element's node ids
+    change in place (same sizes, so only the kept copy of the node ids can tell), each result
+    equals the host pattern of the same inputs."""
+    from icflow_amd.gpu_functionality import CSR
+    col = small_specimen(cells=(5, 4, 4), size=(0.05, 0.04, 0.04), grains=5, seed=9)
+    st = make_state(col, specimen_consts, seed=10, platen_dz_frac=0.3)
+    p = Pipeline(col, specimen_consts, st, contact=True)
+    gf, lib, ctx = p.gf, p.gf.lib, p.gf.ctx
+    icr_host, cr = gf.pack_icr(p.impacts)
+    T, C, P = len(col.elastic), len(col.czs), gf.nImpacts
+    es, cs = gf.el_elastic_stride, gf.cz_stride
+    alt = np.ascontiguousarray(col.alt, np.int32)
+    assert P > 0
+
+    def dev(name, a):
+        b = gf.buf(name, a.size, a.dtype)
+        b.upload(a)
+        return b
+
+    ie = gf.c_ie_pcsr.copy()
+    d_alt, d_ie = dev("c_alt", alt), dev("c_ie", ie)
+    d_icz, d_icr = dev("c_icz", gf.c_icz.copy()), dev("c_icr", icr_host.copy())
+    N = gf.csr.N
+    cap = 4 * gf.csr.nnz
+    d_rows, d_cols = gf.buf("c_rows", N + 1, np.int32), gf.buf("c_cols", cap, np.int32)
+
+    def device(nimp):
+        nnz = ctypes.c_int()
+        _lib.check(lib.icf_build_pattern(ctx, d_alt.p, len(alt), d_ie.p, T, es, d_icz.p, C, cs, d_icr.p, nimp, cr, N,
+                                         d_rows.p, d_cols.p, cap, ctypes.byref(nnz)), ctx, "build_pattern")
+        return d_rows.download(N + 1), d_cols.download(nnz.value)
+
+    def host(elastic, imp):
+        h = CSR(lib).build(alt[elastic], alt[col.czs], alt[imp] if imp is not None else None)
+        assert h.N == N
+        return h.rows, h.cols
+
+    elastic = np.asarray(col.elastic)
+    for nimp, imp in ((P, p.impacts), (0, None), (P, p.impacts)):
+        rows, cols = device(nimp)
+        hr, hc = host(elastic, imp)
+        assert np.array_equal(rows, hr) and np.array_equal(cols, hc), nimp
+    # element 0 takes the node ids of the last element: same sizes, different static part
+    el2 = elastic.copy()
+    el2[0] = el2[T - 1]
+    ie2 = ie.copy()
+    for k in range(4):
+        ie2[0 + es * k] = ie2[T - 1 + es * k]
+    assert np.array_equal(ie2.reshape(-1, es)[:4, :T].T, el2)
+    d_ie.upload(ie2)
+    rows, cols = device(P)
+    hr, hc = host(el2, p.impacts)
+    assert np.array_equal(rows, hr) and np.array_equal(cols, hc)
+    gf.close()
+
+
 def test_assembly_from_device_packed_buffers(orc, specimen_consts):
     """The fused assembly after icf_pack_pcsr rewrote the slot fields in place: the library's
     cached maps are re-checked against the new fields and the result matches the oracle."""
