@@ -311,6 +311,9 @@ int icf_nodes_accept(icf_ctx *ctx, double *st, int st_stride, const double *dn, 
  * (ie fields 0-3, icz 4-9, icr 0-3).  All arrays are device memory. */
 /* Rows 0..N-1; row r's columns are the sorted union of c >= r over pairs (altId_i, altId_j)
  * of non-anchored vertices of each element and CZ, and c > r over each impact's nodes.
+ * The context keeps the elements' and CZs' part with a copy of their node ids and the altIds,
+ * and reuses it on a later call whose node ids and altIds compare equal (only the impacts are
+ * then sorted and merged in); icf_pattern_changed drops it.
  * Writes csr_rows[N+1], csr_cols[*nnz].  Returns ICF_E_CAPACITY (with *nnz set) if the
  * pattern has more than max_nnz blocks, ICF_E_INDEX for a node id outside [0, nNodes) or an
  * altId >= N.  Any unit count may be 0 (its pointer may then be NULL). */
