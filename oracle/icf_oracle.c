@@ -300,6 +300,22 @@ double orc_Dnt(double opn, double opt)
     return coeff * (e1 - e2) * (e3 - e4);
 }
 
+/* Test instrumentation of the law's leaves: while a log is set (orc_set_branch_log),
+ * orc_cz_force writes the leaf each Gauss point took to log[3 * unit + gp]:
+ *   0 opening beyond deln or delt (early failure)
+ *   contact (opn < 0): 1-3 with pmax != 0, 4-6 with pmax == 0; within each, tangential
+ *     loading, unloading, failure
+ *   7 both softening; 8, 9 tangential softening only (pmax != 0, == 0); 10, 11 normal
+ *     softening only (tmax != 0, == 0); 12-15 neither (pmax != 0 / == 0 x tmax != 0 / == 0);
+ *   16 failure away from contact. */
+static int *g_branch_log = 0;
+static _Thread_local int g_branch_at = -1;   /* per thread: orc_assemble_par runs units in parallel */
+void orc_set_branch_log(int *log) { g_branch_log = log; }
+static void log_branch(int code)
+{
+    if (g_branch_log && g_branch_at >= 0) g_branch_log[g_branch_at] = code;
+}
+
 /* PPR law with loading history.  flags[0] = contact, flags[1] = failed (set, never cleared).
  * out = {Tn, Tt, Dnn, Dtt, Dnt, Dtn}.  On the early-failure return Dtn is left at 0 here;
  * the reference leaves it unset, which is harmless because a failed CZ is not assembled.
@@ -311,6 +327,7 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
     if (opn > C.deln || opt > C.delt) {           /* :40-42 */
         flags[1] = 1;
         out[0] = out[1] = out[2] = out[3] = out[4] = 0;
+        log_branch(0);
         return;
     }
     int contact = (opn < 0);
@@ -320,6 +337,7 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
 
     if (contact) {                                 /* :49-87 */
         Dnt = 0;
+        const int base = *pmax != 0 ? 1 : 4;
         if (*pmax != 0) {
             double peak = orc_Tn(*pmax, *tmax);
             Tn = peak * opn / *pmax;
@@ -333,15 +351,18 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
             if (opt >= *tmax) {
                 *tmax = opt;
                 Dtt = orc_Dtt(0, opt);
+                log_branch(base);
             } else {
                 double peak = orc_Tt(0, *tmax);
                 Tt = peak * opt / *tmax;
                 Dtt = peak / *tmax;
+                log_branch(base + 1);
             }
         } else {
             Tt = Dtt = Dnt = 0;
             Tn = Dnn = 0;
             flags[1] = 1;
+            log_branch(base + 2);
         }
     } else {                                       /* :88-172 */
         Tt = orc_Tt(opn, opt);
@@ -350,12 +371,14 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
             !(opn > C.deln * C.rn / 5 && Tn < thr_n)) {
             int tsoft = (opt >= *tmax), nsoft = (opn >= *pmax);
             if (tsoft && nsoft) {                  /* :100-108 */
+                log_branch(7);
                 *tmax = opt;
                 *pmax = opn;
                 Dnn = orc_Dnn(opn, opt);
                 Dnt = orc_Dnt(opn, opt);
                 Dtt = orc_Dtt(opn, opt);
             } else if (tsoft && !nsoft) {          /* :109-124 */
+                log_branch(*pmax != 0 ? 8 : 9);
                 Dnt = 0;
                 if (*pmax != 0) {
                     double peak = orc_Tn(*pmax, *tmax);
@@ -369,6 +392,7 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
                 Tt = orc_Tt(*pmax, opt);
                 Dtt = orc_Dtt(*pmax, opt);
             } else if (!tsoft && nsoft) {          /* :125-141 */
+                log_branch(*tmax != 0 ? 10 : 11);
                 Dnt = 0;
                 if (*tmax != 0) {
                     double peak = orc_Tt(*pmax, *tmax);
@@ -382,6 +406,7 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
                 Tn = orc_Tn(*pmax, *tmax);
                 Dnn = orc_Dnn(*pmax, *tmax);
             } else {                               /* :142-164 */
+                log_branch(12 + (*pmax != 0 ? 0 : 2) + (*tmax != 0 ? 0 : 1));
                 Dnt = 0;
                 double peakN = orc_Tn(*pmax, *tmax);
                 if (*pmax != 0) {
@@ -403,6 +428,7 @@ void orc_cohesive_law(int flags[2], double *pmax, double *tmax, double opn, doub
         } else {                                   /* :167-171 */
             flags[1] = 1;
             Tn = Tt = Dnn = Dtt = Dnt = 0;
+            log_branch(16);
         }
     }
     out[0] = Tn; out[1] = Tt; out[2] = Dnn; out[3] = Dtt; out[4] = Dnt; out[5] = Dnt; /* :173 */
@@ -478,7 +504,9 @@ static void cz_unit(int u, double *dcz, int *icz, double *dn, double *A, double 
             double opn = dnn, opt = sqrt(dt1 * dt1 + dt2 * dt2);
             int fl[2] = {gp_contact[gp], gp_failed[gp]};
             double L[6];
+            g_branch_at = 3 * u + gp;
             orc_cohesive_law(fl, &pmax[gp], &tmax[gp], opn, opt, L);
+            g_branch_at = -1;
             gp_contact[gp] = fl[0];
             gp_failed[gp] = fl[1];
             double Tn = L[0], Tt = L[1], Dnn = L[2], Dtt = L[3], Dnt = L[4], Dtn = L[5];

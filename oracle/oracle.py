@@ -40,6 +40,7 @@ _SIGS = {
     "orc_Dtt": (c_double, [c_double, c_double]),
     "orc_Dnt": (c_double, [c_double, c_double]),
     "orc_cohesive_law": (None, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p]),
+    "orc_set_branch_log": (None, [c_void_p]),
     "orc_cz_rotation": (None, [c_void_p, c_void_p, c_void_p]),
     "orc_cz_force": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, c_int]),
     "orc_ctest": (c_int, [c_void_p, c_double, c_double, c_double]),
@@ -127,6 +128,20 @@ def elem_force(ie, dn, h, A, b, n, es, ns):
 
 def cz_force(dcz, icz, dn, A, b, h, n, cs, ns):
     load().orc_cz_force(_p(dcz), _p(icz), _p(dn), _p(A), _p(b), h, n, cs, ns)
+
+
+def cz_branches(dcz, icz, dn, A, b, h, n, cs, ns):
+    """The PPR-law leaf each Gauss point of the first n CZs takes (codes in
+    orc_set_branch_log's comment), from cz_force on copies of the arrays; -1 where a CZ is
+    skipped (already failed)."""
+    log = np.full(3 * n, -1, np.int32)
+    lib = load()
+    lib.orc_set_branch_log(_p(log))
+    try:
+        cz_force(dcz.copy(), icz.copy(), dn.copy(), A.copy(), b.copy(), h, n, cs, ns)
+    finally:
+        lib.orc_set_branch_log(None)
+    return log
 
 
 def narrow_phase(nPairs, dn, ie, nl, es, ns):

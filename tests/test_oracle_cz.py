@@ -175,3 +175,40 @@ def test_cz_secant_branch_rhs_equals_minus_K_u(orc, C):
         # block structure of B = [s x I, -s x I]: K_top,bot = -K_top,top for coincident faces
         assert rel_err(K[:9, 9:], -K[:9, :9]) <= 1e-12
         assert rel_err(K[9:, 9:], K[:9, :9]) <= 1e-12
+
+
+def test_law_leaves_logged_over_crafted_states(orc, lib, specimen_consts):
+    """The oracle's leaf log (orc_set_branch_log) over the inputs the device branch test uses
+    (a small specimen's displacements scaled 0.3x to 10x, load histories of 0 or two
+    magnitudes per Gauss point): every reachable leaf appears, and the five that need pmax == 0
+    without normal softening or tmax == 0 without tangential softening never do, although
+    zero histories are among the inputs."""
+    from icflow_amd._lib import ptr
+    from icflow_amd.gpu_functionality import CSR, stride_of
+    from workloads import make_state, small_specimen
+    orc.set_constants(specimen_consts)
+    col = small_specimen(cells=(6, 5, 4), grains=6, seed=0)
+    st = make_state(col, specimen_consts, seed=1)
+    alt = np.ascontiguousarray(col.alt, np.int32)
+    csr = CSR(lib).build(alt[col.elastic], alt[col.czs])
+    C, Nn = len(col.czs), len(col.x0)
+    cs, ns = stride_of(C), stride_of(Nn)
+    icz = np.zeros(58 * cs, np.int32)
+    lib.icfh_pack_czs(csr.h, ptr(np.ascontiguousarray(col.czs, np.int32)), ptr(alt), C, cs, ptr(icz))
+    deln, delt = specimen_consts.deln, specimen_consts.delt
+    rng = np.random.default_rng(11)
+    seen = set()
+    for scale in (0.3, 1.0, 3.0, 10.0):
+        for hist in (0, 1, 2):
+            dn = np.zeros(18 * ns)
+            for k in range(3):
+                dn[k * ns:k * ns + Nn] = col.x0[:, k]
+                dn[(3 + k) * ns:(3 + k) * ns + Nn] = st.un[:, k] * scale
+            dcz = np.zeros(16 * cs)
+            if hist:
+                for j in range(3):
+                    dcz[j * cs:j * cs + C] = rng.choice([0.0, 0.02 * hist * deln, 0.2 * hist * deln], C)
+                    dcz[(3 + j) * cs:(3 + j) * cs + C] = rng.choice([0.0, 0.02 * hist * delt, 0.2 * hist * delt], C)
+            log = orc.cz_branches(dcz, icz, dn, np.zeros(9 * csr.nnz), np.zeros(3 * csr.N), 1e-4, C, cs, ns)
+            seen |= set(log[log >= 0].tolist())
+    assert seen == {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16}
