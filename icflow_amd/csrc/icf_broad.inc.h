@@ -319,11 +319,13 @@ __global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__
 // rank among the pairs of the same a (count[a] bumped atomically); rank[i] = -1 for a pair
 // that fails.
 __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ skd, const float4 *__restrict__ sk32,
-                                                  const int *__restrict__ sidx, long long n,
+                                                  const int *__restrict__ sidx,
+                                                  const unsigned long long *__restrict__ ncand, long long cap,
                                                   unsigned long long *__restrict__ io, int *__restrict__ rank,
                                                   int *__restrict__ count)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)*ncand < cap ? (long long)*ncand : cap;   // candidates in the buffer
     if (i >= n) return;
     const unsigned long long v = io[i];
     const int ja = (int)(v >> 32), jb = (int)(v & 0xffffffffULL);
@@ -362,11 +364,13 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
 // The pairs grouped by a, in no particular order within a group: pair i goes to
 // start[a] + rank[i] (start = exclusive scan of the counts).
 __global__ __launch_bounds__(256) void k_bp_scatter(const unsigned long long *__restrict__ io,
-                                                    const int *__restrict__ rank, long long n,
+                                                    const int *__restrict__ rank,
+                                                    const unsigned long long *__restrict__ ncand, long long cap,
                                                     const int *__restrict__ start,
                                                     unsigned long long *__restrict__ grp)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)*ncand < cap ? (long long)*ncand : cap;
     if (i >= n) return;
     const int r = rank[i];
     if (r < 0) return;

@@ -224,7 +224,8 @@ struct icf_ctx {
         DevBuf snap, keys;
     } patc;
     DevBuf bp_f, bp_i, bp_l;         // broad phase and NL2set scratch (doubles, ints, int64)
-    DevBuf bp_c;                     // broad phase grid (first sorted position per cell key)
+    DevBuf bp_c;                     // broad phase grid (cell counts, first sorted position per cell key)
+    DevBuf bp_g;                     // broad phase pair grouping (counts and starts per tet, ranks)
     long long bp_cap = 0;            // pair keys the broad phase's buffer holds (grows with use)
     DevBuf tsum;                     // per-node tet sums of the fused call
     DevBuf ct_f;                     // impact forces (the tangent blocks are in ct)
@@ -1279,7 +1280,7 @@ int icf_destroy(icf_ctx *ctx)
     DevBuf *bufs[] = {&ctx->tet_rhs, &ctx->tet_tan, &ctx->cz_rhs, &ctx->cz_tan, &ctx->ct,   &ctx->scan_tmp,
                       &ctx->pat_a,   &ctx->pat_b,   &ctx->bp_f,   &ctx->bp_i, &ctx->bp_l, &ctx->bp_c,
                       &ctx->tsum,    &ctx->ct_f,    &ctx->adj.off, &ctx->adj.ids, &ctx->batch.d,
-                      &ctx->patc.snap, &ctx->patc.keys};
+                      &ctx->patc.snap, &ctx->patc.keys, &ctx->bp_g};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->small) (void)hipFree(ctx->small);
@@ -2557,21 +2558,43 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b3, count, first, G + 1, ctx->stream));
     hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, key, rank,
                        first, ie, el_all_stride, sidx, skd, snode, sk32);
-    // candidate pairs per cell (FP32 prefilter), collected in LDS and appended in batches; a
-    // second search when they outgrew the buffer
+    // candidate pairs per cell (FP32 prefilter), collected in LDS and appended in batches; the
+    // exact test; the passing pairs grouped by a (a counting sort) and placed by their rank of b
+    // within the group: the pairs in (a, b) order.  The launches after the search take the
+    // candidate count from device memory (their grids cover the buffer), so one read-back at
+    // the end serves all; a second round only when the candidates outgrew the buffer.
     const int ncell = g.n[0] * g.n[1] * g.n[2];
     auto *tot = (unsigned long long *)ctx->bp_l.p;
     long long cap = std::max<long long>(ctx->bp_cap, 4LL * nT);
+    size_t b5 = 0;
+    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b5, (int *)nullptr, (int *)nullptr, nT + 1, ctx->stream));
+    if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;
     long long ncand = 0;
+    int nv = 0;
     for (int pass = 0; pass < 2; pass++) {
+        if (cap >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 candidate pairs");
         if (int r = grow(ctx, ctx->pat_a, sizeof(unsigned long long) * (size_t)cap)) return r;
+        if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)cap)) return r;
+        if (int r = grow(ctx, ctx->bp_g, sizeof(int) * (2 * ((size_t)nT + 1) + (size_t)cap))) return r;
+        int *acount = (int *)ctx->bp_g.p, *start = acount + nT + 1, *rnk = start + nT + 1;
+        auto *pa = (unsigned long long *)ctx->pat_a.p, *grp = (unsigned long long *)ctx->pat_b.p;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)tot, 2, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(blocks_for((long long)nT + 1, 256)), dim3(256), 0, ctx->stream, acount,
+                           nT + 1, 0);
         hipLaunchKernelGGL(k_bp_cellwave, dim3((ncell + BPW_WAVES - 1) / BPW_WAVES), dim3(64 * BPW_WAVES), 0,
-                           ctx->stream, sk32, snode, exclude_shared, g, first, tot,
-                           (unsigned long long *)ctx->pat_a.p, cap);
+                           ctx->stream, sk32, snode, exclude_shared, g, first, tot, pa, cap);
+        hipLaunchKernelGGL(k_bp_exact, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, skd, sk32, sidx, tot,
+                           cap, pa, rnk, acount);
+        HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b5, acount, start, nT + 1, ctx->stream));
+        hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, pa, rnk, tot, cap,
+                           start, grp);
+        if (max_pairs > 0)
+            hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, grp, start, nT,
+                               max_pairs, pairs);
         unsigned long long got = 0;
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipMemcpyAsync(&got, tot, sizeof(got), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(&nv, start + nT, sizeof(nv), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         ncand = (long long)got;
         if (ncand <= cap) break;
@@ -2579,31 +2602,6 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
         cap = ncand + ncand / 4;
     }
     ctx->bp_cap = std::max(ctx->bp_cap, ncand + ncand / 4);
-    if (ncand == 0) return 0;
-    if (ncand >= (1LL << 31)) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more than 2^31 candidate pairs");
-    // the exact test, then the passing pairs grouped by a (a counting sort) and placed by
-    // their rank of b within the group: the pairs in (a, b) order
-    if (int r = grow(ctx, ctx->bp_c, sizeof(int) * (2 * ((size_t)nT + 1) + (size_t)ncand))) return r;
-    if (int r = grow(ctx, ctx->pat_b, sizeof(unsigned long long) * (size_t)ncand)) return r;
-    int *acount = (int *)ctx->bp_c.p, *start = acount + nT + 1, *rnk = start + nT + 1;
-    auto *pa = (unsigned long long *)ctx->pat_a.p, *grp = (unsigned long long *)ctx->pat_b.p;
-    size_t b5 = 0;
-    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b5, acount, start, nT + 1, ctx->stream));
-    if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;
-    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for((long long)nT + 1, 256)), dim3(256), 0, ctx->stream, acount,
-                       nT + 1, 0);
-    hipLaunchKernelGGL(k_bp_exact, dim3(blocks_for(ncand, 256)), dim3(256), 0, ctx->stream, skd, sk32, sidx, ncand,
-                       pa, rnk, acount);
-    HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b5, acount, start, nT + 1, ctx->stream));
-    hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(ncand, 256)), dim3(256), 0, ctx->stream, pa, rnk, ncand, start,
-                       grp);
-    if (max_pairs > 0)
-        hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(ncand, 256)), dim3(256), 0, ctx->stream, grp, start, nT,
-                           max_pairs, pairs);
-    int nv = 0;
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(&nv, start + nT, sizeof(nv), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     *nOut = (long long)nv;
     if ((long long)nv > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
     return 0;
