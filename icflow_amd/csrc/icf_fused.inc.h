@@ -85,7 +85,9 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_fused(int nbf, const int *_
         reduce_vals<MIXED, ISO, false, L, KEYS, STAGE>(local, nchunks, off, codes, nullptr, chunks, nchunks, A, sv);
 }
 
-// The CZ counters (workgroups [0, nbc)), then node forces with the rhs rows (reduce_force_rhs;
+// The CZ counters (workgroups [0, nbc); per specimen when cz_off is given: the CZs of specimen s
+// are cz_off[s] .. cz_off[s + 1], its counts go to spec_counts[2 s], [2 s + 1] and into the
+// totals), then node forces with the rhs rows (reduce_force_rhs;
 // nbf workgroups) interleaved with the BCSR values of the fused call (overwrite; nchunks
 // workgroups): the whole second half of the fused call as one launch.  Workgroups of KEYS lanes.
 template <bool MIXED, bool ISO, typename L, int KEYS, int STAGE>
@@ -94,13 +96,15 @@ __global__ __launch_bounds__(KEYS) void k_vals_force_rhs_fused(int nbf, int nbc,
         const int *__restrict__ foff, const uint32_t *__restrict__ fcodes, const int *__restrict__ fkeys, int nfkeys,
         double *__restrict__ F, int ns, const int *__restrict__ node_row, const int *__restrict__ roff,
         const uint32_t *__restrict__ rcodes, const int *__restrict__ cz_begin, double *__restrict__ b,
-        const int *__restrict__ damaged, const int *__restrict__ failed, int nc, int *__restrict__ counts, Stage sv,
-        int ilv)
+        const int *__restrict__ damaged, const int *__restrict__ failed, int nc, int *__restrict__ counts,
+        const int *__restrict__ cz_off, int nspec, int *__restrict__ spec_counts, Stage sv, int ilv)
 {
     const int w = blockIdx.x;
     int local;
-    if (w < nbc)
-        count_flags_n<KEYS>(w, nbc, damaged, failed, nc, counts);
+    if (w < nbc) {
+        if (cz_off) count_flags_spec<KEYS>(w, nbc, damaged, failed, cz_off, nspec, spec_counts, counts);
+        else count_flags_n<KEYS>(w, nbc, damaged, failed, nc, counts);
+    }
     else if (!split2(w - nbc, nbf, nchunks, ilv != 0, local))
         reduce_force_rhs(local, nbf, foff, fcodes, fkeys, nfkeys, F, ns, node_row, roff, rcodes, cz_begin, b, sv);
     else if (local < nchunks)

@@ -273,6 +273,11 @@ struct icf_ctx {
         DevBuf d;
         // the units and epoch whose block-diagonal layout was last checked against `off`
         std::vector<long long> checked;
+        // set by icf_assemble_batched for its assembly: the fused value launch then also writes
+        // the per-specimen CZ counts (spec_counted reports that it did)
+        const int *cz_off = nullptr;
+        int *spec_counts = nullptr;
+        bool spec_counted = false;
     } batch;
     // per-kernel profiling (icf_set_profiling)
     bool profiling = false;
@@ -1103,17 +1108,23 @@ bool vals_force_rhs_fused(icf_ctx *ctx, const SlotMap *mv, double *A, const Slot
         constexpr int K = decltype(kc)::value, S = decltype(sc)::value;
         const int nch = mv->nlaunch ? mv->nchunks[c][k] : 0;
         const int nbf = mf->nlaunch ? round8(blocks_for(mf->nlaunch, K)) : 0;
-        const int nbc = nCZs ? round8(std::min<long long>(blocks_for(nCZs, K), 64)) : 0;
+        // the CZ counters' workgroups: per specimen in a batched call (one or more specimens
+        // each), else strided over all CZs
+        const auto &bt = ctx->batch;
+        const bool per_spec = bt.cz_off && nCZs;
+        const int nbc = per_spec ? round8(std::min(bt.n, 64))
+                                 : nCZs ? round8(std::min<long long>(blocks_for(nCZs, K), 64)) : 0;
         const dim3 g(nbc + interleave2_grid(nbf, nch)), bl(K);   // not empty: something has keys or CZs
 #define ICF_VFR(MX, IS, L, NAME)                                                                                   \
     LAUNCH(ctx, NAME, (k_vals_force_rhs_fused<MX, IS, L, K, S>), g, bl, nbf, nbc, mv->off, mv->codes, mv->chunks[c][k], \
            nch, A, mf->off, mf->codes, mf->keys, mf->nlaunch, F, ns, mr->node_row, mr->off, mr->codes, mr->cz_begin, \
-           b, dmg, fld, nCZs, counts, st, ilv)
+           b, dmg, fld, nCZs, counts, per_spec ? bt.cz_off : nullptr, bt.n, bt.spec_counts, st, ilv)
         if (mixed && iso) ICF_VFR(true, true, float, "k_vals_force_rhs<mixed,iso>");
         else if (mixed) ICF_VFR(true, false, float, "k_vals_force_rhs<mixed,gen>");
         else if (iso) ICF_VFR(false, true, double, "k_vals_force_rhs<fp64,iso>");
         else ICF_VFR(false, false, double, "k_vals_force_rhs<fp64,gen>");
 #undef ICF_VFR
+        ctx->batch.spec_counted = per_spec;
     });
     shape_end(ctx, ICF_SHAPE_ORDER);
     tune_end(ctx, false, !mixed);
@@ -2220,9 +2231,14 @@ int icf_assemble_batched(icf_ctx *ctx, const icf_worklist *wl, const int *ie, in
             return fail(ctx, ICF_E_INDEX,
                         "icf_assemble_batched: a unit writes a BCSR block or rhs row outside its specimen's ranges");
     }
-    if (int r = assemble_impl(ctx, ie, nE, el_stride, dcz, icz, nC, cz_stride, dn, nd_stride, h, A, nnzb, b, N,
-                              nullptr, nullptr, nullptr))
-        return r;
+    B.cz_off = bo.cz;
+    B.spec_counts = cnt;
+    B.spec_counted = false;
+    const int ra = assemble_impl(ctx, ie, nE, el_stride, dcz, icz, nC, cz_stride, dn, nd_stride, h, A, nnzb, b, N,
+                                 nullptr, nullptr, nullptr);
+    B.cz_off = nullptr;
+    B.spec_counts = nullptr;
+    if (ra) return ra;
     if (sig != B.checked) {
         B.checked = sig;
         B.checked.back() = (long long)ctx->epoch;   // map checks inside the call may have moved it
@@ -2233,7 +2249,9 @@ int icf_assemble_batched(icf_ctx *ctx, const icf_worklist *wl, const int *ie, in
             .rd(nC ? icz : nullptr, (CZI_ROWS + 6) * (size_t)cz_stride);
     });
     if (ln.err) return ln.err;
-    if (nC) {
+    if (nC && B.spec_counted) {
+        // written by the fused value launch
+    } else if (nC) {
         const int *dmg = icz + (size_t)cz_stride * CZI_TDAMAGED, *fld = icz + (size_t)cz_stride * CZI_TFAILED;
         LAUNCH(ctx, "k_batch_counts", k_batch_counts, dim3(n), dim3(256), dmg, fld, bo.cz, cnt);
     } else {

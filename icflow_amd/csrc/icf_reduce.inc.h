@@ -598,6 +598,49 @@ __device__ __forceinline__ void count_flags_n(int bid_, int nb_, const int *__re
     }
 }
 
+// Per-specimen counters: workgroup bid_ takes specimens bid_, bid_ + nb_, ...; each
+// specimen's damaged and failed counts are written (no atomics) and added into the totals.
+template <int NT>
+__device__ __forceinline__ void count_flags_spec(int bid_, int nb_, const int *__restrict__ damaged,
+        const int *__restrict__ failed, const int *__restrict__ cz_off, int nspec, int *__restrict__ spec,
+        int *__restrict__ out)
+{
+    __shared__ int part[2][NT / 64];
+    int td = 0, tf = 0;
+    for (int s = bid_; s < nspec; s += nb_) {
+        int d = 0, f = 0;
+        for (int i = cz_off[s] + (int)threadIdx.x; i < cz_off[s + 1]; i += NT) {
+            d += damaged[i];
+            f += failed[i];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            d += __shfl_xor(d, o);
+            f += __shfl_xor(f, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            part[0][threadIdx.x >> 6] = d;
+            part[1][threadIdx.x >> 6] = f;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            d = f = 0;
+            for (int w = 0; w < NT / 64; w++) {
+                d += part[0][w];
+                f += part[1][w];
+            }
+            spec[2 * s] = d;
+            spec[2 * s + 1] = f;
+            td += d;
+            tf += f;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (td) atomicAdd(out, td);
+        if (tf) atomicAdd(out + 1, tf);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_count_flags(const int *__restrict__ damaged, const int *__restrict__ failed,
                                                      int n, int *__restrict__ out)
 {
