@@ -185,8 +185,8 @@ __device__ inline void bp_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The pair search, first pass: one wave per cell of the grid's interior, no workgroup
-// barriers.  The candidates are the cell's own tets and those of the 13 neighbouring cells
+// The pair search, first pass: one wave per cell of the grid's interior; the waves of a
+// workgroup meet only at its start.  The candidates are the cell's own tets and those of the 13 neighbouring cells
 // with a larger key (each pair of cells visited once): 5 runs of consecutive keys -- (x, x+1)
 // of the cell's own row, and the rows (x-1 .. x+1) at (y+1, z), (y-1, z+1), (y, z+1),
 // (y+1, z+1) -- hence 5 ranges of sorted positions; in the first range only larger positions
@@ -196,20 +196,19 @@ __device__ inline void bp_wave_sync()
 // 12 directions and (when asked) the tets share no node.  The passing (own, candidate)
 // positions collect in the wave's LDS buffer and leave in batches, one atomicAdd on *total
 // per batch (positions beyond cap are only counted; the caller then grows the buffer and
-// searches again).  The second pass applies the exact FP64 test.
+// searches again).  What a wave holds at its end leaves with the rest of its workgroup's:
+// the last wave of the workgroup to finish reserves space for all of them with one atomicAdd
+// (the reservations on the one counter, one per wave, took a quarter of the kernel's time).
+// The second pass applies the exact FP64 test.
 constexpr int BPW_WAVES = 4, BPW_OWN = 32, BPW_BUF = 256;
-__global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__restrict__ sk32,
-                                                                const int4 *__restrict__ snode, int exclude_shared,
-                                                                BpGrid g, const int *__restrict__ first,
-                                                                unsigned long long *__restrict__ total,
-                                                                unsigned long long *__restrict__ out, long long cap)
+
+// One wave's cell of the pair search (k_bp_cellwave): on return, cnt pairs are left in buf.
+__device__ inline void bp_cell_pairs(const float4 *__restrict__ sk32, const int4 *__restrict__ snode,
+                                     int exclude_shared, const BpGrid &g, const int *__restrict__ first,
+                                     unsigned long long *__restrict__ total, unsigned long long *__restrict__ out,
+                                     long long cap, int i, float4 (*ow)[7], unsigned long long *buf, int lane,
+                                     unsigned long long below, int &cnt)
 {
-    __shared__ float4 own[BPW_WAVES][BPW_OWN][7];
-    __shared__ unsigned long long pbuf[BPW_WAVES][BPW_BUF];
-    const int wave = threadIdx.x >> 6, lane = __lane_id();
-    const int ncell = g.n[0] * g.n[1] * g.n[2];
-    const int i = (int)blockIdx.x * BPW_WAVES + wave;
-    if (i >= ncell) return;
     const int x = i % g.n[0] + 1, y = (i / g.n[0]) % g.n[1] + 1, z = i / (g.n[0] * g.n[1]) + 1;
     const int L = x + g.gx * (y + g.gy * z);
     const int s0 = first[L], s1 = first[L + 1];
@@ -224,10 +223,6 @@ __global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__
 #pragma unroll
     for (int r = 0; r < 5; r++) r_pre[r + 1] = r_pre[r] + r_hi[r] - r_lo[r];
     const int K = r_pre[5];
-    float4 (*ow)[7] = own[wave];
-    unsigned long long *buf = pbuf[wave];
-    const unsigned long long below = (1ULL << lane) - 1;
-    int cnt = 0;
     auto flush = [&]() {
         bp_wave_sync();
         unsigned long long base = 0;
@@ -306,7 +301,46 @@ __global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__
             }
         }
     }
-    if (cnt) flush();
+}
+
+__global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__restrict__ sk32,
+                                                                const int4 *__restrict__ snode, int exclude_shared,
+                                                                BpGrid g, const int *__restrict__ first,
+                                                                unsigned long long *__restrict__ total,
+                                                                unsigned long long *__restrict__ out, long long cap)
+{
+    __shared__ float4 own[BPW_WAVES][BPW_OWN][7];
+    __shared__ unsigned long long pbuf[BPW_WAVES][BPW_BUF];
+    __shared__ int wcnt[BPW_WAVES], done;
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    if (threadIdx.x == 0) done = 0;
+    __syncthreads();
+    const unsigned long long below = (1ULL << lane) - 1;
+    unsigned long long *buf = pbuf[wave];
+    int cnt = 0;
+    const int ncell = g.n[0] * g.n[1] * g.n[2];
+    const int i = (int)blockIdx.x * BPW_WAVES + wave;
+    if (i < ncell) bp_cell_pairs(sk32, snode, exclude_shared, g, first, total, out, cap, i, own[wave], buf, lane,
+                                 below, cnt);
+    // the workgroup's last wave out writes every wave's remaining pairs
+    if (lane == 0) wcnt[wave] = cnt;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    int last = 0;
+    if (lane == 0) last = atomicAdd(&done, 1) == BPW_WAVES - 1;
+    if (!__shfl(last, 0)) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    int pre[BPW_WAVES + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int w = 0; w < BPW_WAVES; w++) pre[w + 1] = pre[w] + wcnt[w];
+    if (pre[BPW_WAVES] == 0) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(total, (unsigned long long)pre[BPW_WAVES]);
+    base = __shfl(base, 0);
+#pragma unroll
+    for (int w = 0; w < BPW_WAVES; w++)
+        for (int q = lane; q < wcnt[w]; q += 64)
+            if (base + pre[w] + q < (unsigned long long)cap) out[base + pre[w] + q] = pbuf[w][q];
 }
 
 // The pair search, second pass: per candidate (own, candidate) position pair of the first
