@@ -126,6 +126,31 @@ struct BpGrid {
     int n[3], gx, gy, gz;
 };
 
+// The rank of this lane's key among all ranks taken from count[key] (bumped atomically), for
+// lanes with on set (-1 for the others): one atomicAdd per run of consecutive such lanes with
+// the same key -- the run's first lane reserves the run's length and its lanes take
+// consecutive ranks.  Where keys repeat along a wave this replaces up to 64 atomics on one
+// address, which the L2 serialises, by one.  Lanes above an inactive lane must be inactive
+// too (a kernel's early exit for its last indices).
+__device__ inline int run_rank(bool on, unsigned key, int *__restrict__ count)
+{
+    const int lane = __lane_id();
+    const unsigned long long om = __ballot(on);
+    const unsigned prev = __shfl_up(key, 1);
+    const bool head = on && (lane == 0 || !((om >> (lane - 1)) & 1) || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long ends = heads | ~om;   // lanes where a run cannot continue
+    int r = -1, h = 0;
+    if (on) h = 63 - __clzll(heads & ((2ULL << lane) - 1));   // the run's first lane
+    if (head) {
+        const unsigned long long above = lane == 63 ? 0ULL : ends & ~((2ULL << lane) - 1);
+        const int len = above ? __ffsll((long long)above) - 1 - lane : 64 - lane;
+        r = atomicAdd(&count[key], len);
+    }
+    const int base = __shfl(r, h);
+    return on ? base + lane - h : -1;
+}
+
 // Per tet: its cell key and its rank among the tets of its cell (count[key] is bumped
 // atomically, so the order within a cell varies from call to call; the pairs do not).
 __global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, int nT, BpGrid g,
@@ -142,7 +167,7 @@ __global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, 
     }
     const unsigned k = (unsigned)(c[0] + 1) + (unsigned)g.gx * ((unsigned)(c[1] + 1) + (unsigned)g.gy * (unsigned)(c[2] + 1));
     key[t] = k;
-    rank[t] = atomicAdd(&count[k], 1);
+    rank[t] = run_rank(true, k, count);   // neighbouring tets of a mesh often share a cell
 }
 
 // The k-DOPs and node ids in cell order (a counting sort: first[key] is the exclusive scan
@@ -385,14 +410,15 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
             hit = hit & !(u.x > w.y) & !(u.y < w.x);
         }
     }
-    int r = -1;
+    unsigned a = 0xffffffffu;
     if (hit) {
         const unsigned oa = (unsigned)sidx[ja], ob = (unsigned)sidx[jb];
-        const unsigned a = oa < ob ? oa : ob, b = oa < ob ? ob : oa;
+        const unsigned b = oa < ob ? ob : oa;
+        a = oa < ob ? oa : ob;
         io[i] = ((unsigned long long)a << 32) | b;
-        r = atomicAdd(&count[a], 1);
     }
-    rank[i] = r;
+    // the first pass emits the candidates of one own tet together, so a repeats along the wave
+    rank[i] = run_rank(hit, a, count);
 }
 
 // The pairs grouped by a, in no particular order within a group: pair i goes to
