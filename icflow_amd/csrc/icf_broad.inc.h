@@ -224,8 +224,10 @@ __device__ inline void bp_wave_sync()
 // searches again).  What a wave holds at its end leaves with the rest of its workgroup's:
 // the last wave of the workgroup to finish reserves space for all of them with one atomicAdd
 // (the reservations on the one counter, one per wave, took a quarter of the kernel's time).
-// The second pass applies the exact FP64 test.
-constexpr int BPW_WAVES = 4, BPW_OWN = 32, BPW_BUF = 256;
+// Two waves per workgroup: a workgroup's LDS stays allocated until its slowest wave ends, and
+// the work per cell varies several-fold, so with four waves the resident waves averaged about
+// two per SIMD.  The second pass applies the exact FP64 test.
+constexpr int BPW_WAVES = 2, BPW_OWN = 16, BPW_BUF = 256;
 
 // One wave's cell of the pair search (k_bp_cellwave): on return, cnt pairs are left in buf.
 __device__ inline void bp_cell_pairs(const float4 *__restrict__ sk32, const int4 *__restrict__ snode,
