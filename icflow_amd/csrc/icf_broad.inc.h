@@ -219,15 +219,30 @@ __device__ inline void bp_wave_sync()
 // loaded coalesced); the cell's own tets, BPW_OWN at a time, are staged in the wave's LDS and
 // read by all lanes at once.  A pair passes if its outward-rounded FP32 k-DOPs overlap in all
 // 12 directions and (when asked) the tets share no node.  The passing (own, candidate)
-// positions collect in the wave's LDS buffer and leave in batches, one atomicAdd on *total
-// per batch (positions beyond cap are only counted; the caller then grows the buffer and
-// searches again).  What a wave holds at its end leaves with the rest of its workgroup's:
+// positions collect in the wave's LDS buffer and leave in batches, one atomicAdd on the
+// counter of the workgroup's buffer region per batch (positions beyond the region's capacity
+// are only counted; the caller then grows the buffer and searches again).  What a wave holds at its end leaves with the rest of its workgroup's:
 // the last wave of the workgroup to finish reserves space for all of them with one atomicAdd
 // (the reservations on the one counter, one per wave, took a quarter of the kernel's time).
 // Two waves per workgroup: a workgroup's LDS stays allocated until its slowest wave ends, and
 // the work per cell varies several-fold, so with four waves the resident waves averaged about
 // two per SIMD.  The second pass applies the exact FP64 test.
 constexpr int BPW_WAVES = 2, BPW_OWN = 16, BPW_BUF = 256;
+// The candidate buffer is split into BP_PARTS regions of cap / BP_PARTS entries, each with its
+// own counter (BP_PART_STRIDE counters apart, so that no two share a cache line): workgroup w
+// appends to region w mod BP_PARTS, which spreads the reservations over as many addresses.
+constexpr int BP_PARTS = 8, BP_PART_STRIDE = 64;
+
+// Position i of the candidate buffer: whether it holds a candidate (its region's count is
+// read from the region's counter; positions beyond a region's capacity hold none).
+__device__ inline bool bp_slot_used(long long i, long long cap, const unsigned long long *__restrict__ total)
+{
+    const long long capP = cap / BP_PARTS;
+    const long long r = i / capP;
+    if (r >= BP_PARTS) return false;
+    const long long n = (long long)total[r * BP_PART_STRIDE];
+    return i - r * capP < (n < capP ? n : capP);
+}
 
 // One wave's cell of the pair search (k_bp_cellwave): on return, cnt pairs are left in buf.
 __device__ inline void bp_cell_pairs(const float4 *__restrict__ sk32, const int4 *__restrict__ snode,
@@ -347,7 +362,11 @@ __global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__
     int cnt = 0;
     const int ncell = g.n[0] * g.n[1] * g.n[2];
     const int i = (int)blockIdx.x * BPW_WAVES + wave;
-    if (i < ncell) bp_cell_pairs(sk32, snode, exclude_shared, g, first, total, out, cap, i, own[wave], buf, lane,
+    const long long capP = cap / BP_PARTS;
+    const int part = (int)(blockIdx.x % BP_PARTS);
+    total += (size_t)part * BP_PART_STRIDE;
+    out += (size_t)part * capP;
+    if (i < ncell) bp_cell_pairs(sk32, snode, exclude_shared, g, first, total, out, capP, i, own[wave], buf, lane,
                                  below, cnt);
     // the workgroup's last wave out writes every wave's remaining pairs
     if (lane == 0) wcnt[wave] = cnt;
@@ -367,7 +386,7 @@ __global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__
 #pragma unroll
     for (int w = 0; w < BPW_WAVES; w++)
         for (int q = lane; q < wcnt[w]; q += 64)
-            if (base + pre[w] + q < (unsigned long long)cap) out[base + pre[w] + q] = pbuf[w][q];
+            if (base + pre[w] + q < (unsigned long long)capP) out[base + pre[w] + q] = pbuf[w][q];
 }
 
 // The pair search, second pass: per candidate (own, candidate) position pair of the first
@@ -386,9 +405,9 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
                                                   int *__restrict__ count)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)*ncand < cap ? (long long)*ncand : cap;   // candidates in the buffer
-    if (i >= n) return;
-    const unsigned long long v = io[i];
+    if (!__ballot(bp_slot_used(i, cap, ncand))) return;   // the wave holds no candidate
+    const bool used = bp_slot_used(i, cap, ncand);
+    const unsigned long long v = used ? io[i] : 0ULL;
     const int ja = (int)(v >> 32), jb = (int)(v & 0xffffffffULL);
     float fa[2 * BP_DIRS], fb[2 * BP_DIRS];
 #pragma unroll
@@ -403,8 +422,8 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
         sure = sure & (nextafterf(fa[d], INFINITY) <= nextafterf(fb[BP_DIRS + d], -INFINITY)) &
                (nextafterf(fb[d], INFINITY) <= nextafterf(fa[BP_DIRS + d], -INFINITY));
     }
-    bool hit = sure;
-    if (!sure) {
+    bool hit = used & sure;
+    if (used & !sure) {
         hit = true;
 #pragma unroll
         for (int d = 0; d < BP_DIRS; d++) {
@@ -420,7 +439,8 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
         io[i] = ((unsigned long long)a << 32) | b;
     }
     // the first pass emits the candidates of one own tet together, so a repeats along the wave
-    rank[i] = run_rank(hit, a, count);
+    const int r = run_rank(hit, a, count);
+    if (used) rank[i] = r;
 }
 
 // The pairs grouped by a, in no particular order within a group: pair i goes to
@@ -432,8 +452,7 @@ __global__ __launch_bounds__(256) void k_bp_scatter(const unsigned long long *__
                                                     unsigned long long *__restrict__ grp)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)*ncand < cap ? (long long)*ncand : cap;
-    if (i >= n) return;
+    if (!bp_slot_used(i, cap, ncand)) return;
     const int r = rank[i];
     if (r < 0) return;
     const unsigned long long v = io[i];

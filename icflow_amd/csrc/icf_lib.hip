@@ -2522,7 +2522,7 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     // (12 nT doubles' worth)
     if (int r = grow(ctx, ctx->bp_f, sizeof(double) * ((size_t)60 * nT + 8))) return r;
     if (int r = grow(ctx, ctx->bp_i, sizeof(int) * (8 * (size_t)nT + 4))) return r;
-    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * 2)) return r;
+    if (int r = grow(ctx, ctx->bp_l, sizeof(long long) * BP_PARTS * BP_PART_STRIDE)) return r;
     double *lo = (double *)ctx->bp_f.p, *hi = lo + (size_t)12 * nT;
     auto *bounds = (unsigned long long *)(hi + (size_t)12 * nT);
     double2 *skd = (double2 *)(hi + (size_t)12 * nT + 8);          // 12 double2 = 24 doubles per tet
@@ -2596,7 +2596,8 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
         if (int r = grow(ctx, ctx->bp_g, sizeof(int) * (2 * ((size_t)nT + 1) + (size_t)cap))) return r;
         int *acount = (int *)ctx->bp_g.p, *start = acount + nT + 1, *rnk = start + nT + 1;
         auto *pa = (unsigned long long *)ctx->pat_a.p, *grp = (unsigned long long *)ctx->pat_b.p;
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, (int *)tot, 2, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(2 * BP_PARTS * BP_PART_STRIDE, 256)), dim3(256), 0,
+                           ctx->stream, (int *)tot, 2 * BP_PARTS * BP_PART_STRIDE, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(blocks_for((long long)nT + 1, 256)), dim3(256), 0, ctx->stream, acount,
                            nT + 1, 0);
         hipLaunchKernelGGL(k_bp_cellwave, dim3((ncell + BPW_WAVES - 1) / BPW_WAVES), dim3(64 * BPW_WAVES), 0,
@@ -2609,17 +2610,20 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
         if (max_pairs > 0)
             hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, grp, start, nT,
                                max_pairs, pairs);
-        unsigned long long got = 0;
+        std::vector<unsigned long long> got((size_t)BP_PARTS * BP_PART_STRIDE);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(&got, tot, sizeof(got), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(got.data(), tot, sizeof(unsigned long long) * got.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(&nv, start + nT, sizeof(nv), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        ncand = (long long)got;
-        if (ncand <= cap) break;
+        // the fullest region decides whether the buffer held every candidate
+        ncand = 0;
+        for (int q = 0; q < BP_PARTS; q++) ncand = std::max(ncand, (long long)got[(size_t)q * BP_PART_STRIDE]);
+        if (ncand <= cap / BP_PARTS) break;
         if (pass == 1) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: pair count changed between two searches");
-        cap = ncand + ncand / 4;
+        cap = BP_PARTS * (ncand + ncand / 4);
     }
-    ctx->bp_cap = std::max(ctx->bp_cap, ncand + ncand / 4);
+    ctx->bp_cap = std::max(ctx->bp_cap, BP_PARTS * (ncand + ncand / 4));
     *nOut = (long long)nv;
     if ((long long)nv > max_pairs) return fail(ctx, ICF_E_CAPACITY, "icf_broad_phase: more pairs than max_pairs");
     return 0;
