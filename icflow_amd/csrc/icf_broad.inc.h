@@ -345,7 +345,9 @@ __device__ inline void bp_cell_pairs(const float4 *__restrict__ sk32, const int4
     }
 }
 
-__global__ __launch_bounds__(64 * BPW_WAVES) void k_bp_cellwave(const float4 *__restrict__ sk32,
+// At most 80 registers, for 6 waves per SIMD (88 otherwise: 5 waves; a 12-byte spill; forcing
+// 8 waves spills 100 bytes and doubles the time).
+__global__ __launch_bounds__(64 * BPW_WAVES) __attribute__((amdgpu_waves_per_eu(6))) void k_bp_cellwave(const float4 *__restrict__ sk32,
                                                                 const int4 *__restrict__ snode, int exclude_shared,
                                                                 BpGrid g, const int *__restrict__ first,
                                                                 unsigned long long *__restrict__ total,
