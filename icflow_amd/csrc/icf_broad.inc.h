@@ -45,10 +45,39 @@ __global__ void k_bp_bounds_init(unsigned long long *__restrict__ b)
     if (k <= BP_NBOUNDS) b[k] = (k >= 1 && k <= 3) ? ~0ULL : 0ULL;
 }
 
-// Per surface tet: the 12-direction k-DOP (lo, hi planes) from its 4 nodes' current
-// positions, and the block's contribution to the bounds (one atomic per bound per block).
+// Tet t's 12-direction k-DOP (lo l, hi h) from its 4 nodes' current positions; false if a
+// node id is out of range.  Both kernels below compute it with this code, so their values agree
+// bit for bit.
+__device__ inline bool bp_tet_kdop(const double *__restrict__ dn, int ns, const int *__restrict__ ie, int es, int t,
+                                   int nNodesBound, double l[BP_DIRS], double h[BP_DIRS])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int d = 0; d < BP_DIRS; d++) {
+        l[d] = 1e300;
+        h[d] = -1e300;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int n = ie[t + (size_t)es * v];
+        if (n < 0 || (nNodesBound > 0 && n >= nNodesBound)) return false;
+        const double x = dn[n + (size_t)ns * ND_XC], y = dn[n + (size_t)ns * (ND_XC + 1)],
+                     z = dn[n + (size_t)ns * (ND_XC + 2)];
+        const double p[BP_DIRS] = {x, y, z, x + y, x + z, y + z, x - y, x - z, y - z, x + y - z, x + z - y, y + z - x};
+#pragma unroll
+        for (int d = 0; d < BP_DIRS; d++) {
+            l[d] = fmin(l[d], p[d]);
+            h[d] = fmax(h[d], p[d]);
+        }
+    }
+    return true;
+}
+
+// Per surface tet: the minimum corner of its AABB (the k-DOP's first three lo planes, which
+// bin it into the grid), and the block's contribution to the bounds (one atomic per bound per
+// block).  k_bp_sorted computes the whole k-DOP again rather than reading 24 planes back.
 __global__ __launch_bounds__(256) void k_bp_kdop(const double *__restrict__ dn, int ns, const int *__restrict__ ie,
-                                                 int es, int nT, double *__restrict__ lo, double *__restrict__ hi,
+                                                 int es, int nT, double *__restrict__ lo,
                                                  unsigned long long *__restrict__ bounds, int nNodesBound)
 {
 #pragma clang fp contract(off)
@@ -56,36 +85,11 @@ __global__ __launch_bounds__(256) void k_bp_kdop(const double *__restrict__ dn, 
     double b[BP_NBOUNDS] = {0, 1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
     if (t < nT) {
         double l[BP_DIRS], h[BP_DIRS];
-#pragma unroll
-        for (int d = 0; d < BP_DIRS; d++) {
-            l[d] = 1e300;
-            h[d] = -1e300;
-        }
-        bool ok = true;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int n = ie[t + (size_t)es * v];
-            if (n < 0 || (nNodesBound > 0 && n >= nNodesBound)) {
-                ok = false;
-                break;
-            }
-            const double x = dn[n + (size_t)ns * ND_XC], y = dn[n + (size_t)ns * (ND_XC + 1)],
-                         z = dn[n + (size_t)ns * (ND_XC + 2)];
-            const double p[BP_DIRS] = {x, y, z, x + y, x + z, y + z, x - y, x - z, y - z, x + y - z, x + z - y, y + z - x};
-#pragma unroll
-            for (int d = 0; d < BP_DIRS; d++) {
-                l[d] = fmin(l[d], p[d]);
-                h[d] = fmax(h[d], p[d]);
-            }
-        }
-        if (!ok) {
+        if (!bp_tet_kdop(dn, ns, ie, es, t, nNodesBound, l, h)) {
             atomicMax(&bounds[BP_NBOUNDS], 1ULL);
         } else {
 #pragma unroll
-            for (int d = 0; d < BP_DIRS; d++) {
-                lo[(size_t)d * nT + t] = l[d];
-                hi[(size_t)d * nT + t] = h[d];
-            }
+            for (int d = 0; d < 3; d++) lo[(size_t)d * nT + t] = l[d];
             b[0] = fmax(h[0] - l[0], fmax(h[1] - l[1], h[2] - l[2]));
 #pragma unroll
             for (int k = 0; k < 3; k++) b[1 + k] = b[4 + k] = l[k];
@@ -177,8 +181,8 @@ __global__ __launch_bounds__(256) void k_bp_cell(const double *__restrict__ lo, 
 // node ids (snode), and the k-DOP in FP32 rounded outward, lo down and hi up (sk32: 12 lo then
 // 12 hi): a prefilter that never rejects a pair whose FP64 k-DOPs overlap.  first[L] for
 // L = 0 .. G is then the first position of the cells with keys >= L.
-__global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ lo, const double *__restrict__ hi,
-                                                   int nT, const unsigned *__restrict__ key,
+__global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ dn, int ns, int nNodesBound, int nT,
+                                                   const unsigned *__restrict__ key,
                                                    const int *__restrict__ rank, const int *__restrict__ first,
                                                    const int *__restrict__ ie, int es, int *__restrict__ sidx,
                                                    double2 *__restrict__ skd, int4 *__restrict__ snode,
@@ -188,13 +192,14 @@ __global__ __launch_bounds__(256) void k_bp_sorted(const double *__restrict__ lo
     if (t >= nT) return;
     const int j = first[key[t]] + rank[t];
     sidx[j] = t;
+    double l[BP_DIRS], h[BP_DIRS];
+    (void)bp_tet_kdop(dn, ns, ie, es, t, nNodesBound, l, h);   // node ids checked by k_bp_kdop
     float f[2 * BP_DIRS];
 #pragma unroll
     for (int d = 0; d < BP_DIRS; d++) {
-        const double l = lo[(size_t)d * nT + t], h = hi[(size_t)d * nT + t];
-        skd[(size_t)j * BP_DIRS + d] = make_double2(l, h);
-        f[d] = __double2float_rd(l);
-        f[BP_DIRS + d] = __double2float_ru(h);
+        skd[(size_t)j * BP_DIRS + d] = make_double2(l[d], h[d]);
+        f[d] = __double2float_rd(l[d]);
+        f[BP_DIRS + d] = __double2float_ru(h[d]);
     }
     snode[j] = make_int4(ie[t], ie[t + (size_t)es], ie[t + 2 * (size_t)es], ie[t + 3 * (size_t)es]);
 #pragma unroll
@@ -233,15 +238,14 @@ constexpr int BPW_WAVES = 2, BPW_OWN = 16, BPW_BUF = 256;
 // appends to region w mod BP_PARTS, which spreads the reservations over as many addresses.
 constexpr int BP_PARTS = 8, BP_PART_STRIDE = 64;
 
-// Position i of the candidate buffer: whether it holds a candidate (its region's count is
-// read from the region's counter; positions beyond a region's capacity hold none).
-__device__ inline bool bp_slot_used(long long i, long long cap, const unsigned long long *__restrict__ total)
+// The later passes run a grid row per region (blockIdx.y): this thread's position in the
+// candidate buffer, or -1 beyond the region's count (its counter, capped at the capacity).
+__device__ inline long long bp_slot(long long cap, const unsigned long long *__restrict__ total)
 {
     const long long capP = cap / BP_PARTS;
-    const long long r = i / capP;
-    if (r >= BP_PARTS) return false;
-    const long long n = (long long)total[r * BP_PART_STRIDE];
-    return i - r * capP < (n < capP ? n : capP);
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)total[(size_t)blockIdx.y * BP_PART_STRIDE];
+    return o < (n < capP ? n : capP) ? (long long)blockIdx.y * capP + o : -1;
 }
 
 // One wave's cell of the pair search (k_bp_cellwave): on return, cnt pairs are left in buf.
@@ -406,10 +410,9 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
                                                   unsigned long long *__restrict__ io, int *__restrict__ rank,
                                                   int *__restrict__ count)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!__ballot(bp_slot_used(i, cap, ncand))) return;   // the wave holds no candidate
-    const bool used = bp_slot_used(i, cap, ncand);
-    const unsigned long long v = used ? io[i] : 0ULL;
+    const long long i = bp_slot(cap, ncand);
+    if (i < 0) return;   // the region's used positions come first, so only a wave's top lanes leave
+    const unsigned long long v = io[i];
     const int ja = (int)(v >> 32), jb = (int)(v & 0xffffffffULL);
     float fa[2 * BP_DIRS], fb[2 * BP_DIRS];
 #pragma unroll
@@ -424,8 +427,8 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
         sure = sure & (nextafterf(fa[d], INFINITY) <= nextafterf(fb[BP_DIRS + d], -INFINITY)) &
                (nextafterf(fb[d], INFINITY) <= nextafterf(fa[BP_DIRS + d], -INFINITY));
     }
-    bool hit = used & sure;
-    if (used & !sure) {
+    bool hit = sure;
+    if (!sure) {
         hit = true;
 #pragma unroll
         for (int d = 0; d < BP_DIRS; d++) {
@@ -441,8 +444,7 @@ __global__ __launch_bounds__(256) void k_bp_exact(const double2 *__restrict__ sk
         io[i] = ((unsigned long long)a << 32) | b;
     }
     // the first pass emits the candidates of one own tet together, so a repeats along the wave
-    const int r = run_rank(hit, a, count);
-    if (used) rank[i] = r;
+    rank[i] = run_rank(hit, a, count);
 }
 
 // The pairs grouped by a, in no particular order within a group: pair i goes to
@@ -453,8 +455,8 @@ __global__ __launch_bounds__(256) void k_bp_scatter(const unsigned long long *__
                                                     const int *__restrict__ start,
                                                     unsigned long long *__restrict__ grp)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!bp_slot_used(i, cap, ncand)) return;
+    const long long i = bp_slot(cap, ncand);
+    if (i < 0) return;
     const int r = rank[i];
     if (r < 0) return;
     const unsigned long long v = io[i];

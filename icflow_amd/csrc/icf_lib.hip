@@ -2532,7 +2532,7 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     int4 *snode = (int4 *)(sidx + nT + ((4 - ((3 * nT) & 3)) & 3));   // 4 nT ints, 16-byte aligned
     hipLaunchKernelGGL(k_bp_bounds_init, dim3(1), dim3(64), 0, ctx->stream, bounds);
     hipLaunchKernelGGL(k_bp_kdop, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, dn, nd_stride, ie,
-                       el_all_stride, nT, lo, hi, bounds, nd_stride);
+                       el_all_stride, nT, lo, bounds, nd_stride);
     unsigned long long hb[BP_NBOUNDS + 1];
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, ctx->stream));
@@ -2574,7 +2574,8 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
                        0);
     hipLaunchKernelGGL(k_bp_cell, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, nT, g, key, rank, count);
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b3, count, first, G + 1, ctx->stream));
-    hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, lo, hi, nT, key, rank,
+    hipLaunchKernelGGL(k_bp_sorted, dim3(blocks_for(nT, 256)), dim3(256), 0, ctx->stream, dn, nd_stride, nd_stride,
+                       nT, key, rank,
                        first, ie, el_all_stride, sidx, skd, snode, sk32);
     // candidate pairs per cell (FP32 prefilter), collected in LDS and appended in batches; the
     // exact test; the passing pairs grouped by a (a counting sort) and placed by their rank of b
@@ -2602,10 +2603,11 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
                            nT + 1, 0);
         hipLaunchKernelGGL(k_bp_cellwave, dim3((ncell + BPW_WAVES - 1) / BPW_WAVES), dim3(64 * BPW_WAVES), 0,
                            ctx->stream, sk32, snode, exclude_shared, g, first, tot, pa, cap);
-        hipLaunchKernelGGL(k_bp_exact, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, skd, sk32, sidx, tot,
+        const dim3 gparts(blocks_for(cap / BP_PARTS, 256), BP_PARTS);   // a grid row per region
+        hipLaunchKernelGGL(k_bp_exact, gparts, dim3(256), 0, ctx->stream, skd, sk32, sidx, tot,
                            cap, pa, rnk, acount);
         HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, b5, acount, start, nT + 1, ctx->stream));
-        hipLaunchKernelGGL(k_bp_scatter, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, pa, rnk, tot, cap,
+        hipLaunchKernelGGL(k_bp_scatter, gparts, dim3(256), 0, ctx->stream, pa, rnk, tot, cap,
                            start, grp);
         if (max_pairs > 0)
             hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(cap, 256)), dim3(256), 0, ctx->stream, grp, start, nT,
