@@ -2584,7 +2584,8 @@ int icf_broad_phase(icf_ctx *ctx, const double *dn, int nd_stride, const int *ie
     // the end serves all; a second round only when the candidates outgrew the buffer.
     const int ncell = g.n[0] * g.n[1] * g.n[2];
     auto *tot = (unsigned long long *)ctx->bp_l.p;
-    long long cap = std::max<long long>(ctx->bp_cap, 4LL * nT);
+    // at least 64 places per region, so that every region's grid row has a workgroup
+    long long cap = std::max<long long>({ctx->bp_cap, 4LL * nT, 64LL * BP_PARTS});
     size_t b5 = 0;
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b5, (int *)nullptr, (int *)nullptr, nT + 1, ctx->stream));
     if (int r = grow(ctx, ctx->scan_tmp, b5)) return r;

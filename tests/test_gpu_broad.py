@@ -137,3 +137,39 @@ def test_device_broad_phase_sparse_scene_coarsens_the_grid():
     assert n.value == Q
     assert np.array_equal(d_pairs.download(2 * Q).reshape(-1, 2), host)
     gf.close()
+
+
+@pytest.mark.parametrize("nT", [1, 2, 3])
+def test_device_broad_phase_few_tets(nT):
+    """A chain of one to three overlapping tets (each overlaps the next only) on a fresh
+    context, the smallest candidate buffers: the pairs are the host grid's, none for one tet."""
+    from icflow_amd.gpu_functionality import GPUFunctionality
+    unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.01
+    tets = [unit + np.array([0.004 * t, 0.002 * t, 0.0]) for t in range(nT)]
+    nN = 4 * nT
+    ns, es = stride_of(nN), stride_of(nT)
+    c_dn = np.zeros(18 * ns)
+    for t, x in enumerate(tets):
+        for i in range(4):
+            for k in range(3):
+                c_dn[4 * t + i + ns * (15 + k)] = x[i, k]
+    c_ie = np.zeros(5 * es, np.int32)
+    for t in range(nT):
+        for i in range(4):
+            c_ie[t + es * i] = 4 * t + i
+    gf = GPUFunctionality(0)
+    lib, ctx = gf.lib, gf.ctx
+    d_dn, d_ie = gf.buf("ft_dn", c_dn.size, np.float64), gf.buf("ft_ie", c_ie.size, np.int32)
+    d_dn.upload(c_dn)
+    d_ie.upload(c_ie)
+    host = broad_phase(gf, c_dn, ns, c_ie, es, nT, exclude_shared=True)
+    Q = len(host)
+    assert Q == nT - 1
+    d_pairs = gf.buf("ft_pairs", 2 * max(Q, 1), np.int32)
+    n = ctypes.c_longlong()
+    _lib.check(lib.icf_broad_phase(ctx, d_dn.p, ns, d_ie.p, es, nT, 1, d_pairs.p, max(Q, 1), ctypes.byref(n)), ctx,
+               "broad_phase")
+    assert n.value == Q
+    if Q:
+        assert np.array_equal(d_pairs.download(2 * Q).reshape(-1, 2), host)
+    gf.close()
