@@ -556,6 +556,7 @@ __global__ __launch_bounds__(256) void k_nl2_write(const unsigned long long *__r
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n || !keep[o]) return;
     const int k = dst[o];
+    if (k >= ts) return;   // beyond the output's capacity: the caller reports the count
     nl2[k] = (int)(key[o] >> 32);
     nl2[k + (size_t)ts] = (int)(key[o] & 0xffffffffULL);
 }

@@ -2678,15 +2678,15 @@ int icf_nl2set(icf_ctx *ctx, const int *masks, const int *pairs, int nPairs, con
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, s1, keep, dst, total + 1, ctx->stream));
     if (int r = grow(ctx, ctx->scan_tmp, s1)) return r;
     HIPCHK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, s1, keep, dst, total + 1, ctx->stream));
+    // the tuples that fit are written before the count is read, so one wait serves both
+    hipLaunchKernelGGL(k_nl2_write, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, key, total, keep, dst,
+                       nl2, tet_stride);
     int num = 0;
+    HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(&num, dst + total, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     *nOut = num;
     if (num > tet_stride) return fail(ctx, ICF_E_CAPACITY, "icf_nl2set: more tuples than tet_stride");
-    hipLaunchKernelGGL(k_nl2_write, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, key, total, keep, dst,
-                       nl2, tet_stride);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
